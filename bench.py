@@ -1,0 +1,170 @@
+"""Throughput of the fvs2d residual / time-step path on one MI355X.
+
+Workload (default): the isentropic vortex on the periodic jittered square of
+1024 x 1024 quads split into 2,097,152 triangles, FP64, Roe flux, unlimited
+node-neighbour LSQ gradients, upwind-2nd reconstruction, SSP-RK3 at CFL 0.5
+(dt = 0.5 * min_i V_i / (0.5 ws_i) on the initial state).  A "step" is one
+SSP-RK3 time step (three residual evaluations and stage updates) over the whole
+mesh; value = cells * stages * steps / seconds (cell-stage updates/s).
+
+Prints ONE JSON line.  Extra fields: the per-kernel breakdown, the roofline
+object of the dominant kernel (algorithmic bytes per launch / mean HIP-event
+duration of that kernel on its stream), the other gradient / limiter variants
+of the same mesh, and the CPU baseline (the C oracle, single thread, on a few
+steps of the same mesh).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def model_bytes_per_cell_stage(grad_method: int, limiter: int) -> float:
+    """Compulsory-traffic model per cell-stage for SSP-RK3 on triangle meshes
+    (S 96 + G 160 + M 108 + stencil + L): LSQ-nn 604 B, GGCB 440 B, GGNB 508 B,
+    +40 B with a limiter on LSQ, +88 B on GG (node-neighbour ids)."""
+    stencil = {3: 240.0, 1: 76.0, 2: 144.0}[grad_method]
+    lim = 0.0 if limiter == 0 else (40.0 if grad_method == 3 else 88.0)
+    return 96.0 + 160.0 + 108.0 + stencil + lim
+
+
+def build_case(n: int, periodic: bool = True):
+    from fvs2d_amd import meshgen
+    return meshgen.jittered_square(n, periodic=periodic, seed=0)
+
+
+def scheme_for(grad_method=3, limiter=0, dt=0.01):
+    from fvs2d_amd.solver import Scheme
+    return Scheme(grad_method=grad_method, lsq_nn=1, lsq_p=0.0, limiter=limiter, recon=2, gamma=1.4, mach=0.5,
+                  lvortex=1, vortex_x0=0.0, vortex_y0=0.0, vortex_K=5.0, vortex_inf=(1.0, 1.0, 1.0, 1.0),
+                  rk_scheme=2, dt=dt, cfl=0.5, reorder=0)
+
+
+def timed_run(solver, steps: int, warmup: int):
+    solver.initialize_solution(1, 0.0)
+    if warmup:
+        solver.time_integration(0.0, warmup)
+    t0 = time.perf_counter()
+    solver.time_integration(0.0, steps)  # synchronises the stream before returning
+    wall = time.perf_counter() - t0
+    return wall, solver.last_elapsed_ms / 1e3
+
+
+def cpu_baseline(mesh, dt: float, steps: int):
+    from oracle.oracle import Oracle
+    sch = scheme_for(dt=dt)
+    orc = Oracle(mesh, **sch.oracle_kwargs())
+    orc.initialize(1, 0.0)
+    t0 = time.perf_counter()
+    orc.time_integration(0.0, steps)
+    sec = time.perf_counter() - t0
+    return mesh.ncells * 3 * steps / sec, sec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1024, help="quads per side (1024 = 2,097,152 triangles)")
+    ap.add_argument("--variants", type=int, default=1, help="also time GGCB/GGNB x limiter on/off")
+    ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--profile-steps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the multi-GPU "
+              "decomposition is a design, not an implementation", file=sys.stderr)
+        sys.exit(2)
+
+    from fvs2d_amd.solver import Solver
+
+    t_setup = time.perf_counter()
+    mesh = build_case(args.n)
+    N = mesh.ncells
+    # dt from the reference's local-step formula on the initial state
+    s0 = Solver(mesh, scheme_for(dt=1.0))
+    s0.initialize_solution(1, 0.0)
+    dt = float(np.min(s0.compute_local_time()))
+    s0.set_timestep(dt)
+    setup_s = time.perf_counter() - t_setup
+    print(f"[bench] mesh {N} cells, dt={dt:.6e}, setup {setup_s:.1f}s", file=sys.stderr, flush=True)
+
+    wall, dev = timed_run(s0, args.steps, args.warmup)
+    stages = 3
+    value = N * stages * args.steps / wall
+    print(f"[bench] LSQ-nn: {args.steps} steps in {wall:.3f}s wall ({dev:.3f}s events) -> {value / 1e9:.3f} G "
+          "cell-stage/s", file=sys.stderr, flush=True)
+
+    # per-kernel breakdown with HIP events on the solver's stream
+    kt = s0.kernel_times(0.0, args.profile_steps)
+    per_step = {k: v[0] * v[2] / args.profile_steps for k, v in kt.items()}
+    dom = max(per_step, key=per_step.get)
+    ms, by, _ = kt[dom]
+    achieved = by / (ms * 1e-3) / 1e9
+    kernels = {k: {"ms_per_launch": round(v[0], 5), "launches_per_step": v[2] / args.profile_steps,
+                   "alg_bytes_per_launch": int(v[1]),
+                   "GBps": round(v[1] / (v[0] * 1e-3) / 1e9, 1) if v[0] > 0 else None} for k, v in kt.items()}
+    print(f"[bench] kernels: {json.dumps(kernels)}", file=sys.stderr, flush=True)
+    model_b = model_bytes_per_cell_stage(3, 0)
+
+    variants = {}
+    if args.variants:
+        for gm, name in ((1, "ggcb"), (2, "ggnb"), (3, "lsq_nn")):
+            for lim in (0, 1):
+                if gm == 3 and lim == 0:
+                    variants["lsq_nn"] = round(value, 1)
+                    continue
+                sv = Solver(mesh, scheme_for(grad_method=gm, limiter=lim, dt=dt))
+                w, _ = timed_run(sv, max(5, args.steps // 4), 1)
+                key = name + ("_venk" if lim else "")
+                variants[key] = round(N * stages * max(5, args.steps // 4) / w, 1)
+                sv.close()
+                print(f"[bench] {key}: {variants[key] / 1e9:.3f} G cell-stage/s", file=sys.stderr, flush=True)
+
+    cpu = None
+    if args.cpu_steps > 0:
+        rate, sec = cpu_baseline(mesh, dt, args.cpu_steps)
+        cpu = {"value": round(rate, 1), "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
+               "sample": f"C oracle (gcc -O2, serial), {args.cpu_steps} SSP-RK3 steps of the same "
+                         f"{N}-cell mesh and scheme, {sec:.1f} s"}
+        print(f"[bench] cpu oracle: {rate / 1e6:.3f} M cell-stage/s ({sec:.1f}s)", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "cell-stage updates/s (FP64)",
+        "value": round(value, 1),
+        "unit": "cell-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"isentropic vortex, periodic jittered square {args.n}x{args.n} quads -> {N} "
+                               "triangles, Roe + unlimited LSQ-nn gradients, upwind-2nd, SSP-RK3, CFL 0.5",
+                   "cells": N, "stages_per_step": stages, "dt": dt},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom},
+        "model_bytes_per_cell_stage": model_b,
+        "effective_GBps_vs_model": round(value * model_b / 1e9, 1),
+        "kernels": kernels,
+        "variants_cell_stage_per_s": variants,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

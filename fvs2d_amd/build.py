@@ -1,0 +1,57 @@
+"""Build libfvs2d_hip.so in place (hipcc for gfx950, g++ for host C++).
+
+Run as ``python -m fvs2d_amd.build``.  The library lands next to this file so
+it travels with the source tree.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libfvs2d_hip.so")
+BUILD = os.path.join(HERE, "_build")
+
+ARCH = os.environ.get("FVS_OFFLOAD_ARCH", "gfx950")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+# packed-FP32 instructions are kept off: all arithmetic here is FP64 anyway
+HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result"]
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-variable"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    obj_host = os.path.join(BUILD, "mesh_prep.o")
+    obj_dev = os.path.join(BUILD, "fvs_api.o")
+    cmds = [
+        ["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "mesh_prep.cpp"), "-o", obj_host],
+        [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "fvs_api.hip"), "-o", obj_dev],
+    ]
+    for c in cmds:
+        if verbose:
+            print(" ".join(c), flush=True)
+        _run(c)
+    tmp = LIB + ".tmp"
+    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", obj_dev, obj_host, "-o", tmp]
+    if verbose:
+        print(" ".join(link), flush=True)
+    _run(link)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

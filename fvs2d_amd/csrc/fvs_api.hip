@@ -1,0 +1,1485 @@
+// MI355X (gfx950) implementation of the C-ABI in include/fvs2d.h.
+//
+// One residual evaluation is three kinds of launch:
+//   k_grad_*   cell-parallel: gradient of the primitive state (LSQ, GGCB or
+//              GGNB) and, fused into the same pass over the stencil, the
+//              min/max and face extrapolation of the limiter;
+//   k_flux_*   face-parallel: MUSCL/UMUSCL reconstruction + Roe flux per
+//              face, written once as flux*area (no float atomics);
+//   k_update   cell-parallel: gathers its faces' fluxes in the reference's
+//              per-cell order (a fixed gather table), forms -R/vol and applies
+//              the Runge-Kutta stage update, the local time step, the
+//              primitive state of the new stage and the residual-norm partial
+//              sums in the same pass.
+// Reductions are two-level and in a fixed order, so results are bitwise
+// reproducible run to run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fvs2d.h"
+#include "kernels.hpp"
+#include "mesh_prep.hpp"
+
+using namespace fvs;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPC(x)                                                                           \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) return fail(FVS_EHIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kBlock = 256;
+constexpr int kSlice = 64;  // one wavefront
+
+inline int nblk(long n, int b = kBlock) { return int((n + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void ld4(const double *__restrict__ p, long i, double v[4]) {
+  const double2 a = reinterpret_cast<const double2 *>(p)[2 * i];
+  const double2 b = reinterpret_cast<const double2 *>(p)[2 * i + 1];
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+__device__ __forceinline__ void st4(double *__restrict__ p, long i, const double v[4]) {
+  reinterpret_cast<double2 *>(p)[2 * i] = make_double2(v[0], v[1]);
+  reinterpret_cast<double2 *>(p)[2 * i + 1] = make_double2(v[2], v[3]);
+}
+__device__ __forceinline__ void ld8(const double *__restrict__ p, long i, double v[8]) {
+  const double2 *q = reinterpret_cast<const double2 *>(p) + 4 * i;
+  double2 a = q[0], b = q[1], c = q[2], d = q[3];
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y; v[6] = d.x; v[7] = d.y;
+}
+__device__ __forceinline__ void st8(double *__restrict__ p, long i, const double v[8]) {
+  double2 *q = reinterpret_cast<double2 *>(p) + 4 * i;
+  q[0] = make_double2(v[0], v[1]); q[1] = make_double2(v[2], v[3]);
+  q[2] = make_double2(v[4], v[5]); q[3] = make_double2(v[6], v[7]);
+}
+
+__global__ void k_prim(int N, double g, const double *__restrict__ U, double *__restrict__ W) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double u[4], w[4];
+  ld4(U, i, u);
+  cons2prim(u, g, w);
+  st4(W, i, w);
+}
+
+// Limiter of one cell from its stencil min/max and its gradient
+// (src/gradient_limiter.f90:47-91): phi = min over variables of
+// min(1, min over faces of L(dmax|dmin, q_f - q)).
+__device__ __forceinline__ double cell_limiter(int lt, int nf, int N, int i, const double2 *__restrict__ off,
+                                               double vol, const double w[4], const double mn[4],
+                                               const double mx[4], const double gx[4], const double gy[4]) {
+  double2 o[4];
+  for (int k = 0; k < 4; ++k) o[k] = (k < nf) ? off[long(k) * N + i] : make_double2(0.0, 0.0);
+  double phi = 0.0;
+  for (int v = 0; v < 4; ++v) {
+    const double q = w[v], dmax = mx[v] - q, dmin = mn[v] - q;
+    double pe = 100.0;
+    for (int k = 0; k < nf; ++k) {
+      const double pf = q + o[k].x * gx[v] + o[k].y * gy[v];
+      const double diff = pf - q;
+      double e = 1.0;
+      if (diff > 0.0) e = limiter_fn(lt, dmax, diff, vol);
+      else if (diff < 0.0) e = limiter_fn(lt, dmin, diff, vol);
+      pe = (e < pe) ? e : pe;
+    }
+    const double pv = (1.0 < pe) ? 1.0 : pe;
+    phi = (v == 0 || pv < phi) ? pv : phi;
+  }
+  return phi;
+}
+
+// LSQ gradient (src/gradient_lsq.f90:371-419): grad_i = sum_j cw_j (q_j - q_i),
+// cw = G^-1 D^T w per stencil entry; LIM fuses the limiter.
+template <int LIM>
+__global__ __launch_bounds__(kBlock) void k_grad_lsq(int N, const double *__restrict__ W, const int *__restrict__ soff,
+                                                     const int *__restrict__ swid, const int *__restrict__ snb,
+                                                     const double2 *__restrict__ scw, double *__restrict__ grad,
+                                                     double *__restrict__ phi, int lt, const int *__restrict__ nvrt,
+                                                     const double2 *__restrict__ loff, const double *__restrict__ vol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int s = i / kSlice, l = i % kSlice;
+  const int base = soff[s], wid = swid[s];
+  double w[4], gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 0, 0}, mn[4], mx[4];
+  ld4(W, i, w);
+  for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
+  for (int k = 0; k < wid; ++k) {
+    const long e = base + long(k) * kSlice + l;
+    const int j = snb[e];
+    const double2 c = scw[e];
+    double q[4];
+    ld4(W, j, q);
+    for (int v = 0; v < 4; ++v) {
+      const double d = q[v] - w[v];
+      gx[v] = gx[v] + c.x * d;
+      gy[v] = gy[v] + c.y * d;
+      if (LIM) {
+        mn[v] = (q[v] < mn[v]) ? q[v] : mn[v];
+        mx[v] = (q[v] > mx[v]) ? q[v] : mx[v];
+      }
+    }
+  }
+  double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
+  st8(grad, i, g8);
+  if (LIM) phi[i] = cell_limiter(lt, nvrt[i], N, i, loff, vol[i], w, mn, mx, gx, gy);
+}
+
+// stencil min/max over the limiter's (node-neighbour) stencil, for GG gradients
+__device__ __forceinline__ void stencil_minmax(int i, const int *__restrict__ soff, const int *__restrict__ swid,
+                                               const int *__restrict__ snb, const double *__restrict__ W,
+                                               const double w[4], double mn[4], double mx[4]) {
+  const int s = i / kSlice, l = i % kSlice;
+  for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
+  for (int k = 0; k < swid[s]; ++k) {
+    const int j = snb[soff[s] + long(k) * kSlice + l];
+    double q[4];
+    ld4(W, j, q);
+    for (int v = 0; v < 4; ++v) {
+      mn[v] = (q[v] < mn[v]) ? q[v] : mn[v];
+      mx[v] = (q[v] > mx[v]) ? q[v] : mx[v];
+    }
+  }
+}
+
+// GGCB gradient (src/gradient_ggcb.f90:116-138):
+// grad_i = (coef0 q_i + sum_f coefnb_f q_nb(f)) / vol
+template <int LIM>
+__global__ __launch_bounds__(kBlock) void k_grad_ggcb(int N, int nfw, const double *__restrict__ W,
+                                                      const double2 *__restrict__ c0, const int *__restrict__ nb,
+                                                      const double2 *__restrict__ cn, const double *__restrict__ vol,
+                                                      double *__restrict__ grad, double *__restrict__ phi, int lt,
+                                                      const int *__restrict__ nvrt, const double2 *__restrict__ loff,
+                                                      const int *__restrict__ soff, const int *__restrict__ swid,
+                                                      const int *__restrict__ snb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double w[4], gx[4], gy[4];
+  ld4(W, i, w);
+  const double2 a = c0[i];
+  for (int v = 0; v < 4; ++v) gx[v] = a.x * w[v], gy[v] = a.y * w[v];
+  for (int k = 0; k < nfw; ++k) {
+    const int j = nb[long(k) * N + i];
+    const double2 c = cn[long(k) * N + i];
+    double q[4];
+    ld4(W, j, q);
+    for (int v = 0; v < 4; ++v) {
+      gx[v] = gx[v] + c.x * q[v];
+      gy[v] = gy[v] + c.y * q[v];
+    }
+  }
+  const double V = vol[i];
+  for (int v = 0; v < 4; ++v) gx[v] = gx[v] / V, gy[v] = gy[v] / V;
+  double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
+  st8(grad, i, g8);
+  if (LIM) {
+    double mn[4], mx[4];
+    stencil_minmax(i, soff, swid, snb, W, w, mn, mx);
+    phi[i] = cell_limiter(lt, nvrt[i], N, i, loff, V, w, mn, mx, gx, gy);
+  }
+}
+
+// GGNB gradient (src/gradient_ggnb.f90:351-378):
+// grad_i = (coef0 q_i + sum_incidences (coefedg*coefnb) q_j) / vol
+template <int LIM>
+__global__ __launch_bounds__(kBlock) void k_grad_ggnb(int N, const double *__restrict__ W,
+                                                      const double2 *__restrict__ c0, const int *__restrict__ goff,
+                                                      const int *__restrict__ gwid, const int *__restrict__ gnb,
+                                                      const double2 *__restrict__ gc, const double *__restrict__ vol,
+                                                      double *__restrict__ grad, double *__restrict__ phi, int lt,
+                                                      const int *__restrict__ nvrt, const double2 *__restrict__ loff,
+                                                      const int *__restrict__ soff, const int *__restrict__ swid,
+                                                      const int *__restrict__ snb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int s = i / kSlice, l = i % kSlice;
+  double w[4], gx[4], gy[4];
+  ld4(W, i, w);
+  const double2 a = c0[i];
+  for (int v = 0; v < 4; ++v) gx[v] = a.x * w[v], gy[v] = a.y * w[v];
+  const int base = goff[s], wid = gwid[s];
+  for (int k = 0; k < wid; ++k) {
+    const long e = base + long(k) * kSlice + l;
+    const int j = gnb[e];
+    const double2 c = gc[e];
+    double q[4];
+    ld4(W, j, q);
+    for (int v = 0; v < 4; ++v) {
+      gx[v] = gx[v] + c.x * q[v];
+      gy[v] = gy[v] + c.y * q[v];
+    }
+  }
+  const double V = vol[i];
+  for (int v = 0; v < 4; ++v) gx[v] = gx[v] / V, gy[v] = gy[v] / V;
+  double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
+  st8(grad, i, g8);
+  if (LIM) {
+    double mn[4], mx[4];
+    stencil_minmax(i, soff, swid, snb, W, w, mn, mx);
+    phi[i] = cell_limiter(lt, nvrt[i], N, i, loff, V, w, mn, mx, gx, gy);
+  }
+}
+
+// interior faces: reconstruction (src/residual.f90:87-92) + Roe (95)
+__global__ __launch_bounds__(kBlock) void k_flux_int(int Fi, Phys P, const int2 *__restrict__ fc,
+                                                     const double *__restrict__ geo, const double *__restrict__ W,
+                                                     const double *__restrict__ grad, const double *__restrict__ phi,
+                                                     double *__restrict__ FA, double *__restrict__ WSA) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= Fi) return;
+  const int2 c = fc[f];
+  double g[8];
+  ld8(geo, f, g);
+  double wL[4], wR[4], gL[8], gR[8];
+  ld4(W, c.x, wL);
+  ld4(W, c.y, wR);
+  ld8(grad, c.x, gL);
+  ld8(grad, c.y, gR);
+  const double phL = phi[c.x], phR = phi[c.y];
+  const double k2 = P.kappa / 2.0, k1 = 1.0 - P.kappa;
+  double pL[4], pR[4];
+  for (int v = 0; v < 4; ++v) {
+    const double gC = wR[v] - wL[v];
+    const double dL = g[3] * gL[v] + g[4] * gL[4 + v];
+    const double dR = g[5] * gR[v] + g[6] * gR[4 + v];
+    pL[v] = wL[v] + phL * (k2 * gC + k1 * dL);
+    pR[v] = wR[v] + phR * (-(k2 * gC) + k1 * dR);
+  }
+  double fl[4], ws;
+  roe_flux(P.gamma, pL, pR, g[0], g[1], fl, ws);
+  const double A = g[2];
+  double fa[4] = {fl[0] * A, fl[1] * A, fl[2] * A, fl[3] * A};
+  st4(FA, f, fa);
+  WSA[f] = ws * A;
+}
+
+// boundary faces: extrapolation (src/residual.f90:130), ghost state
+// (bc_flux, 183-255), Roe (142)
+__global__ __launch_bounds__(kBlock) void k_flux_bnd(int Fb, int Fi, Phys P, double t, const int *__restrict__ bcell,
+                                                     const int *__restrict__ btype, const double *__restrict__ geo,
+                                                     const double *__restrict__ W, const double *__restrict__ grad,
+                                                     const double *__restrict__ phi, double *__restrict__ FA,
+                                                     double *__restrict__ WSA) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= Fb) return;
+  const int c = bcell[b];
+  double g[8], w[4], gr[8];
+  ld8(geo, b, g);
+  ld4(W, c, w);
+  ld8(grad, c, gr);
+  const double ph = phi[c];
+  double pL[4], pR[4];
+  for (int v = 0; v < 4; ++v) pL[v] = w[v] + ph * (g[3] * gr[v] + g[4] * gr[4 + v]);
+  const int ty = btype[b];
+  const double nx = g[0], ny = g[1];
+  if (ty == FVS_BC_FREESTREAM) {
+    for (int v = 0; v < 4; ++v) pR[v] = P.pinf[v];
+  } else if (ty == FVS_BC_SLIP_WALL) {
+    const double un = pL[1] * nx + pL[2] * ny;
+    pR[0] = pL[0];
+    pR[1] = pL[1] - 2.0 * un * nx;
+    pR[2] = pL[2] - 2.0 * un * ny;
+    pR[3] = pL[3];
+  } else if (P.lvortex) {
+    vortex_state(P, t, g[5], g[6], pR);
+  } else {
+    mms_state(g[5], g[6], pR);
+  }
+  double fl[4], ws;
+  roe_flux(P.gamma, pL, pR, nx, ny, fl, ws);
+  const double A = g[2];
+  double fa[4] = {fl[0] * A, fl[1] * A, fl[2] * A, fl[3] * A};
+  st4(FA, Fi + b, fa);
+  WSA[Fi + b] = ws * A;
+}
+
+enum { M_RESID = 0, M_SSP3 = 1, M_SSP42 = 2, M_CLASSIC = 3 };
+
+struct Upd {
+  int mode, steady, first, last;
+  double a, b;   // SSP3: U = a*U0 + b*(U + h R)
+  double h;      // unsteady step factor, or the stage constant when steady
+  double coef;   // rk coefficient (4-stage schemes)
+  int use_acc;   // CLASSIC final stage: update with the accumulator
+  double cfl, gamma;
+};
+
+__device__ __forceinline__ void block_sum4(double v[4], double *red) {
+  // fixed-shape tree over the block: deterministic
+  const int t = threadIdx.x;
+  for (int k = 0; k < 4; ++k) red[k * kBlock + t] = v[k];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (t < s)
+      for (int k = 0; k < 4; ++k) red[k * kBlock + t] = red[k * kBlock + t] + red[k * kBlock + t + s];
+    __syncthreads();
+  }
+  for (int k = 0; k < 4; ++k) v[k] = red[k * kBlock];
+}
+
+// gather (src/residual.f90:97-101,153-155 in the reference's per-cell order),
+// -R/vol (164-166), local time step (src/runge_kutta.f90:433) and stage update
+__global__ __launch_bounds__(kBlock) void k_update(int N, int gw, Upd A, const int *__restrict__ gat,
+                                                   const double *__restrict__ FA, const double *__restrict__ WSA,
+                                                   const double *__restrict__ vol, double *__restrict__ Ua,
+                                                   double *__restrict__ Ub, double *__restrict__ Facc,
+                                                   double *__restrict__ W, double *__restrict__ dtl,
+                                                   double *__restrict__ rout, double *__restrict__ wsout,
+                                                   double *__restrict__ part) {
+  __shared__ double red[4 * kBlock];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2[4] = {0, 0, 0, 0};
+  if (i < N) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0}, wacc = 0.0;
+    for (int k = 0; k < gw; ++k) {
+      const int s = gat[long(k) * N + i];
+      if (s == 0) break;
+      const int slot = (s > 0 ? s : -s) - 1;
+      double fa[4];
+      ld4(FA, slot, fa);
+      if (s > 0)
+        for (int v = 0; v < 4; ++v) acc[v] = acc[v] + fa[v];
+      else
+        for (int v = 0; v < 4; ++v) acc[v] = acc[v] - fa[v];
+      wacc = wacc + WSA[slot];
+    }
+    const double V = vol[i];
+    double R[4];
+    for (int v = 0; v < 4; ++v) R[v] = -acc[v] / V;
+    if (A.mode == M_RESID) {
+      st4(rout, i, R);
+      wsout[i] = wacc;
+    } else {
+      double hs = A.h;
+      if (A.steady) {
+        double dl;
+        if (A.first) {
+          dl = A.cfl * V / (0.5 * wacc);
+          dtl[i] = dl;
+        } else {
+          dl = dtl[i];
+        }
+        hs = dl * A.h;
+      }
+      double u0[4], un[4];
+      ld4(Ua, i, u0);
+      if (A.mode == M_SSP3) {
+        double uc[4];
+        if (A.first) for (int v = 0; v < 4; ++v) uc[v] = u0[v];
+        else ld4(Ub, i, uc);
+        for (int v = 0; v < 4; ++v) un[v] = A.a * u0[v] + A.b * (uc[v] + hs * R[v]);
+      } else if (A.mode == M_SSP42) {
+        double fo[4] = {0, 0, 0, 0};
+        if (!A.first) ld4(Facc, i, fo);
+        for (int v = 0; v < 4; ++v) un[v] = u0[v] + hs * (A.coef * R[v] + fo[v]);
+        if (!A.last) {
+          double fn[4];
+          for (int v = 0; v < 4; ++v) fn[v] = fo[v] + R[v];
+          st4(Facc, i, fn);
+        }
+      } else {
+        double fo[4] = {0, 0, 0, 0}, fn[4];
+        if (!A.first) ld4(Facc, i, fo);
+        for (int v = 0; v < 4; ++v) fn[v] = fo[v] + A.coef * R[v];
+        if (A.use_acc) for (int v = 0; v < 4; ++v) un[v] = u0[v] + hs * fn[v];
+        else for (int v = 0; v < 4; ++v) un[v] = u0[v] + hs * R[v];
+        if (!A.last) st4(Facc, i, fn);
+      }
+      if (A.last) {
+        st4(Ua, i, un);
+        for (int v = 0; v < 4; ++v) {
+          const double d = fabs(un[v] - u0[v]);
+          d2[v] = d * d;
+        }
+      } else if (A.mode == M_SSP3) {
+        st4(Ub, i, un);
+      }
+      double w[4];
+      cons2prim(un, A.gamma, w);
+      st4(W, i, w);
+    }
+  }
+  if (A.mode != M_RESID && A.last) {
+    block_sum4(d2, red);
+    if (threadIdx.x == 0)
+      for (int v = 0; v < 4; ++v) part[4 * blockIdx.x + v] = d2[v];
+  }
+}
+
+// residual norms of one step: sqrt(sum_i (U-U0)^2 / N) per variable
+// (src/runge_kutta.f90:169-184), partials summed in block order
+__global__ void k_norm_final(int nparts, int N, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double red[4 * kBlock];
+  double s[4] = {0, 0, 0, 0};
+  for (int p = threadIdx.x; p < nparts; p += kBlock)
+    for (int v = 0; v < 4; ++v) s[v] = s[v] + part[4 * p + v];
+  block_sum4(s, red);
+  if (threadIdx.x == 0)
+    for (int v = 0; v < 4; ++v) out[v] = sqrt(s[v] / double(N));
+}
+
+// isentropic-vortex error over interior cells (src/mms.f90:315-361)
+__global__ __launch_bounds__(kBlock) void k_vortex_err(int n, Phys P, double t, const int *__restrict__ cells,
+                                                       const double *__restrict__ cx, const double *__restrict__ cy,
+                                                       const double *__restrict__ U, double *__restrict__ part) {
+  __shared__ double red[4 * kBlock];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double d[4] = {0, 0, 0, 0};
+  if (k < n) {
+    const int i = cells[k];
+    double pv[4], ex[4], u[4];
+    vortex_state(P, t, cx[i], cy[i], pv);
+    ex[0] = pv[0];
+    ex[1] = pv[0] * pv[1];
+    ex[2] = pv[0] * pv[2];
+    ex[3] = pv[3] / (P.gamma - 1.0) + 0.5 * ex[0] * (pv[1] * pv[1] + pv[2] * pv[2]);
+    ld4(U, i, u);
+    for (int v = 0; v < 4; ++v) d[v] = fabs(u[v] - ex[v]);
+  }
+  // max
+  double m[4] = {d[0], d[1], d[2], d[3]};
+  {
+    const int t2 = threadIdx.x;
+    for (int q = 0; q < 4; ++q) red[q * kBlock + t2] = m[q];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+      if (t2 < s)
+        for (int q = 0; q < 4; ++q) red[q * kBlock + t2] = fmax(red[q * kBlock + t2], red[q * kBlock + t2 + s]);
+      __syncthreads();
+    }
+    for (int q = 0; q < 4; ++q) m[q] = red[q * kBlock];
+    __syncthreads();
+  }
+  double l1[4] = {d[0], d[1], d[2], d[3]};
+  block_sum4(l1, red);
+  __syncthreads();
+  double l2[4] = {d[0] * d[0], d[1] * d[1], d[2] * d[2], d[3] * d[3]};
+  block_sum4(l2, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 4; ++q) {
+      part[12 * blockIdx.x + q] = m[q];
+      part[12 * blockIdx.x + 4 + q] = l1[q];
+      part[12 * blockIdx.x + 8 + q] = l2[q];
+    }
+}
+
+__global__ void k_vortex_final(int nparts, int n, const double *__restrict__ part, double *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double m[4] = {0, 0, 0, 0}, l1[4] = {0, 0, 0, 0}, l2[4] = {0, 0, 0, 0};
+  for (int p = 0; p < nparts; ++p)
+    for (int q = 0; q < 4; ++q) {
+      m[q] = fmax(m[q], part[12 * p + q]);
+      l1[q] = l1[q] + part[12 * p + 4 + q];
+      l2[q] = l2[q] + part[12 * p + 8 + q];
+    }
+  const double dn = double(n);
+  double tot = 0.0;
+  for (int q = 0; q < 4; ++q) {
+    out[3 * q] = m[q];
+    out[3 * q + 1] = l1[q] / dn;
+    out[3 * q + 2] = sqrt(l2[q] / dn);
+    tot = tot + l2[q];
+  }
+  out[12] = sqrt(tot / dn);
+}
+
+// C_L / C_D over wall faces (src/io.f90:401-442), one thread in face order
+__global__ void k_wall_forces(int nw, const int *__restrict__ wf, const int *__restrict__ bcell,
+                              const double *__restrict__ geo, const double *__restrict__ W,
+                              const double *__restrict__ grad, double mach, double pinf, double ca_, double sa_,
+                              double *__restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double cn = 0, ca = 0, cn1 = 0, ca1 = 0;
+  for (int k = 0; k < nw; ++k) {
+    const int b = wf[k], c = bcell[b];
+    double g[8], gr[8], w[4];
+    ld8(geo, b, g);
+    ld8(grad, c, gr);
+    ld4(W, c, w);
+    const double pw = w[3] + g[3] * gr[3] + g[4] * gr[7];
+    const double cp = 2.0 / (mach * mach) * (pw - pinf);
+    const double cp1 = 2.0 / (mach * mach) * (w[3] - pinf);
+    cn = cn + cp * g[1] * g[2];
+    ca = ca + cp * g[0] * g[2];
+    cn1 = cn1 + cp1 * g[1] * g[2];
+    ca1 = ca1 + cp1 * g[0] * g[2];
+  }
+  out[0] = cn * ca_ - ca * sa_;
+  out[1] = cn * sa_ + ca * ca_;
+  out[2] = cn1 * ca_ - ca1 * sa_;
+  out[3] = cn1 * sa_ + ca1 * ca_;
+}
+
+__global__ void k_roe_batch(int n, double g, const double *__restrict__ pL, const double *__restrict__ pR,
+                            const double *__restrict__ nxy, double *__restrict__ F, double *__restrict__ ws) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a[4], b[4], f[4], w;
+  ld4(pL, i, a);
+  ld4(pR, i, b);
+  roe_flux(g, a, b, nxy[2 * i], nxy[2 * i + 1], f, w);
+  st4(F, i, f);
+  ws[i] = w;
+}
+
+__global__ void k_fill(long n, double v, double *__restrict__ p) {
+  const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace
+
+struct fvs_mesh {
+  MeshPrep mp;
+};
+
+struct fvs_ctx {
+  fvs_options opt{};
+  fvs_mesh mesh;
+  MeshPrep &mp = mesh.mp;
+  Phys P{};
+  int N = 0, Fi = 0, Fb = 0, GW = 0, NFW = 3;
+  std::vector<int> perm, iperm;  // perm[new] = old
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  // device arrays
+  double *U0 = nullptr, *Ub = nullptr, *Facc = nullptr, *W = nullptr, *grad = nullptr, *phi = nullptr,
+         *dtl = nullptr, *FA = nullptr, *WSA = nullptr, *vol = nullptr, *cx = nullptr, *cy = nullptr;
+  double *rout = nullptr, *wsout = nullptr, *part = nullptr, *small = nullptr;
+  int2 *fi_cells = nullptr;
+  double *fi_geo = nullptr;
+  int *fb_cell = nullptr, *fb_type = nullptr;
+  double *fb_geo = nullptr;
+  int *gat = nullptr;
+  int *nvrt = nullptr;
+  double2 *loff = nullptr;
+  int *soff = nullptr, *swid = nullptr, *snb = nullptr;
+  double2 *scw = nullptr;
+  double2 *ggcb0 = nullptr, *ggcbc = nullptr;
+  int *ggcbnb = nullptr;
+  double2 *ggnb0 = nullptr, *ggnbc = nullptr;
+  int *goff = nullptr, *gwid = nullptr, *gnb = nullptr;
+  int *cell_intr = nullptr, *wall = nullptr;
+  int ncell_intr = 0, nwall = 0;
+  int nparts = 0;
+  bool have_state = false;
+  double last_ms = 0.0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // per-launch event pairs while profiling (kernel class, start, stop)
+  bool profiling = false;
+  std::vector<hipEvent_t> evpool;
+  std::vector<int> evclass;
+  size_t evused = 0;
+
+  template <class T>
+  int alloc(T *&p, size_t n) {
+    void *q = nullptr;
+    HIPC(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+    allocs.push_back(q);
+    p = static_cast<T *>(q);
+    return 0;
+  }
+  template <class T>
+  int upload(T *&p, const std::vector<T> &h) {
+    if (int rc = alloc(p, h.size())) return rc;
+    if (!h.empty()) HIPC(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    return 0;
+  }
+  ~fvs_ctx() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void *q : allocs) (void)hipFree(q);
+    for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Morton order of cell centroids: compact blocks of consecutive cells
+std::vector<int> morton_order(const MeshPrep &m) {
+  const int N = m.ncells;
+  double xa = *std::min_element(m.cx.begin(), m.cx.end()), xb = *std::max_element(m.cx.begin(), m.cx.end());
+  double ya = *std::min_element(m.cy.begin(), m.cy.end()), yb = *std::max_element(m.cy.begin(), m.cy.end());
+  double s = std::max(xb - xa, yb - ya);
+  if (!(s > 0)) s = 1.0;
+  auto spread = [](uint64_t v) {
+    v &= 0xffffffffull;
+    v = (v | (v << 16)) & 0x0000ffff0000ffffull;
+    v = (v | (v << 8)) & 0x00ff00ff00ff00ffull;
+    v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0full;
+    v = (v | (v << 2)) & 0x3333333333333333ull;
+    v = (v | (v << 1)) & 0x5555555555555555ull;
+    return v;
+  };
+  std::vector<std::pair<uint64_t, int>> key(N);
+  for (int c = 0; c < N; ++c) {
+    uint64_t ix = uint64_t((m.cx[c] - xa) / s * 4294967295.0), iy = uint64_t((m.cy[c] - ya) / s * 4294967295.0);
+    key[c] = {spread(ix) | (spread(iy) << 1), c};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int> perm(N);
+  for (int c = 0; c < N; ++c) perm[c] = key[c].second;
+  return perm;
+}
+
+// sliced-ELL packing of a CSR stencil (cells in device order)
+void pack_sell(int N, const std::vector<int> &perm, const std::vector<int> &iperm, const std::vector<int> &ptr,
+               const std::vector<int> &col, const double *val2, std::vector<int> &off, std::vector<int> &wid,
+               std::vector<int> &nb, std::vector<double2> &cw) {
+  const int ns = (N + kSlice - 1) / kSlice;
+  off.assign(ns + 1, 0);
+  wid.assign(ns, 0);
+  for (int s = 0; s < ns; ++s) {
+    int w = 0;
+    for (int l = 0; l < kSlice; ++l) {
+      int i = s * kSlice + l;
+      if (i < N) w = std::max(w, ptr[perm[i] + 1] - ptr[perm[i]]);
+    }
+    wid[s] = w;
+    off[s + 1] = off[s] + w * kSlice;
+  }
+  nb.assign(off[ns], 0);
+  cw.assign(val2 ? off[ns] : 0, make_double2(0.0, 0.0));
+  for (int s = 0; s < ns; ++s)
+    for (int l = 0; l < kSlice; ++l) {
+      int i = s * kSlice + l;
+      for (int k = 0; k < wid[s]; ++k) {
+        long e = off[s] + long(k) * kSlice + l;
+        if (i >= N) { nb[e] = 0; continue; }
+        int o = perm[i], b = ptr[o], n = ptr[o + 1] - b;
+        if (k < n) {
+          nb[e] = iperm[col[b + k]];
+          if (val2) cw[e] = make_double2(val2[2 * size_t(b + k)], val2[2 * size_t(b + k) + 1]);
+        } else {
+          nb[e] = i;  // padding: the cell itself with zero weight
+        }
+      }
+    }
+}
+
+int setup_device(fvs_ctx &C) {
+  MeshPrep &m = C.mp;
+  const int N = C.N = m.ncells;
+  C.perm.resize(N);
+  if (C.opt.reorder == 1) C.perm = morton_order(m);
+  else std::iota(C.perm.begin(), C.perm.end(), 0);
+  C.iperm.resize(N);
+  for (int i = 0; i < N; ++i) C.iperm[C.perm[i]] = i;
+  const auto &P = C.perm, &IP = C.iperm;
+  C.NFW = (m.nquad > 0) ? 4 : 3;
+  // cells
+  std::vector<double> vol(N), cx(N), cy(N);
+  std::vector<int> nv(N);
+  for (int i = 0; i < N; ++i) vol[i] = m.vol[P[i]], cx[i] = m.cx[P[i]], cy[i] = m.cy[P[i]], nv[i] = m.nvrt[P[i]];
+  if (int rc = C.upload(C.vol, vol)) return rc;
+  if (int rc = C.upload(C.cx, cx)) return rc;
+  if (int rc = C.upload(C.cy, cy)) return rc;
+  if (int rc = C.upload(C.nvrt, nv)) return rc;
+  // interior faces in ascending reference edge id
+  const int Fi = C.Fi = int(m.edge_intr.size());
+  std::vector<int2> fic(Fi);
+  std::vector<double> fig(8 * size_t(Fi), 0.0);
+  std::vector<int> slot_of_edge(m.nedges, -1);
+  for (int f = 0; f < Fi; ++f) {
+    const int e = m.edge_intr[f], c1 = m.ec1[e], c2 = m.ec2[e];
+    slot_of_edge[e] = f;
+    fic[f] = make_int2(IP[c1], IP[c2]);
+    const double xf = m.ex[e], yf = m.ey[e];
+    const double xr = m.cx[c2] + m.esx[e], yr = m.cy[c2] + m.esy[e];
+    double *g = &fig[8 * size_t(f)];
+    g[0] = m.enx[e]; g[1] = m.eny[e]; g[2] = m.ea[e];
+    g[3] = xf - m.cx[c1]; g[4] = yf - m.cy[c1];
+    g[5] = xf - xr; g[6] = yf - yr;
+  }
+  if (int rc = C.upload(C.fi_cells, fic)) return rc;
+  if (int rc = C.upload(C.fi_geo, fig)) return rc;
+  // boundary faces: per boundary, edge i paired with listed cell i
+  const int Fb = C.Fb = int(m.bnd_edges.size());
+  std::vector<int> fbc(Fb), fbt(Fb), wall;
+  std::vector<double> fbg(8 * size_t(Fb), 0.0);
+  std::vector<std::vector<int>> cell_faces(N);  // reference order
+  for (int f = 0; f < Fi; ++f) {
+    const int e = m.edge_intr[f];
+    cell_faces[m.ec1[e]].push_back(f + 1);
+    cell_faces[m.ec2[e]].push_back(-(f + 1));
+  }
+  int b = 0;
+  for (int ib = 0; ib < m.nb; ++ib)
+    for (int k = 0; k < m.bnd_nedges[ib]; ++k, ++b) {
+      const int e = m.bnd_edges[m.bnd_eptr[ib] + k];
+      const int c = m.bnd_cells[m.bnd_cptr[ib] + k];
+      fbc[b] = IP[c];
+      fbt[b] = m.bnd_type[ib];
+      double *g = &fbg[8 * size_t(b)];
+      g[0] = m.enx[e]; g[1] = m.eny[e]; g[2] = m.ea[e];
+      g[3] = m.ex[e] - m.cx[c]; g[4] = m.ey[e] - m.cy[c];
+      g[5] = m.ex[e]; g[6] = m.ey[e];
+      cell_faces[c].push_back(Fi + b + 1);
+      if (m.bnd_type[ib] == FVS_BC_SLIP_WALL) wall.push_back(b);
+    }
+  if (int rc = C.upload(C.fb_cell, fbc)) return rc;
+  if (int rc = C.upload(C.fb_type, fbt)) return rc;
+  if (int rc = C.upload(C.fb_geo, fbg)) return rc;
+  C.nwall = int(wall.size());
+  if (int rc = C.upload(C.wall, wall)) return rc;
+  // gather table: interior faces are already in ascending edge id because
+  // slots follow edge_intr; boundary slots follow them
+  int gw = 1;
+  for (auto &v : cell_faces) gw = std::max(gw, int(v.size()));
+  C.GW = gw;
+  std::vector<int> gat(size_t(gw) * N, 0);
+  for (int i = 0; i < N; ++i) {
+    const auto &v = cell_faces[P[i]];
+    for (size_t k = 0; k < v.size(); ++k) gat[k * N + i] = v[k];
+  }
+  if (int rc = C.upload(C.gat, gat)) return rc;
+  // limiter face offsets (xf - xc) per local edge, in the cell's own frame
+  if (C.opt.limiter > 0) {
+    std::vector<double2> lo(4 * size_t(N), make_double2(0.0, 0.0));
+    for (int i = 0; i < N; ++i) {
+      const int c = P[i];
+      for (int k = 0; k < m.nvrt[c]; ++k) {
+        const int e = m.cedge[4 * c + k];
+        double xc, yc;
+        side_centroid(m, e, c, xc, yc);
+        lo[size_t(k) * N + i] = make_double2(m.ex[e] - xc, m.ey[e] - yc);
+      }
+    }
+    if (int rc = C.upload(C.loff, lo)) return rc;
+  }
+  // LSQ stencil (gradient and/or limiter)
+  if (!m.lsq_ptr.empty()) {
+    std::vector<double> cw(2 * m.lsq_cell.size());
+    for (size_t k = 0; k < m.lsq_cell.size(); ++k) {
+      cw[2 * k] = m.lsq_coef[2 * k] * m.lsq_w[k];
+      cw[2 * k + 1] = m.lsq_coef[2 * k + 1] * m.lsq_w[k];
+    }
+    std::vector<int> off, wid, nb;
+    std::vector<double2> c2;
+    pack_sell(N, P, IP, m.lsq_ptr, m.lsq_cell, cw.data(), off, wid, nb, c2);
+    if (int rc = C.upload(C.soff, off)) return rc;
+    if (int rc = C.upload(C.swid, wid)) return rc;
+    if (int rc = C.upload(C.snb, nb)) return rc;
+    if (int rc = C.upload(C.scw, c2)) return rc;
+  }
+  if (C.opt.grad_method == 1) {
+    std::vector<double2> c0(N), cn(size_t(C.NFW) * N, make_double2(0.0, 0.0));
+    std::vector<int> nb(size_t(C.NFW) * N);
+    for (int i = 0; i < N; ++i) {
+      const int c = P[i];
+      c0[i] = make_double2(m.ggcb_coef0[2 * c], m.ggcb_coef0[2 * c + 1]);
+      for (int k = 0; k < C.NFW; ++k) {
+        if (k < m.nvrt[c]) {
+          nb[size_t(k) * N + i] = IP[m.ggcb_ptr[4 * c + k]];
+          cn[size_t(k) * N + i] = make_double2(m.ggcb_coefnb[8 * c + 2 * k], m.ggcb_coefnb[8 * c + 2 * k + 1]);
+        } else {
+          nb[size_t(k) * N + i] = i;
+        }
+      }
+    }
+    if (int rc = C.upload(C.ggcb0, c0)) return rc;
+    if (int rc = C.upload(C.ggcbnb, nb)) return rc;
+    if (int rc = C.upload(C.ggcbc, cn)) return rc;
+  }
+  if (C.opt.grad_method == 2) {
+    std::vector<double2> c0(N);
+    for (int i = 0; i < N; ++i) c0[i] = make_double2(m.ggnb_coef0[2 * P[i]], m.ggnb_coef0[2 * P[i] + 1]);
+    std::vector<int> off, wid, nb;
+    std::vector<double2> c2;
+    pack_sell(N, P, IP, m.ggnb_ptr, m.ggnb_cell, m.ggnb_c.data(), off, wid, nb, c2);
+    if (int rc = C.upload(C.ggnb0, c0)) return rc;
+    if (int rc = C.upload(C.goff, off)) return rc;
+    if (int rc = C.upload(C.gwid, wid)) return rc;
+    if (int rc = C.upload(C.gnb, nb)) return rc;
+    if (int rc = C.upload(C.ggnbc, c2)) return rc;
+  }
+  // interior cells (vortex error), device ids, reference order
+  std::vector<int> ci(m.cell_intr.size());
+  for (size_t k = 0; k < ci.size(); ++k) ci[k] = IP[m.cell_intr[k]];
+  C.ncell_intr = int(ci.size());
+  if (int rc = C.upload(C.cell_intr, ci)) return rc;
+  // state and work arrays
+  const size_t n4 = 4 * size_t(N);
+  if (int rc = C.alloc(C.U0, n4)) return rc;
+  if (int rc = C.alloc(C.Ub, n4)) return rc;
+  if (int rc = C.alloc(C.Facc, n4)) return rc;
+  if (int rc = C.alloc(C.W, n4)) return rc;
+  if (int rc = C.alloc(C.grad, 2 * n4)) return rc;
+  if (int rc = C.alloc(C.phi, N)) return rc;
+  if (int rc = C.alloc(C.dtl, N)) return rc;
+  if (int rc = C.alloc(C.FA, 4 * size_t(Fi + Fb))) return rc;
+  if (int rc = C.alloc(C.WSA, size_t(Fi + Fb))) return rc;
+  if (int rc = C.alloc(C.rout, n4)) return rc;
+  if (int rc = C.alloc(C.wsout, N)) return rc;
+  C.nparts = std::max(nblk(N), nblk(std::max(C.ncell_intr, 1)));
+  if (int rc = C.alloc(C.part, 12 * size_t(C.nparts))) return rc;
+  if (int rc = C.alloc(C.small, 64)) return rc;
+  // phi = 1 (no limiter) or 0 (first order); grad = 0 for first order
+  const double phi0 = (C.opt.recon == 1) ? 0.0 : 1.0;
+  hipLaunchKernelGGL(k_fill, dim3(nblk(N)), dim3(kBlock), 0, C.stream, long(N), phi0, C.phi);
+  hipLaunchKernelGGL(k_fill, dim3(nblk(8L * N)), dim3(kBlock), 0, C.stream, 8L * N, 0.0, C.grad);
+  hipLaunchKernelGGL(k_fill, dim3(nblk(N)), dim3(kBlock), 0, C.stream, long(N), C.opt.dt, C.dtl);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(C.stream));
+  return 0;
+}
+
+// kernel classes timed by fvs_kernel_times
+enum { KC_GRAD = 0, KC_FLUX_INT = 1, KC_FLUX_BND = 2, KC_UPDATE = 3, KC_NORM = 4, KC_N = 5 };
+
+struct ProfScope {
+  fvs_ctx &C;
+  int cls;
+  bool on;
+  ProfScope(fvs_ctx &c, int k) : C(c), cls(k), on(c.profiling) {
+    if (!on) return;
+    if (C.evused + 2 > C.evpool.size()) {
+      for (int q = 0; q < 64; ++q) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
+        C.evpool.push_back(e);
+      }
+    }
+    (void)hipEventRecord(C.evpool[C.evused], C.stream);
+  }
+  ~ProfScope() {
+    if (!on) return;
+    (void)hipEventRecord(C.evpool[C.evused + 1], C.stream);
+    C.evclass.push_back(cls);
+    C.evused += 2;
+  }
+};
+
+int launch_gradient(fvs_ctx &C, bool with_limiter) {
+  const int N = C.N;
+  const bool lim = with_limiter && C.opt.limiter > 0 && C.opt.recon != 1;
+  const int lt = C.opt.limiter;
+  dim3 g(nblk(N)), b(kBlock);
+  ProfScope ps(C, KC_GRAD);
+  if (C.opt.grad_method == 3) {
+    if (lim)
+      hipLaunchKernelGGL(k_grad_lsq<1>, g, b, 0, C.stream, N, C.W, C.soff, C.swid, C.snb, C.scw, C.grad, C.phi, lt,
+                         C.nvrt, C.loff, C.vol);
+    else
+      hipLaunchKernelGGL(k_grad_lsq<0>, g, b, 0, C.stream, N, C.W, C.soff, C.swid, C.snb, C.scw, C.grad, C.phi, lt,
+                         C.nvrt, C.loff, C.vol);
+  } else if (C.opt.grad_method == 1) {
+    if (lim)
+      hipLaunchKernelGGL(k_grad_ggcb<1>, g, b, 0, C.stream, N, C.NFW, C.W, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
+                         C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
+    else
+      hipLaunchKernelGGL(k_grad_ggcb<0>, g, b, 0, C.stream, N, C.NFW, C.W, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
+                         C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
+  } else {
+    if (lim)
+      hipLaunchKernelGGL(k_grad_ggnb<1>, g, b, 0, C.stream, N, C.W, C.ggnb0, C.goff, C.gwid, C.gnb, C.ggnbc, C.vol,
+                         C.grad, C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
+    else
+      hipLaunchKernelGGL(k_grad_ggnb<0>, g, b, 0, C.stream, N, C.W, C.ggnb0, C.goff, C.gwid, C.gnb, C.ggnbc, C.vol,
+                         C.grad, C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
+  }
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+// gradient + limiter + face fluxes of the current primitive state W
+int launch_fluxes(fvs_ctx &C, double t) {
+  if (C.opt.recon != 1)
+    if (int rc = launch_gradient(C, true)) return rc;
+  if (C.Fi > 0) {
+    ProfScope ps(C, KC_FLUX_INT);
+    hipLaunchKernelGGL(k_flux_int, dim3(nblk(C.Fi)), dim3(kBlock), 0, C.stream, C.Fi, C.P, C.fi_cells, C.fi_geo,
+                       C.W, C.grad, C.phi, C.FA, C.WSA);
+  }
+  if (C.Fb > 0) {
+    ProfScope ps(C, KC_FLUX_BND);
+    hipLaunchKernelGGL(k_flux_bnd, dim3(nblk(C.Fb)), dim3(kBlock), 0, C.stream, C.Fb, C.Fi, C.P, t, C.fb_cell,
+                       C.fb_type, C.fb_geo, C.W, C.grad, C.phi, C.FA, C.WSA);
+  }
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int launch_update(fvs_ctx &C, const Upd &A) {
+  ProfScope ps(C, KC_UPDATE);
+  hipLaunchKernelGGL(k_update, dim3(nblk(C.N)), dim3(kBlock), 0, C.stream, C.N, C.GW, A, C.gat, C.FA, C.WSA, C.vol,
+                     C.U0, C.Ub, C.Facc, C.W, C.dtl, C.rout, C.wsout, C.part);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int launch_vortex_err(fvs_ctx &C, double t, double *dst) {
+  const int n = C.ncell_intr;
+  if (n == 0) return 0;
+  const int nb = nblk(n);
+  hipLaunchKernelGGL(k_vortex_err, dim3(nb), dim3(kBlock), 0, C.stream, n, C.P, t, C.cell_intr, C.cx, C.cy, C.U0,
+                     C.part);
+  hipLaunchKernelGGL(k_vortex_final, dim3(1), dim3(64), 0, C.stream, nb, n, C.part, dst);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+// host <-> device cell-order permutation of per-cell records of width k
+template <class T>
+void to_device_order(const fvs_ctx &C, const T *src, std::vector<T> &dst, int k) {
+  dst.resize(size_t(C.N) * k);
+  for (int i = 0; i < C.N; ++i)
+    for (int j = 0; j < k; ++j) dst[size_t(i) * k + j] = src[size_t(C.perm[i]) * k + j];
+}
+template <class T>
+void to_ref_order(const fvs_ctx &C, const std::vector<T> &src, T *dst, int k) {
+  for (int i = 0; i < C.N; ++i)
+    for (int j = 0; j < k; ++j) dst[size_t(C.perm[i]) * k + j] = src[size_t(i) * k + j];
+}
+
+int need_state(const fvs_ctx *C) {
+  if (!C) return fail(FVS_EINVAL, "null context");
+  if (!C->have_state) return fail(FVS_ESTATE, "no state: call fvs_set_state or fvs_initialize first");
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char *fvs_last_error(void) { return g_err.c_str(); }
+
+int fvs_create(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
+               const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types, const int *bnd_cells,
+               const fvs_options *opt) {
+  if (!out || !xy || !cell_nodes || !opt || (nb > 0 && (!bnd_ncells || !bnd_types)))
+    return fail(FVS_EINVAL, "fvs_create: null argument");
+  *out = nullptr;
+  if (opt->recon < 1 || opt->recon > 3) return fail(FVS_EINVAL, "recon must be 1, 2 or 3");
+  if (opt->limiter < 0 || opt->limiter > 3) return fail(FVS_EINVAL, "limiter must be 0..3");
+  if (opt->rk_scheme < 0 || opt->rk_scheme > 2) return fail(FVS_EINVAL, "unknown rk_scheme");
+  if (opt->rk_scheme == FVS_RK_CLASSIC && (opt->rk_order < 1 || opt->rk_order > 4))
+    return fail(FVS_EINVAL, "rk_order must be 1..4");
+  if (!(opt->gamma > 1.0)) return fail(FVS_EINVAL, "gamma must exceed 1");
+  int devs = 0;
+  if (hipGetDeviceCount(&devs) != hipSuccess || devs < 1) return fail(FVS_EHIP, "no HIP device available");
+  auto *C = new fvs_ctx();
+  C->opt = *opt;
+  SchemeFlags fl;
+  fl.grad_method = opt->grad_method;
+  fl.lsq_nn = opt->lsq_nn;
+  fl.lsq_p = opt->lsq_p;
+  fl.limiter = opt->limiter;
+  std::string err = build_mesh(C->mp, nnodes, xy, ntri, nquad, cell_nodes, node_canon, nb, bnd_ncells, bnd_types,
+                               bnd_cells, fl);
+  if (!err.empty()) {
+    delete C;
+    return fail(FVS_EINVAL, err);
+  }
+  const double kPi = 3.14159265358979323846;
+  Phys &P = C->P;
+  P.gamma = opt->gamma;
+  P.kappa = (opt->recon == 3) ? opt->kappa : 0.0;  // src/input.f90:258-271
+  P.pinf[0] = 1.0;                                  // src/data_solution.f90:56-59
+  P.pinf[1] = opt->mach * std::cos(opt->aoa_deg * (kPi / 180.0));
+  P.pinf[2] = opt->mach * std::sin(opt->aoa_deg * (kPi / 180.0));
+  P.pinf[3] = 1.0 / opt->gamma;
+  P.vx0 = opt->vortex_x0;
+  P.vy0 = opt->vortex_y0;
+  P.vK = opt->vortex_K;
+  for (int k = 0; k < 4; ++k) P.vinf[k] = opt->vortex_inf[k];
+  P.lvortex = opt->lvortex;
+  P.limiter = opt->limiter;
+  if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete C;
+    return fail(FVS_EHIP, "hipStreamCreate failed");
+  }
+  (void)hipEventCreate(&C->ev0);
+  (void)hipEventCreate(&C->ev1);
+  if (int rc = setup_device(*C)) {
+    std::string e = g_err;
+    delete C;
+    g_err = e;
+    return rc;
+  }
+  *out = C;
+  return 0;
+}
+
+int fvs_destroy(fvs_ctx *ctx) {
+  delete ctx;
+  return 0;
+}
+
+int fvs_sizes(const fvs_ctx *C, int *o) {
+  if (!C || !o) return fail(FVS_EINVAL, "null argument");
+  const MeshPrep &m = C->mp;
+  o[0] = m.ncells; o[1] = m.nedges; o[2] = m.nnodes; o[3] = int(m.edge_intr.size());
+  o[4] = m.nedges_bndr; o[5] = int(m.cell_intr.size()); o[6] = m.ncells_bndr; o[7] = m.nb;
+  return 0;
+}
+
+int fvs_mesh_create(fvs_mesh **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
+                    const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types, const int *bnd_cells,
+                    int grad_method, int lsq_nn, double lsq_p, int limiter) {
+  if (!out || !xy || !cell_nodes || (nb > 0 && (!bnd_ncells || !bnd_types)))
+    return fail(FVS_EINVAL, "fvs_mesh_create: null argument");
+  *out = nullptr;
+  SchemeFlags fl;
+  fl.grad_method = grad_method;
+  fl.lsq_nn = lsq_nn;
+  fl.lsq_p = lsq_p;
+  fl.limiter = limiter;
+  auto *M = new fvs_mesh();
+  std::string err = build_mesh(M->mp, nnodes, xy, ntri, nquad, cell_nodes, node_canon, nb, bnd_ncells, bnd_types,
+                               bnd_cells, fl);
+  if (!err.empty()) {
+    delete M;
+    return fail(FVS_EINVAL, err);
+  }
+  *out = M;
+  return 0;
+}
+
+int fvs_mesh_destroy(fvs_mesh *M) {
+  delete M;
+  return 0;
+}
+
+const fvs_mesh *fvs_ctx_mesh(const fvs_ctx *C) { return C ? &C->mesh : nullptr; }
+
+int fvs_mesh_sizes(const fvs_mesh *M, int *o) {
+  if (!M || !o) return fail(FVS_EINVAL, "null argument");
+  const MeshPrep &m = M->mp;
+  o[0] = m.ncells; o[1] = m.nedges; o[2] = m.nnodes; o[3] = int(m.edge_intr.size());
+  o[4] = m.nedges_bndr; o[5] = int(m.cell_intr.size()); o[6] = m.ncells_bndr; o[7] = m.nb;
+  return 0;
+}
+
+int fvs_mesh_topology(const fvs_mesh *M, int *n1, int *n2, int *c1, int *c2, double *g7, double *xyv3, int *nghbre4,
+                      int *edge4) {
+  if (!M) return fail(FVS_EINVAL, "null mesh");
+  const MeshPrep &m = M->mp;
+  const size_t E = m.nedges, N = m.ncells;
+  if (n1) std::copy(m.en1.begin(), m.en1.end(), n1);
+  if (n2) std::copy(m.en2.begin(), m.en2.end(), n2);
+  if (c1) std::copy(m.ec1.begin(), m.ec1.end(), c1);
+  if (c2) std::copy(m.ec2.begin(), m.ec2.end(), c2);
+  if (g7) {
+    const std::vector<double> *v[7] = {&m.ex, &m.ey, &m.ea, &m.enx, &m.eny, &m.esx, &m.esy};
+    for (int k = 0; k < 7; ++k) std::copy(v[k]->begin(), v[k]->end(), g7 + k * E);
+  }
+  if (xyv3) {
+    std::copy(m.cx.begin(), m.cx.end(), xyv3);
+    std::copy(m.cy.begin(), m.cy.end(), xyv3 + N);
+    std::copy(m.vol.begin(), m.vol.end(), xyv3 + 2 * N);
+  }
+  if (nghbre4) std::copy(m.nghbre.begin(), m.nghbre.end(), nghbre4);
+  if (edge4) std::copy(m.cedge.begin(), m.cedge.end(), edge4);
+  return 0;
+}
+
+int fvs_mesh_lists(const fvs_mesh *M, int *edge_intr, int *cell_intr, int *bnd_nedges, int *bnd_edges) {
+  if (!M) return fail(FVS_EINVAL, "null mesh");
+  const MeshPrep &m = M->mp;
+  if (edge_intr) std::copy(m.edge_intr.begin(), m.edge_intr.end(), edge_intr);
+  if (cell_intr) std::copy(m.cell_intr.begin(), m.cell_intr.end(), cell_intr);
+  if (bnd_nedges) std::copy(m.bnd_nedges.begin(), m.bnd_nedges.end(), bnd_nedges);
+  if (bnd_edges) std::copy(m.bnd_edges.begin(), m.bnd_edges.end(), bnd_edges);
+  return 0;
+}
+
+int fvs_mesh_lsq_nnz(const fvs_mesh *M) { return (M && !M->mp.lsq_ptr.empty()) ? M->mp.lsq_ptr.back() : 0; }
+
+int fvs_mesh_lsq(const fvs_mesh *M, int *ptr, int *cell, double *w, double *coef) {
+  if (!M) return fail(FVS_EINVAL, "null mesh");
+  const MeshPrep &m = M->mp;
+  if (m.lsq_ptr.empty()) return fail(FVS_ESTATE, "no LSQ stencil in this configuration");
+  if (ptr) std::copy(m.lsq_ptr.begin(), m.lsq_ptr.end(), ptr);
+  if (cell) std::copy(m.lsq_cell.begin(), m.lsq_cell.end(), cell);
+  if (w) std::copy(m.lsq_w.begin(), m.lsq_w.end(), w);
+  if (coef) std::copy(m.lsq_coef.begin(), m.lsq_coef.end(), coef);
+  return 0;
+}
+
+int fvs_set_state(fvs_ctx *C, const double *cvar) {
+  if (!C || !cvar) return fail(FVS_EINVAL, "null argument");
+  std::vector<double> h;
+  to_device_order(*C, cvar, h, 4);
+  HIPC(hipMemcpyAsync(C->U0, h.data(), h.size() * 8, hipMemcpyHostToDevice, C->stream));
+  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(C->stream));
+  C->have_state = true;
+  return 0;
+}
+
+int fvs_get_state(fvs_ctx *C, double *cvar) {
+  if (int rc = need_state(C)) return rc;
+  if (!cvar) return fail(FVS_EINVAL, "null argument");
+  std::vector<double> h(4 * size_t(C->N));
+  HIPC(hipMemcpyAsync(h.data(), C->U0, h.size() * 8, hipMemcpyDeviceToHost, C->stream));
+  HIPC(hipStreamSynchronize(C->stream));
+  to_ref_order(*C, h, cvar, 4);
+  return 0;
+}
+
+int fvs_initialize(fvs_ctx *C, int mode, double t0) {
+  if (!C) return fail(FVS_EINVAL, "null context");
+  // initial primitive state at the cell centres, then pvar2cvar
+  // (src/initialize.f90:36-54,79-87; src/data_solution.f90:92-106)
+  const MeshPrep &m = C->mp;
+  const double kPi = 3.14159265358979323846, g = C->opt.gamma;
+  std::vector<double> cv(4 * size_t(C->N));
+  for (int c = 0; c < C->N; ++c) {
+    double pv[4];
+    const double x = m.cx[c], y = m.cy[c];
+    if (mode == 0) {
+      pv[0] = 1.12 + 0.15 * std::sin((3.12 * kPi) * x + (2.92 * kPi) * y);
+      pv[1] = 1.32 + 0.06 * std::sin((2.09 * kPi) * x + (3.12 * kPi) * y);
+      pv[2] = 1.18 + 0.03 * std::sin((2.15 * kPi) * x + (3.32 * kPi) * y);
+      pv[3] = 1.62 + 0.31 * std::sin((3.79 * kPi) * x + (2.98 * kPi) * y);
+    } else if (C->opt.lvortex) {
+      const Phys &P = C->P;
+      const double Tinf = P.vinf[3] / P.vinf[0];
+      const double dx = x - (P.vx0 + P.vinf[1] * t0), dy = y - (P.vy0 + P.vinf[2] * t0);
+      const double r = std::sqrt(dx * dx + dy * dy), cc = P.vK / (2.0 * kPi);
+      pv[1] = P.vinf[1] - cc * dy * std::exp(0.5 * (1.0 - r * r));
+      pv[2] = P.vinf[2] + cc * dx * std::exp(0.5 * (1.0 - r * r));
+      const double temp = Tinf - (cc * cc) * (g - 1.0) / (2.0 * g) * std::exp(1.0 - r * r);
+      pv[0] = std::pow(temp, 1.0 / (g - 1.0));
+      pv[3] = std::pow(pv[0], g);
+    } else {
+      for (int k = 0; k < 4; ++k) pv[k] = C->P.pinf[k];
+    }
+    double *u = &cv[4 * size_t(c)];
+    u[0] = pv[0];
+    u[1] = pv[0] * pv[1];
+    u[2] = pv[0] * pv[2];
+    u[3] = pv[3] / (g - 1.0) + 0.5 * pv[0] * (pv[1] * pv[1] + pv[2] * pv[2]);
+  }
+  return fvs_set_state(C, cv.data());
+}
+
+int fvs_primitives(fvs_ctx *C, double *pvar) {
+  if (int rc = need_state(C)) return rc;
+  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  HIPC(hipGetLastError());
+  if (pvar) {
+    std::vector<double> h(4 * size_t(C->N));
+    HIPC(hipMemcpyAsync(h.data(), C->W, h.size() * 8, hipMemcpyDeviceToHost, C->stream));
+    HIPC(hipStreamSynchronize(C->stream));
+    to_ref_order(*C, h, pvar, 4);
+  }
+  HIPC(hipStreamSynchronize(C->stream));
+  return 0;
+}
+
+int fvs_gradient(fvs_ctx *C, double *grad) {
+  if (int rc = need_state(C)) return rc;
+  if (C->opt.recon != 1)
+    if (int rc = launch_gradient(*C, false)) return rc;
+  if (grad) {
+    std::vector<double> h(8 * size_t(C->N));
+    HIPC(hipMemcpyAsync(h.data(), C->grad, h.size() * 8, hipMemcpyDeviceToHost, C->stream));
+    HIPC(hipStreamSynchronize(C->stream));
+    // device record [x0..x3, y0..y3] -> Fortran grad(4,N,2)
+    const int N = C->N;
+    for (int i = 0; i < N; ++i)
+      for (int v = 0; v < 4; ++v) {
+        grad[size_t(C->perm[i]) * 4 + v] = h[8 * size_t(i) + v];
+        grad[size_t(N) * 4 + size_t(C->perm[i]) * 4 + v] = h[8 * size_t(i) + 4 + v];
+      }
+  }
+  HIPC(hipStreamSynchronize(C->stream));
+  return 0;
+}
+
+int fvs_gradient_1var(fvs_ctx *C, const double *fc, double *dfc) {
+  if (!C || !fc || !dfc) return fail(FVS_EINVAL, "null argument");
+  // the scalar rides in slot 0 of a scratch primitive record; the gradient
+  // kernels treat the four slots independently
+  const int N = C->N;
+  std::vector<double> h(4 * size_t(N), 0.0);
+  for (int i = 0; i < N; ++i) h[4 * size_t(i)] = fc[C->perm[i]];
+  HIPC(hipMemcpyAsync(C->rout, h.data(), h.size() * 8, hipMemcpyHostToDevice, C->stream));
+  std::swap(C->W, C->rout);
+  int rc = launch_gradient(*C, false);
+  std::swap(C->W, C->rout);
+  if (rc) return rc;
+  std::vector<double> g(8 * size_t(N));
+  HIPC(hipMemcpyAsync(g.data(), C->grad, g.size() * 8, hipMemcpyDeviceToHost, C->stream));
+  HIPC(hipStreamSynchronize(C->stream));
+  for (int i = 0; i < N; ++i) {
+    dfc[C->perm[i]] = g[8 * size_t(i)];
+    dfc[size_t(N) + C->perm[i]] = g[8 * size_t(i) + 4];
+  }
+  return 0;
+}
+
+int fvs_limiter(fvs_ctx *C, double *phi) {
+  if (int rc = need_state(C)) return rc;
+  if (C->opt.recon != 1 && C->opt.limiter > 0)
+    if (int rc = launch_gradient(*C, true)) return rc;
+  if (phi) {
+    std::vector<double> h(C->N);
+    HIPC(hipMemcpyAsync(h.data(), C->phi, h.size() * 8, hipMemcpyDeviceToHost, C->stream));
+    HIPC(hipStreamSynchronize(C->stream));
+    to_ref_order(*C, h, phi, 1);
+  }
+  HIPC(hipStreamSynchronize(C->stream));
+  return 0;
+}
+
+int fvs_compute_residual(fvs_ctx *C, double time, double *resid, double *ws) {
+  if (int rc = need_state(C)) return rc;
+  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  if (int rc = launch_fluxes(*C, time)) return rc;
+  Upd A{};
+  A.mode = M_RESID;
+  A.gamma = C->opt.gamma;
+  if (int rc = launch_update(*C, A)) return rc;
+  std::vector<double> h;
+  if (resid) {
+    h.resize(4 * size_t(C->N));
+    HIPC(hipMemcpyAsync(h.data(), C->rout, h.size() * 8, hipMemcpyDeviceToHost, C->stream));
+    HIPC(hipStreamSynchronize(C->stream));
+    to_ref_order(*C, h, resid, 4);
+  }
+  if (ws) {
+    h.resize(C->N);
+    HIPC(hipMemcpyAsync(h.data(), C->wsout, h.size() * 8, hipMemcpyDeviceToHost, C->stream));
+    HIPC(hipStreamSynchronize(C->stream));
+    to_ref_order(*C, h, ws, 1);
+  }
+  HIPC(hipStreamSynchronize(C->stream));
+  return 0;
+}
+
+int fvs_local_dt(fvs_ctx *C, double *dt) {
+  if (int rc = need_state(C)) return rc;
+  // ws_nrml of the current state, then dt = CFL*vol/(0.5*ws)
+  std::vector<double> ws(C->N);
+  if (int rc = fvs_compute_residual(C, 0.0, nullptr, ws.data())) return rc;
+  if (dt)
+    for (int c = 0; c < C->N; ++c) dt[c] = C->opt.cfl * C->mp.vol[c] / (0.5 * ws[c]);
+  return 0;
+}
+
+int fvs_roe_flux_batch(int n, double gamma, const double *pL, const double *pR, const double *nxy, double *flux,
+                       double *ws) {
+  if (n <= 0) return 0;
+  if (!pL || !pR || !nxy || !flux || !ws) return fail(FVS_EINVAL, "null argument");
+  double *d = nullptr;
+  const size_t bytes = size_t(n) * 8 * (4 + 4 + 2 + 4 + 1);
+  HIPC(hipMalloc(&d, bytes));
+  double *dL = d, *dR = dL + 4 * size_t(n), *dn = dR + 4 * size_t(n), *dF = dn + 2 * size_t(n), *dw = dF + 4 * size_t(n);
+  int rc = 0;
+  if (hipMemcpy(dL, pL, 32 * size_t(n), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dR, pR, 32 * size_t(n), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dn, nxy, 16 * size_t(n), hipMemcpyHostToDevice) != hipSuccess) {
+    rc = fail(FVS_EHIP, "hipMemcpy failed");
+  } else {
+    hipLaunchKernelGGL(k_roe_batch, dim3(nblk(n)), dim3(kBlock), 0, 0, n, gamma, dL, dR, dn, dF, dw);
+    if (hipGetLastError() != hipSuccess || hipMemcpy(flux, dF, 32 * size_t(n), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ws, dw, 8 * size_t(n), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(FVS_EHIP, "Roe batch kernel failed");
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+int fvs_time_integration(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr) {
+  if (int rc = need_state(C)) return rc;
+  if (nsteps < 0) return fail(FVS_EINVAL, "nsteps < 0");
+  if (nsteps == 0) return 0;
+  const fvs_options &o = C->opt;
+  const double dt = o.dt;
+  double *dn = nullptr, *dv = nullptr;
+  HIPC(hipMalloc(&dn, std::max(1, nsteps) * 4 * sizeof(double)));
+  if (verr && hipMalloc(&dv, size_t(nsteps) * 13 * sizeof(double)) != hipSuccess) {
+    (void)hipFree(dn);
+    return fail(FVS_EHIP, "hipMalloc failed");
+  }
+  // runge_kutta_init coefficient tables (src/runge_kutta.f90:36-65)
+  double ak = 1, bk = 2, ck = 2, dk = 1;
+  if (o.rk_order == 1) ak = 3.60897, bk = 2.04, ck = 0.34206, dk = 0.00897;
+  if (o.rk_order == 2) ak = 0.11, bk = 3.92, ck = 1.86, dk = 0.11;
+  if (o.rk_order == 3) ak = 0.65, bk = 2.7, ck = 2.0, dk = 0.65;
+  const double rk_coef[4] = {ak, bk, ck, dk};
+  const double h_rk[4] = {dt / 2.0, dt / 2.0, dt, dt / 6.0};
+  const double dts[4] = {0.0, dt / 3.0, dt * 2.0 / 3.0, dt};
+  const double ssp3_a[3] = {0.0, 0.75, 1.0 / 3.0}, ssp3_b[3] = {1.0, 0.25, 2.0 / 3.0};
+  const int nst = (o.rk_scheme == FVS_RK_SSP3) ? 3 : 4;
+  int rc = 0;
+  HIPC(hipEventRecord(C->ev0, C->stream));
+  for (int s = 0; s < nsteps && !rc; ++s) {
+    const double told = t1 + double(s) * dt;
+    for (int k = 0; k < nst && !rc; ++k) {
+      Upd A{};
+      A.steady = o.steady;
+      A.first = (k == 0);
+      A.last = (k == nst - 1);
+      A.cfl = o.cfl;
+      A.gamma = o.gamma;
+      double ts = told;
+      if (o.rk_scheme == FVS_RK_SSP3) {
+        A.mode = M_SSP3;
+        A.a = ssp3_a[k];
+        A.b = ssp3_b[k];
+        A.h = o.steady ? 1.0 : dt;
+        ts = (k == 0) ? told : (k == 1 ? told + dt : told + 0.5 * dt);
+      } else if (o.rk_scheme == FVS_RK_SSP42) {
+        A.mode = M_SSP42;
+        A.coef = 1.0;
+        A.h = o.steady ? (k == 3 ? 1.0 / 4.0 : 1.0 / 3.0) : (k == 3 ? dt / 4.0 : dt / 3.0);
+        ts = told + dts[k];
+      } else {
+        A.mode = M_CLASSIC;
+        A.coef = rk_coef[k];
+        A.use_acc = (k == 3);
+        A.h = o.steady ? (k == 2 ? 1.0 : (k == 3 ? 1.0 / 6.0 : 1.0 / 2.0)) : h_rk[k];
+        const double t12 = told + 0.5 * dt, tnew = told + dt;
+        const double tst[4] = {told, t12, t12, tnew};
+        ts = tst[k];
+      }
+      rc = launch_fluxes(*C, ts);
+      if (!rc) rc = launch_update(*C, A);
+    }
+    if (rc) break;
+    {
+      ProfScope ps(*C, KC_NORM);
+      hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, C->stream, nblk(C->N), C->N, C->part, dn + 4 * s);
+    }
+    if (dv) rc = launch_vortex_err(*C, told + dt, dv + 13 * size_t(s));
+  }
+  if (!rc) {
+    hipError_t e = hipEventRecord(C->ev1, C->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(C->stream);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, C->ev0, C->ev1);
+    if (e != hipSuccess) rc = fail(FVS_EHIP, std::string("time integration: ") + hipGetErrorString(e));
+    C->last_ms = ms;
+  }
+  if (!rc && norms && hipMemcpy(norms, dn, size_t(nsteps) * 4 * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(FVS_EHIP, "copy of residual norms failed");
+  if (!rc && verr && hipMemcpy(verr, dv, size_t(nsteps) * 13 * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(FVS_EHIP, "copy of vortex errors failed");
+  (void)hipFree(dn);
+  if (dv) (void)hipFree(dv);
+  return rc;
+}
+
+int fvs_vortex_error(fvs_ctx *C, double t, double *out13) {
+  if (int rc = need_state(C)) return rc;
+  if (!out13) return fail(FVS_EINVAL, "null argument");
+  if (int rc = launch_vortex_err(*C, t, C->small)) return rc;
+  HIPC(hipMemcpyAsync(out13, C->small, 13 * 8, hipMemcpyDeviceToHost, C->stream));
+  HIPC(hipStreamSynchronize(C->stream));
+  return 0;
+}
+
+int fvs_wall_forces(fvs_ctx *C, double *out4) {
+  if (int rc = need_state(C)) return rc;
+  if (!out4) return fail(FVS_EINVAL, "null argument");
+  const double kPi = 3.14159265358979323846;
+  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  if (int rc = launch_gradient(*C, false)) return rc;
+  const double al = C->opt.aoa_deg * (kPi / 180.0);
+  hipLaunchKernelGGL(k_wall_forces, dim3(1), dim3(64), 0, C->stream, C->nwall, C->wall, C->fb_cell, C->fb_geo, C->W,
+                     C->grad, C->opt.mach, C->P.pinf[3], std::cos(al), std::sin(al), C->small + 16);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(out4, C->small + 16, 4 * 8, hipMemcpyDeviceToHost, C->stream));
+  HIPC(hipStreamSynchronize(C->stream));
+  return 0;
+}
+
+int fvs_set_timestep(fvs_ctx *C, double dt, double cfl) {
+  if (!C) return fail(FVS_EINVAL, "null context");
+  if (!(dt > 0.0) || !(cfl > 0.0)) return fail(FVS_EINVAL, "dt and cfl must be positive");
+  C->opt.dt = dt;
+  C->opt.cfl = cfl;
+  return 0;
+}
+
+int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *bytes, int *launches) {
+  if (int rc = need_state(C)) return rc;
+  if (nsteps <= 0 || !ms || !bytes || !launches) return fail(FVS_EINVAL, "bad argument");
+  C->profiling = true;
+  C->evused = 0;
+  C->evclass.clear();
+  int rc = fvs_time_integration(C, t1, nsteps, nullptr, nullptr);
+  C->profiling = false;
+  if (rc) return rc;
+  double tot[KC_N] = {0, 0, 0, 0, 0};
+  int cnt[KC_N] = {0, 0, 0, 0, 0};
+  for (size_t k = 0; k < C->evclass.size(); ++k) {
+    float t = 0.f;
+    HIPC(hipEventElapsedTime(&t, C->evpool[2 * k], C->evpool[2 * k + 1]));
+    tot[C->evclass[k]] += t;
+    cnt[C->evclass[k]]++;
+  }
+  // algorithmic bytes per launch: every array the kernel must stream, each
+  // touched once; neighbour re-reads of W / grad / phi / FA are cache hits
+  const MeshPrep &m = C->mp;
+  const double N = C->N, Fi = C->Fi, Fb = C->Fb;
+  const bool lim = C->opt.limiter > 0 && C->opt.recon != 1;
+  double bg = 0.0;
+  if (C->opt.grad_method == 3) bg = N * (32 + 64) + 20.0 * m.lsq_ptr.back();
+  else if (C->opt.grad_method == 1) bg = N * (32 + 16 + 8 + 64) + N * C->NFW * 20.0;
+  else bg = N * (32 + 16 + 8 + 64) + 20.0 * m.ggnb_ptr.back();
+  if (lim) {
+    bg += N * (8 + 4) + 16.0 * (3 * m.ntri + 4 * m.nquad);
+    if (C->opt.grad_method == 3) bg += 0.0;
+    else bg += 4.0 * m.lsq_ptr.back() + 8.0 * (C->opt.grad_method == 1 ? N : 0.0);
+  }
+  const double bfi = Fi * (8 + 56 + 40) + N * (32 + 64 + 8);
+  const double bfb = Fb * (8 + 56 + 40) + Fb * (32 + 64 + 8);
+  // update: gather ids, face fluxes once, vol, W write, state traffic per stage
+  double state = 0.0;
+  const int nst = (C->opt.rk_scheme == FVS_RK_SSP3) ? 3 : 4;
+  if (C->opt.rk_scheme == FVS_RK_SSP3) state = (64 + 96 + 96) / 3.0;
+  else if (C->opt.rk_scheme == FVS_RK_SSP42) state = (32 + 32 + 64 + 64 + 64 + 64 + 32 + 32 + 32) / 4.0;
+  else state = (32 + 32 + 64 + 64 + 64 + 64 + 32 + 32 + 32) / 4.0;
+  if (C->opt.steady) state += (8.0 + 8.0 * (nst - 1)) / nst;
+  const double bup = N * (C->GW * 4.0 + 8 + 32 + state) + (Fi + Fb) * 40.0;
+  const double bno = 4.0 * 8 * nblk(C->N) + 32;
+  const double b5[KC_N] = {bg, bfi, bfb, bup, bno};
+  for (int k = 0; k < KC_N; ++k) {
+    ms[k] = cnt[k] ? tot[k] / cnt[k] : 0.0;
+    bytes[k] = b5[k];
+    launches[k] = cnt[k];
+  }
+  return 0;
+}
+
+double fvs_last_elapsed_ms(const fvs_ctx *C) { return C ? C->last_ms : 0.0; }
+
+}  // extern "C"

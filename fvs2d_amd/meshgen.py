@@ -1,0 +1,303 @@
+"""Mesh containers, synthesisers and the reference's text mesh format.
+
+A mesh is what the reference reads from ``<base>.grid`` / ``<base>.bc``
+(src/grid_procs.f90:65-164): node coordinates, triangles then quads as
+counter-clockwise node lists, and boundaries, each a type name plus the list of
+cells that carry one of its edges.  Indices here are 0-based.
+
+Synthesisers:
+
+* :func:`jittered_square` -- the isentropic-vortex square: ``nx*ny`` quads over
+  ``[-5,5]^2``, each split by its bottom-left/top-right diagonal into two
+  triangles, interior nodes displaced uniformly inside a disc of radius
+  ``jitter*h``.  ``periodic=True`` identifies the seam nodes (builder-new
+  periodic boundary); otherwise the whole outline is one ``dirichlet``
+  boundary and the two corner quads whose lower/upper triangle would carry two
+  boundary edges get the other diagonal, so every boundary cell has exactly one
+  boundary edge (the reference pairs boundary cells and boundary edges one to
+  one, src/residual.f90:115-125).
+* :func:`naca_ogrid` -- O-grid around a symmetric NACA 4-digit airfoil.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+BC_TYPES = {"freestream": 0, "slip_wall": 1, "dirichlet": 2, "solid_wall": 3}
+
+
+@dataclasses.dataclass
+class Mesh:
+    xy: np.ndarray                      # (nnodes, 2) float64
+    ntri: int
+    nquad: int
+    cell_nodes: np.ndarray              # ntri*3 + nquad*4 int32, 0-based, CCW
+    boundaries: List[Tuple[str, np.ndarray]]
+    canon: Optional[np.ndarray] = None  # periodic node identification (or None)
+    period: Tuple[float, float] = (0.0, 0.0)
+
+    @property
+    def nnodes(self) -> int:
+        return int(self.xy.shape[0])
+
+    @property
+    def ncells(self) -> int:
+        return self.ntri + self.nquad
+
+    def cells_as_array(self) -> np.ndarray:
+        """(ncells, 4) node table, -1 in the 4th slot of triangles."""
+        out = -np.ones((self.ncells, 4), np.int32)
+        t = self.cell_nodes[: 3 * self.ntri].reshape(-1, 3)
+        out[: self.ntri, :3] = t
+        if self.nquad:
+            out[self.ntri:, :] = self.cell_nodes[3 * self.ntri:].reshape(-1, 4)
+        return out
+
+    def bc_arrays(self):
+        """(ncells per boundary, type ids, concatenated cells) as int32."""
+        n = np.array([len(c) for _, c in self.boundaries], np.int32)
+        t = np.array([BC_TYPES[name] for name, _ in self.boundaries], np.int32)
+        cells = (np.concatenate([np.asarray(c, np.int32) for _, c in self.boundaries])
+                 if self.boundaries else np.zeros(0, np.int32))
+        return n, t, cells
+
+
+def _tri_area(xy, tri):
+    p1, p2, p3 = xy[tri[:, 0]], xy[tri[:, 1]], xy[tri[:, 2]]
+    return 0.5 * ((p2[:, 0] - p1[:, 0]) * (p3[:, 1] - p1[:, 1]) - (p3[:, 0] - p1[:, 0]) * (p2[:, 1] - p1[:, 1]))
+
+
+def jittered_square(nx: int, ny: Optional[int] = None, *, lo: float = -5.0, hi: float = 5.0,
+                    jitter: float = 0.1, seed: int = 0, periodic: bool = False) -> Mesh:
+    """Split-quad square with seeded jitter (see module docstring).
+
+    Jitter sampling: for interior node k (row-major order), draw u1, u2 from
+    ``numpy.random.Generator(PCG64(seed)).random((n_interior, 2))`` and move the
+    node by ``jitter*h*sqrt(u1) * (cos 2 pi u2, sin 2 pi u2)``, h = (hi-lo)/nx.
+    """
+    ny = nx if ny is None else ny
+    if nx < 3 or ny < 3:
+        raise ValueError("jittered_square needs at least 3x3 quads")
+    hx, hy = (hi - lo) / nx, (hi - lo) / ny
+    ii, jj = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1))
+    xy = np.stack([lo + ii.ravel() * hx, lo + jj.ravel() * hy], axis=1).astype(np.float64)
+    interior = ((ii > 0) & (ii < nx) & (jj > 0) & (jj < ny)).ravel()
+    idx = np.nonzero(interior)[0]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    u = rng.random((idx.size, 2))
+    r = jitter * min(hx, hy) * np.sqrt(u[:, 0])
+    th = 2.0 * math.pi * u[:, 1]
+    xy[idx, 0] += r * np.cos(th)
+    xy[idx, 1] += r * np.sin(th)
+
+    qi, qj = np.meshgrid(np.arange(nx), np.arange(ny))
+    qi, qj = qi.ravel(), qj.ravel()
+    bl = qj * (nx + 1) + qi
+    br, tl = bl + 1, bl + nx + 1
+    tr = tl + 1
+    flip = np.zeros(nx * ny, bool)
+    if not periodic:
+        flip[(qi == nx - 1) & (qj == 0)] = True
+        flip[(qi == 0) & (qj == ny - 1)] = True
+    t1 = np.where(flip[:, None], np.stack([bl, br, tl], 1), np.stack([bl, br, tr], 1))
+    t2 = np.where(flip[:, None], np.stack([br, tr, tl], 1), np.stack([bl, tr, tl], 1))
+    tri = np.empty((2 * nx * ny, 3), np.int64)
+    tri[0::2] = t1
+    tri[1::2] = t2
+    assert np.all(_tri_area(xy, tri) > 0)
+    cell_nodes = tri.astype(np.int32).ravel()
+
+    if periodic:
+        canon = ((jj % ny) * (nx + 1) + (ii % nx)).ravel().astype(np.int32)
+        return Mesh(xy, 2 * nx * ny, 0, cell_nodes, [], canon=canon, period=(hi - lo, hi - lo))
+
+    # cells with a boundary edge, ascending: an edge lies on the outline when
+    # both of its end nodes sit on the same side of the square
+    side = np.zeros((xy.shape[0], 4), bool)
+    side[:, 0] = (ii == 0).ravel()
+    side[:, 1] = (ii == nx).ravel()
+    side[:, 2] = (jj == 0).ravel()
+    side[:, 3] = (jj == ny).ravel()
+    nbe = np.zeros(tri.shape[0], np.int64)
+    for e in range(3):
+        a, b = tri[:, e], tri[:, (e + 1) % 3]
+        nbe += np.any(side[a] & side[b], axis=1)
+    assert nbe.max() == 1, "boundary cell with two boundary edges"
+    bcells = np.nonzero(nbe)[0].astype(np.int32)
+    return Mesh(xy, 2 * nx * ny, 0, cell_nodes, [("dirichlet", bcells)])
+
+
+def naca_thickness(x: np.ndarray, t: float, closed_te: bool = True) -> np.ndarray:
+    """Half thickness of a symmetric NACA 00tt section (chord 1).
+
+    The closed-trailing-edge variant uses -0.1036 for the x^4 coefficient so
+    that y(1) = 0 (the open form uses -0.1015)."""
+    a4 = -0.1036 if closed_te else -0.1015
+    return 5.0 * t * (0.2969 * np.sqrt(x) - 0.1260 * x - 0.3516 * x ** 2 + 0.2843 * x ** 3 + a4 * x ** 4)
+
+
+def _growth_ratio(first: float, n: int, total: float) -> float:
+    """Geometric ratio r with first*(r^n - 1)/(r - 1) = total (bisection)."""
+    lo, hi = 1.0 + 1e-12, 2.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        s = first * (mid ** n - 1.0) / (mid - 1.0)
+        if s > total:
+            hi = mid
+        else:
+            lo = mid
+    return 0.5 * (lo + hi)
+
+
+def naca_ogrid(n_around: int = 512, n_layers: int = 256, *, thickness: float = 0.12,
+               first_spacing: float = 2e-3, far_radius: float = 50.0) -> Mesh:
+    """O-grid around a symmetric NACA 4-digit airfoil with a closed trailing edge.
+
+    ``n_around`` surface nodes, cosine-clustered in chord (clockwise from the
+    trailing edge over the lower... i.e. ordered so that cells are CCW),
+    ``n_layers`` cell layers.  Layer k (0 at the wall) sits at wall distance
+    ``first*(r^k-1)/(r-1)`` along a blend of the surface normal and the ray to
+    a circle of radius ``far_radius`` around mid-chord; the ratio r is solved
+    so the last layer reaches the far field.  Each quad is split into two
+    triangles.  Boundaries: ``slip_wall`` (cells on the airfoil) and
+    ``freestream`` (cells on the outer circle), each cell with one boundary
+    edge."""
+    m = n_around
+    # surface: parameter s in [0, 2pi): x = 0.5(1+cos s); upper for s in (0,pi)
+    s = 2.0 * math.pi * np.arange(m) / m
+    xs = 0.5 * (1.0 + np.cos(s))
+    ys = naca_thickness(xs, thickness) * np.where(s <= math.pi, 1.0, -1.0)
+    ys[0] = 0.0
+    # counter-clockwise circle around mid-chord matching each surface point
+    xc0 = 0.5
+    ang = np.arctan2(ys, xs - xc0)
+    ang = np.unwrap(ang)
+    r_lay = _growth_ratio(first_spacing, n_layers, far_radius)
+    dist = first_spacing * (r_lay ** np.arange(n_layers + 1) - 1.0) / (r_lay - 1.0)
+    # surface normals (outward) from central differences along the surface
+    dx = np.roll(xs, -1) - np.roll(xs, 1)
+    dy = np.roll(ys, -1) - np.roll(ys, 1)
+    ln = np.hypot(dx, dy)
+    nxs, nys = dy / ln, -dx / ln  # surface runs CCW (upper first, TE->LE): outward = right normal
+    # make sure normals point away from the chord centre
+    sgn = np.sign((xs - xc0) * nxs + ys * nys)
+    sgn[sgn == 0] = 1.0
+    nxs, nys = nxs * sgn, nys * sgn
+    xy = np.empty(((n_layers + 1) * m, 2))
+    for k in range(n_layers + 1):
+        d = dist[k]
+        blend = min(1.0, d / far_radius) ** 0.5
+        # normal-extruded point and far-circle point
+        px = xs + d * nxs
+        py = ys + d * nys
+        fx = xc0 + (far_radius * (d / far_radius)) * np.cos(ang) if k else xs
+        fy = (far_radius * (d / far_radius)) * np.sin(ang) if k else ys
+        if k == n_layers:
+            px, py = xc0 + far_radius * np.cos(ang), far_radius * np.sin(ang)
+            blend = 1.0
+        xy[k * m:(k + 1) * m, 0] = (1 - blend) * px + blend * fx
+        xy[k * m:(k + 1) * m, 1] = (1 - blend) * py + blend * fy
+    # quads (k, i) -> (k, i+1) -> (k+1, i+1) -> (k+1, i); orientation fixed below
+    tris = []
+    kk, ii = np.meshgrid(np.arange(n_layers), np.arange(m), indexing="ij")
+    kk, ii = kk.ravel(), ii.ravel()
+    a = kk * m + ii
+    b = kk * m + (ii + 1) % m
+    c = (kk + 1) * m + (ii + 1) % m
+    d = (kk + 1) * m + ii
+    t1 = np.stack([a, b, c], 1)
+    t2 = np.stack([a, c, d], 1)
+    tri = np.empty((2 * a.size, 3), np.int64)
+    tri[0::2] = t1
+    tri[1::2] = t2
+    area = _tri_area(xy, tri)
+    if np.all(area < 0):
+        tri = tri[:, ::-1].copy()
+        area = -area
+    if not np.all(area > 0):
+        raise RuntimeError("O-grid has inverted cells")
+    # boundary cells: wall edges are (a,b) at k=0 -> triangle t1 of layer 0;
+    # far-field edges (c,d) at k=n_layers-1 -> triangle t2 of the last layer
+    ncq = m
+    wall = np.arange(0, 2 * ncq, 2, dtype=np.int32)
+    far = (2 * (n_layers - 1) * m + 1 + np.arange(0, 2 * ncq, 2)).astype(np.int32)
+    return Mesh(xy, tri.shape[0], 0, tri.astype(np.int32).ravel(),
+                [("slip_wall", wall), ("freestream", far)])
+
+
+# ---------------------------------------------------------------------------
+# reference text format: src/grid_procs.f90:65-164
+# ---------------------------------------------------------------------------
+def write_grid(mesh: Mesh, base: str) -> None:
+    """Write ``base.grid`` and ``base.bc`` (1-based ids)."""
+    cells = mesh.cells_as_array()
+    with open(base + ".grid", "w") as f:
+        f.write(" #nodes, #cells_tri, #cells_quad\n")
+        f.write(f"{mesh.nnodes} {mesh.ntri} {mesh.nquad}\n")
+        for x, y in mesh.xy:
+            f.write(f"{x!r} {y!r}\n")
+        for c in cells[: mesh.ntri]:
+            f.write(f"{c[0] + 1} {c[1] + 1} {c[2] + 1}\n")
+        for c in cells[mesh.ntri:]:
+            f.write(f"{c[0] + 1} {c[1] + 1} {c[2] + 1} {c[3] + 1}\n")
+    with open(base + ".bc", "w") as f:
+        f.write(f"{len(mesh.boundaries)}\n")
+        for name, c in mesh.boundaries:
+            f.write(f"{len(c)} {name}\n")
+        for _, c in mesh.boundaries:
+            for v in c:
+                f.write(f"{int(v) + 1}\n")
+
+
+def read_grid(base: str) -> Mesh:
+    """Read ``base.grid`` / ``base.bc`` written in the reference's format."""
+    with open(base + ".grid") as f:
+        f.readline()
+        nn, nt, nq = (int(v) for v in f.readline().split()[:3])
+        xy = np.array([[float(v.replace("d", "e").replace("D", "e")) for v in f.readline().split()[:2]]
+                       for _ in range(nn)])
+        tri = [[int(v) - 1 for v in f.readline().split()[:3]] for _ in range(nt)]
+        quad = [[int(v) - 1 for v in f.readline().split()[:4]] for _ in range(nq)]
+    cell_nodes = np.array([v for c in tri for v in c] + [v for c in quad for v in c], np.int32)
+    bnds = []
+    with open(base + ".bc") as f:
+        nb = int(f.readline().split()[0])
+        heads = []
+        for _ in range(nb):
+            parts = f.readline().split()
+            heads.append((int(parts[0]), parts[1]))
+        for n, name in heads:
+            cells = np.array([int(f.readline().split()[0]) - 1 for _ in range(n)], np.int32)
+            bnds.append((name, cells))
+    return Mesh(xy, nt, nq, cell_nodes, bnds)
+
+
+def concat(meshes: Sequence[Mesh]) -> Tuple[Mesh, np.ndarray, np.ndarray]:
+    """Concatenate independent meshes into one ragged batch.
+
+    Returns the batch mesh and the per-instance cell and node offset tables
+    (length n+1).  Triangles of all instances come first, then quads, so the
+    batch keeps the reference's tri-before-quad numbering only when no
+    instance has quads (the batched solver only uses triangle meshes)."""
+    if any(m.nquad for m in meshes):
+        raise ValueError("concat supports triangle meshes only")
+    coff = np.zeros(len(meshes) + 1, np.int64)
+    noff = np.zeros(len(meshes) + 1, np.int64)
+    for i, m in enumerate(meshes):
+        coff[i + 1] = coff[i] + m.ncells
+        noff[i + 1] = noff[i] + m.nnodes
+    xy = np.concatenate([m.xy for m in meshes])
+    cn = np.concatenate([m.cell_nodes.astype(np.int64) + noff[i] for i, m in enumerate(meshes)]).astype(np.int32)
+    names = []
+    for m in meshes:
+        for n, _ in m.boundaries:
+            if n not in names:
+                names.append(n)
+    bnds = []
+    for n in names:
+        cells = [np.asarray(c, np.int64) + coff[i] for i, m in enumerate(meshes) for nm, c in m.boundaries if nm == n]
+        bnds.append((n, np.concatenate(cells).astype(np.int32)))
+    return Mesh(xy, int(coff[-1]), 0, cn, bnds), coff, noff

@@ -1,0 +1,335 @@
+"""Python mirror of the reference's module interface for the residual /
+time-step path, over the C-ABI (include/fvs2d.h).
+
+Method names follow the Fortran procedures they replace:
+
+====================================  ===========================================
+reference procedure                   here
+====================================  ===========================================
+grid_procs_init + gradient_init +     ``Solver(mesh, scheme)``
+runge_kutta_init + data_solution_init
+initialize_solution                   ``Solver.initialize_solution(mode, t0)``
+cvar2pvar                             ``Solver.cvar2pvar()``
+compute_gradient_cellcntr             ``Solver.compute_gradient_cellcntr()``
+gradient_cellcntr_1var                ``Solver.gradient_cellcntr_1var(fc)``
+compute_gradient_limiter              ``Solver.compute_gradient_limiter()``
+compute_residual                      ``Solver.compute_residual(time)``
+compute_flux_invscid                  :func:`compute_flux_invscid`
+compute_local_time                    ``Solver.compute_local_time()``
+time_integration                      ``Solver.time_integration(t1, n)``
+error_isentropic_vortex               ``Solver.error_isentropic_vortex(t)``
+write_inst_cp_un (C_L, C_D)           ``Solver.wall_forces()``
+====================================  ===========================================
+
+Arrays use the Fortran layouts transposed to C order: ``cvar``, ``pvar`` and
+``resid`` are ``(ncells, 4)``, ``grad`` is ``(2, ncells, 4)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import FvsError, check, ptr
+from .meshgen import Mesh
+
+
+@dataclasses.dataclass
+class Scheme:
+    """Scheme flags with src/input.f90 semantics (see fvs2d_options)."""
+    grad_method: int = 3      # 1 GGCB, 2 GGNB, 3 LSQ
+    lsq_nn: int = 1           # 1 "nn", 0 "fn"
+    lsq_p: float = 0.0
+    limiter: int = 0          # 0 none, 1 venk, 2 barth, 3 albada
+    recon: int = 2            # 1 upwind-1st, 2 upwind-2nd, 3 UMUSCL
+    kappa: float = 0.0
+    gamma: float = 1.4
+    mach: float = 0.5
+    aoa_deg: float = 0.0
+    lvortex: int = 0
+    vortex_x0: float = 0.0
+    vortex_y0: float = 0.0
+    vortex_K: float = 5.0
+    vortex_inf: Tuple[float, float, float, float] = (1.0, 1.0, 1.0, 1.0)
+    rk_scheme: int = _lib.RK_SSP3
+    rk_order: int = 4
+    steady: int = 0
+    dt: float = 0.01
+    cfl: float = 1.0
+    reorder: int = 0
+
+    def to_c(self) -> _lib.Options:
+        o = _lib.Options()
+        for f in dataclasses.fields(self):
+            v = getattr(self, f.name)
+            if f.name == "vortex_inf":
+                o.vortex_inf = (C.c_double * 4)(*v)
+            else:
+                setattr(o, f.name, v)
+        return o
+
+    def oracle_kwargs(self) -> dict:
+        d = dataclasses.asdict(self)
+        d.pop("reorder")
+        return d
+
+
+def read_input(path: str) -> Tuple[str, Scheme, dict]:
+    """Read a reference ``fvs2d.input`` (fixed line order, src/input.f90:86-120).
+
+    Returns the grid base name, the scheme and the run control values
+    (dt, ntimes, nsaves, ntstart)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+
+    def val(i, k=0):
+        return lines[i].replace(",", " ").split()[k]
+
+    def flt(s):
+        return float(s.lower().replace("d", "e"))
+
+    def flag(s):
+        return s.strip().upper().startswith("T")
+
+    base = val(3).split("-")[0]
+    sch = Scheme()
+    sch.mach = flt(val(5))
+    sch.aoa_deg = flt(val(6))
+    sch.gamma = flt(val(7))
+    run = dict(dt=flt(val(9)), ntimes=int(val(10)), nsaves=int(val(11)), ntstart=int(val(12)))
+    sch.dt = run["dt"]
+    sch.steady = int(flag(val(13, 0)))
+    sch.cfl = flt(val(13, 1))
+    sch.lvortex = int(flag(val(14)))
+    if run["ntstart"] == 0:
+        sch.lvortex = 0  # src/input.f90:140
+    sch.grad_method = int(val(21, 0))
+    sch.lsq_nn = int(val(21, 1).lower() == "nn")
+    sch.lsq_p = flt(val(21, 2))
+    sch.limiter = int(val(22))
+    sch.recon = int(val(23, 0))
+    sch.kappa = flt(val(23, 1)) if sch.recon == 3 else 0.0
+    if int(val(24)) != 1:
+        raise ValueError("only the Roe flux (1) exists")
+    if int(val(26)) != 4:
+        raise ValueError("the reference codes 4-stage Runge-Kutta only")
+    sch.rk_order = int(val(27))
+    ssp = flag(val(28))
+    sch.rk_scheme = _lib.RK_SSP42 if (ssp and sch.rk_order == 2) else _lib.RK_CLASSIC
+    return base, sch, run
+
+
+def read_vortex(path: str, sch: Scheme) -> Scheme:
+    """Read ``fvs2d.vortex`` (src/mms.f90:55-60) into the scheme."""
+    with open(path) as f:
+        v = [ln.replace(",", " ").split() for ln in f.read().splitlines()]
+    flt = lambda s: float(s.lower().replace("d", "e"))
+    sch.vortex_x0, sch.vortex_y0 = flt(v[0][0]), flt(v[0][1])
+    sch.vortex_K = flt(v[1][0])
+    sch.vortex_inf = tuple(flt(v[k][0]) for k in range(2, 6))
+    return sch
+
+
+def _mesh_arrays(mesh: Mesh):
+    nb, bt, bc = mesh.bc_arrays()
+    keep = [np.ascontiguousarray(mesh.xy, np.float64), np.ascontiguousarray(mesh.cell_nodes, np.int32),
+            np.ascontiguousarray(nb), np.ascontiguousarray(bt), np.ascontiguousarray(bc)]
+    canon = None if mesh.canon is None else np.ascontiguousarray(mesh.canon, np.int32)
+    keep.append(canon)
+    return keep
+
+
+class HostMesh:
+    """Host-only topology / stencils (fvs_mesh_*), no device needed."""
+
+    def __init__(self, mesh: Mesh, grad_method=3, lsq_nn=1, lsq_p=0.0, limiter=0):
+        L = _lib.load()
+        xy, cn, nb, bt, bc, canon = _mesh_arrays(mesh)
+        h = C.c_void_p()
+        check(L.fvs_mesh_create(C.byref(h), mesh.nnodes, ptr(xy), mesh.ntri, mesh.nquad, ptr(cn), ptr(canon),
+                                len(nb), ptr(nb), ptr(bt), ptr(bc), grad_method, lsq_nn, lsq_p, limiter))
+        self._h, self._owned = h, True
+        self._L = L
+        self._load_sizes()
+
+    @classmethod
+    def _view(cls, L, h):
+        obj = cls.__new__(cls)
+        obj._h, obj._owned, obj._L = C.c_void_p(h), False, L
+        obj._load_sizes()
+        return obj
+
+    def _load_sizes(self):
+        s = np.zeros(8, np.int32)
+        check(self._L.fvs_mesh_sizes(self._h, ptr(s)))
+        (self.ncells, self.nedges, self.nnodes, self.nedges_intr, self.nedges_bndr, self.ncells_intr,
+         self.ncells_bndr, self.nb) = (int(v) for v in s)
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and self._h:
+            self._L.fvs_mesh_destroy(self._h)
+            self._h = None
+
+    def edges(self):
+        E = self.nedges
+        n1, n2, c1, c2 = (np.zeros(E, np.int32) for _ in range(4))
+        g = np.zeros(7 * E)
+        check(self._L.fvs_mesh_topology(self._h, ptr(n1), ptr(n2), ptr(c1), ptr(c2), ptr(g), None, None, None))
+        g = g.reshape(7, E)
+        return dict(n1=n1, n2=n2, c1=c1, c2=c2, x=g[0], y=g[1], area=g[2], nx=g[3], ny=g[4], sx=g[5], sy=g[6])
+
+    def cells(self):
+        N = self.ncells
+        xyv = np.zeros(3 * N)
+        nghbre = np.zeros(4 * N, np.int32)
+        edge = np.zeros(4 * N, np.int32)
+        check(self._L.fvs_mesh_topology(self._h, None, None, None, None, None, ptr(xyv), ptr(nghbre), ptr(edge)))
+        xyv = xyv.reshape(3, N)
+        return dict(x=xyv[0], y=xyv[1], vol=xyv[2], nghbre=nghbre.reshape(N, 4), edge=edge.reshape(N, 4))
+
+    def lists(self):
+        ei = np.zeros(max(self.nedges_intr, 1), np.int32)
+        ci = np.zeros(max(self.ncells_intr, 1), np.int32)
+        bn = np.zeros(max(self.nb, 1), np.int32)
+        be = np.zeros(max(self.nedges_bndr, 1), np.int32)
+        check(self._L.fvs_mesh_lists(self._h, ptr(ei), ptr(ci), ptr(bn), ptr(be)))
+        return dict(edge_intr=ei[: self.nedges_intr], cell_intr=ci[: self.ncells_intr], bnd_nedges=bn[: self.nb],
+                    bnd_edges=be[: self.nedges_bndr])
+
+    def lsq(self):
+        nnz = self._L.fvs_mesh_lsq_nnz(self._h)
+        p = np.zeros(self.ncells + 1, np.int32)
+        cell = np.zeros(max(nnz, 1), np.int32)
+        w = np.zeros(max(nnz, 1))
+        coef = np.zeros(max(2 * nnz, 2))
+        check(self._L.fvs_mesh_lsq(self._h, ptr(p), ptr(cell), ptr(w), ptr(coef)))
+        return dict(ptr=p, cell=cell[:nnz], w=w[:nnz], coef=coef[: 2 * nnz].reshape(-1, 2))
+
+
+class Solver:
+    """One mesh + scheme on the GPU (an fvs_ctx)."""
+
+    def __init__(self, mesh: Mesh, scheme: Optional[Scheme] = None, **overrides):
+        self.scheme = dataclasses.replace(scheme or Scheme(), **overrides)
+        L = _lib.load()
+        self._L = L
+        xy, cn, nb, bt, bc, canon = _mesh_arrays(mesh)
+        self._opts = self.scheme.to_c()
+        h = C.c_void_p()
+        check(L.fvs_create(C.byref(h), mesh.nnodes, ptr(xy), mesh.ntri, mesh.nquad, ptr(cn), ptr(canon), len(nb),
+                           ptr(nb), ptr(bt), ptr(bc), C.byref(self._opts)))
+        self._h = h
+        s = np.zeros(8, np.int32)
+        check(L.fvs_sizes(h, ptr(s)))
+        (self.ncells, self.nedges, self.nnodes, self.nedges_intr, self.nedges_bndr, self.ncells_intr,
+         self.ncells_bndr, self.nb) = (int(v) for v in s)
+        self.mesh = HostMesh._view(L, L.fvs_ctx_mesh(h))
+        self.ws_nrml = None
+        self.resid = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.fvs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- state ------------------------------------------------------------
+    @property
+    def cvar(self) -> np.ndarray:
+        out = np.zeros((self.ncells, 4))
+        check(self._L.fvs_get_state(self._h, ptr(out)))
+        return out
+
+    @cvar.setter
+    def cvar(self, v):
+        a = np.ascontiguousarray(v, np.float64).reshape(self.ncells, 4)
+        check(self._L.fvs_set_state(self._h, ptr(a)))
+
+    def initialize_solution(self, mode: int = 1, t0: float = 0.0):
+        check(self._L.fvs_initialize(self._h, mode, t0))
+
+    # -- operators --------------------------------------------------------
+    def cvar2pvar(self) -> np.ndarray:
+        out = np.zeros((self.ncells, 4))
+        check(self._L.fvs_primitives(self._h, ptr(out)))
+        return out
+
+    def compute_gradient_cellcntr(self) -> np.ndarray:
+        out = np.zeros((2, self.ncells, 4))
+        check(self._L.fvs_gradient(self._h, ptr(out)))
+        return out
+
+    def gradient_cellcntr_1var(self, fc) -> np.ndarray:
+        fc = np.ascontiguousarray(fc, np.float64)
+        out = np.zeros((2, self.ncells))
+        check(self._L.fvs_gradient_1var(self._h, ptr(fc), ptr(out)))
+        return out
+
+    def compute_gradient_limiter(self) -> np.ndarray:
+        out = np.zeros(self.ncells)
+        check(self._L.fvs_limiter(self._h, ptr(out)))
+        return out
+
+    def compute_residual(self, time: float = 0.0):
+        r = np.zeros((self.ncells, 4))
+        w = np.zeros(self.ncells)
+        check(self._L.fvs_compute_residual(self._h, time, ptr(r), ptr(w)))
+        self.resid, self.ws_nrml = r, w
+        return r, w
+
+    def compute_local_time(self) -> np.ndarray:
+        out = np.zeros(self.ncells)
+        check(self._L.fvs_local_dt(self._h, ptr(out)))
+        return out
+
+    def time_integration(self, t1: float, nsteps: int, vortex_error: bool = False):
+        norms = np.zeros((nsteps, 4))
+        verr = np.zeros((nsteps, 13)) if vortex_error else None
+        check(self._L.fvs_time_integration(self._h, t1, nsteps, ptr(norms), ptr(verr)))
+        return (norms, verr) if vortex_error else norms
+
+    def error_isentropic_vortex(self, t: float) -> np.ndarray:
+        out = np.zeros(13)
+        check(self._L.fvs_vortex_error(self._h, t, ptr(out)))
+        return out
+
+    def wall_forces(self) -> np.ndarray:
+        out = np.zeros(4)
+        check(self._L.fvs_wall_forces(self._h, ptr(out)))
+        return out
+
+    def set_timestep(self, dt: float, cfl: Optional[float] = None):
+        cfl = self.scheme.cfl if cfl is None else cfl
+        check(self._L.fvs_set_timestep(self._h, dt, cfl))
+        self.scheme.dt, self.scheme.cfl = dt, cfl
+
+    KERNEL_CLASSES = ("gradient", "flux_interior", "flux_boundary", "gather_update", "norm_reduce")
+
+    def kernel_times(self, t1: float, nsteps: int):
+        """Per kernel class: (mean ms per launch, algorithmic bytes per launch,
+        launches) over nsteps steps (the state advances)."""
+        ms, by = np.zeros(5), np.zeros(5)
+        n = np.zeros(5, np.int32)
+        check(self._L.fvs_kernel_times(self._h, t1, nsteps, ptr(ms), ptr(by), ptr(n)))
+        return {k: (float(ms[i]), float(by[i]), int(n[i])) for i, k in enumerate(self.KERNEL_CLASSES)}
+
+    @property
+    def last_elapsed_ms(self) -> float:
+        return float(self._L.fvs_last_elapsed_ms(self._h))
+
+
+def compute_flux_invscid(pL, pR, nx, ny, gamma=1.4):
+    """Roe flux of n face states on the device (src/flux_invscid.f90:17-32)."""
+    pL = np.ascontiguousarray(np.atleast_2d(pL), np.float64)
+    pR = np.ascontiguousarray(np.atleast_2d(pR), np.float64)
+    n = pL.shape[0]
+    nxy = np.ascontiguousarray(np.stack([np.broadcast_to(nx, n), np.broadcast_to(ny, n)], 1), np.float64)
+    f = np.zeros((n, 4))
+    w = np.zeros(n)
+    check(_lib.load().fvs_roe_flux_batch(n, gamma, ptr(pL), ptr(pR), ptr(nxy), ptr(f), ptr(w)))
+    return f, w

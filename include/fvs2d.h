@@ -1,0 +1,161 @@
+/*
+ * fvs2d.h -- C-ABI of the MI355X-native fvs2d residual / time-step path.
+ *
+ * The reference (shirzadgit/fvs2d) exposes this path as Fortran module
+ * procedures working on module-global arrays; a Fortran driver reaches this
+ * library through the iso_c_binding modules in fvs2d_amd/fortran/, which keep
+ * the reference's module and procedure names.  Each entry point below names
+ * the reference procedure it replaces.
+ *
+ * Conventions
+ *   - plain pointers and sizes only; all reals are double, all ints int32;
+ *   - cell, node and edge ids are 0-based (the Fortran side subtracts 1);
+ *   - per-cell arrays are in the caller's (reference) cell order; the library
+ *     may store cells in another order on the device and permutes at the
+ *     boundary;
+ *   - cvar / pvar / resid are (4, ncells) column-major, i.e. C double[N][4]
+ *     like the Fortran cvar(4,N); grad is double[2][N][4] like grad(4,N,2);
+ *   - every function returns 0 on success and a negative code on error; the
+ *     message is then available from fvs_last_error();
+ *   - one context owns one HIP stream and is not re-entrant; calls on one
+ *     context must come from one host thread at a time.
+ */
+#ifndef FVS2D_H
+#define FVS2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fvs_ctx fvs_ctx;
+
+/* boundary types: src/residual.f90:194-221 */
+enum { FVS_BC_FREESTREAM = 0, FVS_BC_SLIP_WALL = 1, FVS_BC_DIRICHLET = 2, FVS_BC_SOLID_WALL = 3 };
+
+/* time integrators: src/runge_kutta.f90:94-114; FVS_RK_SSP3 has no reference
+ * counterpart (Shu-Osher SSP-RK3) */
+enum { FVS_RK_CLASSIC = 0, FVS_RK_SSP42 = 1, FVS_RK_SSP3 = 2 };
+
+/* error codes */
+enum { FVS_OK = 0, FVS_EINVAL = -1, FVS_EHIP = -2, FVS_ESTATE = -3, FVS_ENOMEM = -4 };
+
+/* Scheme flags with the meaning src/input.f90:181-277 gives them. */
+typedef struct {
+  int grad_method;      /* 1 GGCB, 2 GGNB, 3 LSQ                                */
+  int lsq_nn;           /* LSQ stencil: 1 node neighbours ("nn"), 0 face ("fn") */
+  double lsq_p;         /* LSQ weight power: w = 1/d^p                          */
+  int limiter;          /* 0 none, 1 Venkatakrishnan, 2 Barth-Jespersen, 3 Van Albada */
+  int recon;            /* 1 upwind-1st, 2 upwind-2nd, 3 UMUSCL                 */
+  double kappa;         /* UMUSCL constant (ignored unless recon == 3)          */
+  double gamma, mach, aoa_deg;
+  int lvortex;          /* Dirichlet boundaries carry the isentropic vortex (else MMS) */
+  double vortex_x0, vortex_y0, vortex_K, vortex_inf[4]; /* fvs2d.vortex contents */
+  int rk_scheme;        /* FVS_RK_*                                             */
+  int rk_order;         /* 1..4, FVS_RK_CLASSIC only                            */
+  int steady;           /* local time stepping                                  */
+  double dt, cfl;
+  int reorder;          /* device cell order: 0 reference order, 1 Morton blocks */
+} fvs_options;
+
+const char *fvs_last_error(void);
+
+/* Mesh + scheme set-up: replaces grid_procs_init's topology part
+ * (src/grid_procs.f90:31-59, 170-794), data_solution_init
+ * (src/data_solution.f90:41-66), gradient_init (src/gradient.f90:18-38) and
+ * runge_kutta_init (src/runge_kutta.f90:25-88).
+ *   xy          2*nnodes node coordinates
+ *   cell_nodes  ntri*3 then nquad*4 node ids, counter-clockwise
+ *   node_canon  NULL, or per node the id it is identified with (periodic
+ *               seams; no reference counterpart)
+ *   bnd_*       the .bc file: per boundary its cell count and type, then the
+ *               concatenated cell lists */
+int fvs_create(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
+               const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types,
+               const int *bnd_cells, const fvs_options *opt);
+/* replaces data_sol_close / data_grid_close (src/data_solution.f90:130-140) */
+int fvs_destroy(fvs_ctx *ctx);
+
+/* [ncells, nedges, nnodes, nedges_intr, nedges_bndr, ncells_intr, ncells_bndr, nbndries] */
+int fvs_sizes(const fvs_ctx *ctx, int *out8);
+
+/* Host-only mesh preparation (no device needed): the topology, geometry and
+ * stencils fvs_create builds, for callers that want the reference's grid data
+ * (grid_procs, src/grid_procs.f90:170-794; stencil set-ups
+ * src/gradient_lsq.f90:70-365) without a solver context. */
+typedef struct fvs_mesh fvs_mesh;
+int fvs_mesh_create(fvs_mesh **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
+                    const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types,
+                    const int *bnd_cells, int grad_method, int lsq_nn, double lsq_p, int limiter);
+int fvs_mesh_destroy(fvs_mesh *mesh);
+/* the mesh a solver context was built on (owned by the context) */
+const fvs_mesh *fvs_ctx_mesh(const fvs_ctx *ctx);
+int fvs_mesh_sizes(const fvs_mesh *mesh, int *out8);
+/* Topology as grid_data builds it (src/grid_procs.f90:401-791), reference
+ * numbering: edges n1,n2,c1,c2 (c2 = -1 on boundaries) and geometry
+ * x,y,area,nx,ny,sx,sy (7*nedges; sx,sy = periodic shift of c2, else 0);
+ * cells x,y,vol (3*ncells), nghbre and edge (4*ncells, -1 padded).  Any
+ * output may be NULL. */
+int fvs_mesh_topology(const fvs_mesh *mesh, int *n1, int *n2, int *c1, int *c2, double *edge_geom7,
+                      double *cell_xyv3, int *nghbre4, int *cell_edge4);
+/* interior edge list (ascending), interior cell list, per-boundary edge
+ * counts and the concatenated boundary edge lists (src/grid_procs.f90:697-791) */
+int fvs_mesh_lists(const fvs_mesh *mesh, int *edge_intr, int *cell_intr, int *bnd_nedges, int *bnd_edges);
+/* LSQ stencil (src/gradient_lsq.f90:16-365) in CSR form: ptr[ncells+1]; the
+ * caller sizes cell/w/coef from fvs_mesh_lsq_nnz; coef holds 2 per entry. */
+int fvs_mesh_lsq_nnz(const fvs_mesh *mesh);
+int fvs_mesh_lsq(const fvs_mesh *mesh, int *ptr, int *cell, double *w, double *coef);
+
+/* cvar in / out (module array data_solution::cvar) */
+int fvs_set_state(fvs_ctx *ctx, const double *cvar);
+int fvs_get_state(fvs_ctx *ctx, double *cvar);
+/* initialize_solution (src/initialize.f90:19-90): mode 1 free stream
+ * (+ vortex at t0 if lvortex), mode 0 manufactured solution */
+int fvs_initialize(fvs_ctx *ctx, int mode, double t0);
+
+/* cvar2pvar (src/data_solution.f90:72-86); pvar may be NULL */
+int fvs_primitives(fvs_ctx *ctx, double *pvar);
+/* compute_gradient_cellcntr (src/gradient.f90:44-69) on the current pvar;
+ * grad may be NULL */
+int fvs_gradient(fvs_ctx *ctx, double *grad);
+/* gradient_cellcntr_1var (src/gradient.f90:75-96): dfc is double[2][N] */
+int fvs_gradient_1var(fvs_ctx *ctx, const double *fc, double *dfc);
+/* compute_gradient_limiter (src/gradient_limiter.f90:19-97) on the current
+ * pvar and grad; phi may be NULL */
+int fvs_limiter(fvs_ctx *ctx, double *phi);
+/* compute_residual (src/residual.f90:23-177): resid (dQ/dt) and ws_nrml;
+ * either output may be NULL */
+int fvs_compute_residual(fvs_ctx *ctx, double time, double *resid, double *ws_nrml);
+/* compute_flux_invscid (src/flux_invscid.f90:17-32) on n face states:
+ * pL,pR 4*n primitive states, nxy 2*n unit normals; flux 4*n, ws n */
+int fvs_roe_flux_batch(int n, double gamma, const double *pL, const double *pR, const double *nxy,
+                       double *flux, double *ws);
+/* compute_local_time (src/runge_kutta.f90:424-437) from the last residual's
+ * ws_nrml; dt may be NULL */
+int fvs_local_dt(fvs_ctx *ctx, double *dt);
+/* time_integration (src/runge_kutta.f90:94-114): nsteps steps from t1.
+ * norms (nsteps*4, may be NULL) receives the log_res.plt norms
+ * (src/runge_kutta.f90:169-184); verr (nsteps*13, may be NULL) the
+ * error_isentropic_vortex record (src/mms.f90:351-361) after each step. */
+int fvs_time_integration(fvs_ctx *ctx, double t1, int nsteps, double *norms, double *verr);
+/* error_isentropic_vortex (src/mms.f90:271-365) on the current state */
+int fvs_vortex_error(fvs_ctx *ctx, double t, double *out13);
+/* write_inst_cp_un C_L/C_D (src/io.f90:340-449), summed over slip/solid wall
+ * boundaries: out4 = cl, cd, cl1, cd1 */
+int fvs_wall_forces(fvs_ctx *ctx, double *out4);
+/* Change dt / CFL after creation (the reference reads them from fvs2d.input,
+ * module input) */
+int fvs_set_timestep(fvs_ctx *ctx, double dt, double cfl);
+/* Measurement helper, no reference counterpart: runs nsteps steps like
+ * fvs_time_integration (advancing the state) with HIP events around every
+ * launch and returns, per kernel class [gradient(+limiter), interior-face
+ * flux, boundary-face flux, gather+update, norm reduction], the mean device
+ * time per launch (ms), the algorithmic bytes per launch and the number of
+ * launches timed. */
+int fvs_kernel_times(fvs_ctx *ctx, double t1, int nsteps, double *ms5, double *bytes5, int *launches5);
+/* wall-clock of the last fvs_time_integration call's device work (ms) */
+double fvs_last_elapsed_ms(const fvs_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
