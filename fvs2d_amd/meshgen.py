@@ -238,7 +238,7 @@ def write_grid(mesh: Mesh, base: str) -> None:
         f.write(" #nodes, #cells_tri, #cells_quad\n")
         f.write(f"{mesh.nnodes} {mesh.ntri} {mesh.nquad}\n")
         for x, y in mesh.xy:
-            f.write(f"{x!r} {y!r}\n")
+            f.write(f"{float(x)!r} {float(y)!r}\n")
         for c in cells[: mesh.ntri]:
             f.write(f"{c[0] + 1} {c[1] + 1} {c[2] + 1}\n")
         for c in cells[mesh.ntri:]:
