@@ -43,11 +43,11 @@ def build_case(n: int, periodic: bool = True):
     return meshgen.jittered_square(n, periodic=periodic, seed=0)
 
 
-def scheme_for(grad_method=3, limiter=0, dt=0.01):
+def scheme_for(grad_method=3, limiter=0, dt=0.01, reorder=1, path=0):
     from fvs2d_amd.solver import Scheme
     return Scheme(grad_method=grad_method, lsq_nn=1, lsq_p=0.0, limiter=limiter, recon=2, gamma=1.4, mach=0.5,
                   lvortex=1, vortex_x0=0.0, vortex_y0=0.0, vortex_K=5.0, vortex_inf=(1.0, 1.0, 1.0, 1.0),
-                  rk_scheme=2, dt=dt, cfl=0.5, reorder=0)
+                  rk_scheme=2, dt=dt, cfl=0.5, reorder=reorder, kernel_path=path)
 
 
 def timed_run(solver, steps: int, warmup: int):
@@ -74,12 +74,14 @@ def cpu_baseline(mesh, dt: float, steps: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1024, help="quads per side (1024 = 2,097,152 triangles)")
     ap.add_argument("--variants", type=int, default=1, help="also time GGCB/GGNB x limiter on/off")
     ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--reorder", type=int, default=1, help="device cell order: 1 Morton, 0 reference")
+    ap.add_argument("--path", type=int, default=0, help="0 fused LDS kernels, 1 face-flux buffer kernels")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -92,13 +94,16 @@ def main():
     mesh = build_case(args.n)
     N = mesh.ncells
     # dt from the reference's local-step formula on the initial state
-    s0 = Solver(mesh, scheme_for(dt=1.0))
+    s0 = Solver(mesh, scheme_for(dt=1.0, reorder=args.reorder, path=args.path))
     s0.initialize_solution(1, 0.0)
     dt = float(np.min(s0.compute_local_time()))
     s0.set_timestep(dt)
     setup_s = time.perf_counter() - t_setup
     print(f"[bench] mesh {N} cells, dt={dt:.6e}, setup {setup_s:.1f}s", file=sys.stderr, flush=True)
 
+    s0.initialize_solution(1, 0.0)
+    tuned, tuned_ms = s0.autotune(10)
+    print(f"[bench] autotune: {tuned} ({tuned_ms:.4f} ms/step)", file=sys.stderr, flush=True)
     wall, dev = timed_run(s0, args.steps, args.warmup)
     stages = 3
     value = N * stages * args.steps / wall
@@ -124,7 +129,10 @@ def main():
                 if gm == 3 and lim == 0:
                     variants["lsq_nn"] = round(value, 1)
                     continue
-                sv = Solver(mesh, scheme_for(grad_method=gm, limiter=lim, dt=dt))
+                sv = Solver(mesh, scheme_for(grad_method=gm, limiter=lim, dt=dt, reorder=args.reorder,
+                                             path=args.path))
+                sv.initialize_solution(1, 0.0)
+                sv.autotune(5)
                 w, _ = timed_run(sv, max(5, args.steps // 4), 1)
                 key = name + ("_venk" if lim else "")
                 variants[key] = round(N * stages * max(5, args.steps // 4) / w, 1)
@@ -154,7 +162,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"isentropic vortex, periodic jittered square {args.n}x{args.n} quads -> {N} "
                                "triangles, Roe + unlimited LSQ-nn gradients, upwind-2nd, SSP-RK3, CFL 0.5",
-                   "cells": N, "stages_per_step": stages, "dt": dt},
+                   "cell_order": "morton" if args.reorder else "reference",
+                   "kernels": "fused" if args.path == 0 else "face-buffer",
+                   "cells": N, "stages_per_step": stages, "dt": dt, "xcd_numbering": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom},
         "model_bytes_per_cell_stage": model_b,

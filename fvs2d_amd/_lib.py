@@ -23,7 +23,8 @@ EXPORTS = (
     "fvs_ctx_mesh", "fvs_mesh_sizes", "fvs_mesh_topology", "fvs_mesh_lists", "fvs_mesh_lsq_nnz", "fvs_mesh_lsq",
     "fvs_set_state", "fvs_get_state", "fvs_initialize", "fvs_primitives", "fvs_gradient", "fvs_gradient_1var",
     "fvs_limiter", "fvs_compute_residual", "fvs_roe_flux_batch", "fvs_local_dt", "fvs_time_integration",
-    "fvs_vortex_error", "fvs_wall_forces", "fvs_set_timestep", "fvs_kernel_times", "fvs_last_elapsed_ms",
+    "fvs_vortex_error", "fvs_wall_forces", "fvs_set_timestep", "fvs_kernel_times", "fvs_autotune",
+    "fvs_last_elapsed_ms",
 )
 
 
@@ -40,7 +41,7 @@ class Options(C.Structure):
         ("lvortex", C.c_int), ("vortex_x0", C.c_double), ("vortex_y0", C.c_double),
         ("vortex_K", C.c_double), ("vortex_inf", C.c_double * 4),
         ("rk_scheme", C.c_int), ("rk_order", C.c_int), ("steady", C.c_int),
-        ("dt", C.c_double), ("cfl", C.c_double), ("reorder", C.c_int),
+        ("dt", C.c_double), ("cfl", C.c_double), ("reorder", C.c_int), ("kernel_path", C.c_int),
     ]
 
 
@@ -84,6 +85,7 @@ def load():
     L.fvs_wall_forces.argtypes = [V, V]
     L.fvs_set_timestep.argtypes = [V, D, D]
     L.fvs_kernel_times.argtypes = [V, D, I, V, V, V]
+    L.fvs_autotune.argtypes = [V, I, V, V]
     L.fvs_last_elapsed_ms.restype = D
     L.fvs_last_elapsed_ms.argtypes = [V]
     _lib = L

@@ -1,0 +1,268 @@
+// Bandwidth-lean kernels of the time loop (included by fvs_api.hip inside its
+// anonymous namespace, after the generic kernels and helpers).
+//
+// * XCD-aware block numbering: MI355X hands consecutive workgroups to its 8
+//   XCDs round robin; remapping makes each XCD walk one contiguous eighth of
+//   the cells, so the neighbour gathers of a block hit the L2 of its own XCD.
+// * k_grad_lsqc: the LSQ gradient as G^-1 * sum_j w_j^2 d_j (q_j - q_i), with
+//   d_j from the cell centres and G^-1 stored per cell, so a stencil entry
+//   costs 4 bytes (the neighbour id) instead of 20 (id + two coefficients).
+//   Mathematically the reference's coef_j w_j (src/gradient_lsq.f90:397-398);
+//   the rounding differs in the last bits.
+// * k_flux_update: one workgroup owns a block of 256 consecutive cells.  It
+//   first computes the Roe flux of every face touching the block into LDS
+//   (faces shared with another block are computed by both), then each thread
+//   gathers its cell's faces from LDS in the reference's summation order and
+//   applies the stage update.  Face fluxes never travel through HBM.
+
+struct ShiftTable {
+  double sx[15], sy[15];
+};
+
+// PMODE: 0 -> p = 0, 1 -> p = 1, 2 -> p = 2, 3 -> general p (pow)
+template <int LIM, int PMODE>
+__global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const double *__restrict__ W,
+                                                      const double2 *__restrict__ cxy,
+                                                      const double *__restrict__ ginv, const int *__restrict__ soff,
+                                                      const int *__restrict__ swid, const unsigned *__restrict__ sid,
+                                                      ShiftTable st, double p, double *__restrict__ grad,
+                                                      double *__restrict__ phi, int lt, const int *__restrict__ nvrt,
+                                                      const double2 *__restrict__ loff,
+                                                      const double *__restrict__ vol) {
+  const int i = xcd_block(nb) * kBlock + threadIdx.x;
+  if (i >= N) return;
+  const int s = i / kSlice, l = i % kSlice;
+  const int base = soff[s], wid = swid[s];
+  const double2 ci = cxy[i];
+  double w[4], sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, mn[4], mx[4];
+  ld4(W, i, w);
+  for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
+  // four neighbours per round: their ids, then their centres and states are
+  // all in flight before the first one is used (memory-level parallelism for a
+  // gather-latency-bound loop); the remainder goes one at a time
+  auto accumulate = [&](unsigned code, const double2 cj, const double q[4]) {
+    double xj = cj.x, yj = cj.y;
+    const int sh = int(code >> 27);
+    if (sh) {
+      xj = xj + st.sx[sh - 1];
+      yj = yj + st.sy[sh - 1];
+    }
+    const double dx = xj - ci.x, dy = yj - ci.y;
+    double w2 = 1.0;
+    if (PMODE != 0) {
+      const double r2 = dx * dx + dy * dy;
+      if (PMODE == 1) w2 = (r2 > 0.0) ? 1.0 / r2 : 0.0;
+      else if (PMODE == 2) w2 = (r2 > 0.0) ? 1.0 / (r2 * r2) : 0.0;
+      else w2 = (r2 > 0.0) ? pow(r2, -p) : 0.0;
+    }
+    for (int v = 0; v < 4; ++v) {
+      const double t = w2 * (q[v] - w[v]);
+      sx[v] = sx[v] + t * dx;
+      sy[v] = sy[v] + t * dy;
+      if (LIM) {
+        mn[v] = (q[v] < mn[v]) ? q[v] : mn[v];
+        mx[v] = (q[v] > mx[v]) ? q[v] : mx[v];
+      }
+    }
+  };
+  int k = 0;
+  for (; k + 4 <= wid; k += 4) {
+    unsigned code[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) code[u] = sid[base + long(k + u) * kSlice + l];
+    double2 cj[4];
+    double q[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = int(code[u] & 0x07ffffffu);
+      cj[u] = cxy[j];
+      ld4(W, j, q[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) accumulate(code[u], cj[u], q[u]);
+  }
+  for (; k < wid; ++k) {
+    const unsigned code = sid[base + long(k) * kSlice + l];
+    const int j = int(code & 0x07ffffffu);
+    double q[4];
+    ld4(W, j, q);
+    accumulate(code, cxy[j], q);
+  }
+  const double2 g01 = reinterpret_cast<const double2 *>(ginv)[2 * i];
+  const double2 g2 = reinterpret_cast<const double2 *>(ginv)[2 * i + 1];
+  double gx[4], gy[4];
+  for (int v = 0; v < 4; ++v) {
+    gx[v] = g01.x * sx[v] + g01.y * sy[v];
+    gy[v] = g01.y * sx[v] + g2.x * sy[v];
+  }
+  double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
+  st8(grad, i, g8);
+  if (LIM) phi[i] = cell_limiter(lt, nvrt[i], N, i, loff, vol[i], w, mn, mx, gx, gy);
+}
+
+// interior face flux with L = c1, R = c2 (shared by both flux kernels)
+__device__ __forceinline__ void interior_flux(const Phys &P, int2 c, const double g[8], const double *__restrict__ W,
+                                              const double *__restrict__ grad, const double *__restrict__ phi,
+                                              double fa[4], double &wsa) {
+  double wL[4], wR[4], gL[8], gR[8];
+  ld4(W, c.x, wL);
+  ld4(W, c.y, wR);
+  ld8(grad, c.x, gL);
+  ld8(grad, c.y, gR);
+  const double phL = phi[c.x], phR = phi[c.y];
+  const double k2 = P.kappa / 2.0, k1 = 1.0 - P.kappa;
+  double pL[4], pR[4];
+  for (int v = 0; v < 4; ++v) {
+    const double gC = wR[v] - wL[v];
+    const double dL = g[3] * gL[v] + g[4] * gL[4 + v];
+    const double dR = g[5] * gR[v] + g[6] * gR[4 + v];
+    pL[v] = wL[v] + phL * (k2 * gC + k1 * dL);
+    pR[v] = wR[v] + phR * (-(k2 * gC) + k1 * dR);
+  }
+  double fl[4], ws;
+  roe_flux(P.gamma, pL, pR, g[0], g[1], fl, ws);
+  const double A = g[2];
+  for (int v = 0; v < 4; ++v) fa[v] = fl[v] * A;
+  wsa = ws * A;
+}
+
+__device__ __forceinline__ void boundary_flux(const Phys &P, double t, int c, int ty, const double g[8],
+                                              const double *__restrict__ W, const double *__restrict__ grad,
+                                              const double *__restrict__ phi, double fa[4], double &wsa) {
+  double w[4], gr[8];
+  ld4(W, c, w);
+  ld8(grad, c, gr);
+  const double ph = phi[c];
+  double pL[4], pR[4];
+  for (int v = 0; v < 4; ++v) pL[v] = w[v] + ph * (g[3] * gr[v] + g[4] * gr[4 + v]);
+  const double nx = g[0], ny = g[1];
+  if (ty == FVS_BC_FREESTREAM) {
+    for (int v = 0; v < 4; ++v) pR[v] = P.pinf[v];
+  } else if (ty == FVS_BC_SLIP_WALL) {
+    const double un = pL[1] * nx + pL[2] * ny;
+    pR[0] = pL[0];
+    pR[1] = pL[1] - 2.0 * un * nx;
+    pR[2] = pL[2] - 2.0 * un * ny;
+    pR[3] = pL[3];
+  } else if (P.lvortex) {
+    vortex_state(P, t, g[5], g[6], pR);
+  } else {
+    mms_state(g[5], g[6], pR);
+  }
+  double fl[4], ws;
+  roe_flux(P.gamma, pL, pR, nx, ny, fl, ws);
+  const double A = g[2];
+  for (int v = 0; v < 4; ++v) fa[v] = fl[v] * A;
+  wsa = ws * A;
+}
+
+// fused face fluxes (LDS) + ordered gather + stage update; reads the stage's
+// primitive state from Win and writes the next one to Wout (separate buffers:
+// neighbouring blocks still read Win)
+__global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, int maxf, Upd A, Phys P, double t,
+                                                        const int *__restrict__ boff, const int *__restrict__ bfaces,
+                                                        const int *__restrict__ lgat, int Fi,
+                                                        const int2 *__restrict__ fc, const double *__restrict__ fgeo,
+                                                        const int *__restrict__ bcell, const int *__restrict__ btype,
+                                                        const double *__restrict__ bgeo,
+                                                        const double *__restrict__ Win,
+                                                        const double *__restrict__ grad,
+                                                        const double *__restrict__ phi,
+                                                        const double *__restrict__ vol, double *__restrict__ Ua,
+                                                        double *__restrict__ Ub, double *__restrict__ Facc,
+                                                        double *__restrict__ Wout, double *__restrict__ dtl,
+                                                        double *__restrict__ part) {
+  extern __shared__ double lds[];
+  double *lF = lds;             // [maxf][4] flux*area
+  double *lS = lds + 4 * maxf;  // [maxf]    wave speed*area
+  const int lb = xcd_block(nb);
+  const int nbl = nb < 0 ? -nb : nb;
+  if (lb >= nbl) return;  // the whole workgroup leaves before any barrier
+  const int f0 = boff[lb], nf = boff[lb + 1] - f0;
+  for (int f = threadIdx.x; f < nf; f += kBlock) {
+    const int slot = bfaces[f0 + f];
+    double fa[4], wsa, g[8];
+    if (slot < Fi) {
+      ld8(fgeo, slot, g);
+      interior_flux(P, fc[slot], g, Win, grad, phi, fa, wsa);
+    } else {
+      const int b = slot - Fi;
+      ld8(bgeo, b, g);
+      boundary_flux(P, t, bcell[b], btype[b], g, Win, grad, phi, fa, wsa);
+    }
+    for (int v = 0; v < 4; ++v) lF[4 * f + v] = fa[v];
+    lS[f] = wsa;
+  }
+  __syncthreads();
+  __shared__ double red[4 * kBlock];
+  const int i = lb * kBlock + threadIdx.x;
+  double d2[4] = {0, 0, 0, 0};
+  if (i < N) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0}, wacc = 0.0;
+    for (int k = 0; k < gw; ++k) {
+      const int s = lgat[long(k) * N + i];
+      if (s == 0) break;
+      const int f = (s > 0 ? s : -s) - 1;
+      if (s > 0)
+        for (int v = 0; v < 4; ++v) acc[v] = acc[v] + lF[4 * f + v];
+      else
+        for (int v = 0; v < 4; ++v) acc[v] = acc[v] - lF[4 * f + v];
+      wacc = wacc + lS[f];
+    }
+    const double V = vol[i];
+    double R[4];
+    for (int v = 0; v < 4; ++v) R[v] = -acc[v] / V;
+    double hs = A.h;
+    if (A.steady) {
+      double dl;
+      if (A.first) {
+        dl = A.cfl * V / (0.5 * wacc);
+        dtl[i] = dl;
+      } else {
+        dl = dtl[i];
+      }
+      hs = dl * A.h;
+    }
+    double u0[4], un[4];
+    ld4(Ua, i, u0);
+    if (A.mode == M_SSP3) {
+      double uc[4];
+      if (A.first) for (int v = 0; v < 4; ++v) uc[v] = u0[v];
+      else ld4(Ub, i, uc);
+      for (int v = 0; v < 4; ++v) un[v] = A.a * u0[v] + A.b * (uc[v] + hs * R[v]);
+    } else if (A.mode == M_SSP42) {
+      double fo[4] = {0, 0, 0, 0};
+      if (!A.first) ld4(Facc, i, fo);
+      for (int v = 0; v < 4; ++v) un[v] = u0[v] + hs * (A.coef * R[v] + fo[v]);
+      if (!A.last) {
+        double fn[4];
+        for (int v = 0; v < 4; ++v) fn[v] = fo[v] + R[v];
+        st4(Facc, i, fn);
+      }
+    } else {
+      double fo[4] = {0, 0, 0, 0}, fn[4];
+      if (!A.first) ld4(Facc, i, fo);
+      for (int v = 0; v < 4; ++v) fn[v] = fo[v] + A.coef * R[v];
+      if (A.use_acc) for (int v = 0; v < 4; ++v) un[v] = u0[v] + hs * fn[v];
+      else for (int v = 0; v < 4; ++v) un[v] = u0[v] + hs * R[v];
+      if (!A.last) st4(Facc, i, fn);
+    }
+    if (A.last) {
+      st4(Ua, i, un);
+      for (int v = 0; v < 4; ++v) {
+        const double d = fabs(un[v] - u0[v]);
+        d2[v] = d * d;
+      }
+    } else if (A.mode == M_SSP3) {
+      st4(Ub, i, un);
+    }
+    double w[4];
+    cons2prim(un, A.gamma, w);
+    st4(Wout, i, w);
+  }
+  if (A.last) {
+    block_sum4(d2, red);
+    if (threadIdx.x == 0)
+      for (int v = 0; v < 4; ++v) part[4 * lb + v] = d2[v];
+  }
+}

@@ -47,7 +47,22 @@ int fail(int code, const std::string &msg) {
 constexpr int kBlock = 256;
 constexpr int kSlice = 64;  // one wavefront
 
-inline int nblk(long n, int b = kBlock) { return int((n + b - 1) / b); }
+__host__ __device__ inline int nblk(long n, int b = kBlock) { return int((n + b - 1) / b); }
+
+// XCD-aware block numbering: MI355X dispatches consecutive workgroups to its
+// 8 XCDs round robin; remapping gives each XCD one contiguous eighth of the
+// index range, so neighbour gathers stay in that XCD's L2.  Launch
+// xcd_grid(nb) workgroups; logical blocks >= nb exit.
+constexpr int kXcd = 8;
+// A negative nb passed to a kernel means "plain numbering" (the autotuner
+// picks per kernel class which one is faster on the device).
+__device__ __forceinline__ int xcd_block(int nb) {
+  if (nb < 0) return int(blockIdx.x);
+  const int per = (nb + kXcd - 1) / kXcd;
+  return int(blockIdx.x % kXcd) * per + int(blockIdx.x / kXcd);
+}
+inline int xcd_grid(int nb) { return ((nb + kXcd - 1) / kXcd) * kXcd; }
+inline int xcd_arg(int nb, bool on) { return on ? nb : -nb; }
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -110,12 +125,12 @@ __device__ __forceinline__ double cell_limiter(int lt, int nf, int N, int i, con
 // LSQ gradient (src/gradient_lsq.f90:371-419): grad_i = sum_j cw_j (q_j - q_i),
 // cw = G^-1 D^T w per stencil entry; LIM fuses the limiter.
 template <int LIM>
-__global__ __launch_bounds__(kBlock) void k_grad_lsq(int N, const double *__restrict__ W, const int *__restrict__ soff,
+__global__ __launch_bounds__(kBlock) void k_grad_lsq(int nbx, int N, const double *__restrict__ W, const int *__restrict__ soff,
                                                      const int *__restrict__ swid, const int *__restrict__ snb,
                                                      const double2 *__restrict__ scw, double *__restrict__ grad,
                                                      double *__restrict__ phi, int lt, const int *__restrict__ nvrt,
                                                      const double2 *__restrict__ loff, const double *__restrict__ vol) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block(nbx) * kBlock + threadIdx.x;
   if (i >= N) return;
   const int s = i / kSlice, l = i % kSlice;
   const int base = soff[s], wid = swid[s];
@@ -163,14 +178,14 @@ __device__ __forceinline__ void stencil_minmax(int i, const int *__restrict__ so
 // GGCB gradient (src/gradient_ggcb.f90:116-138):
 // grad_i = (coef0 q_i + sum_f coefnb_f q_nb(f)) / vol
 template <int LIM>
-__global__ __launch_bounds__(kBlock) void k_grad_ggcb(int N, int nfw, const double *__restrict__ W,
+__global__ __launch_bounds__(kBlock) void k_grad_ggcb(int nbx, int N, int nfw, const double *__restrict__ W,
                                                       const double2 *__restrict__ c0, const int *__restrict__ nb,
                                                       const double2 *__restrict__ cn, const double *__restrict__ vol,
                                                       double *__restrict__ grad, double *__restrict__ phi, int lt,
                                                       const int *__restrict__ nvrt, const double2 *__restrict__ loff,
                                                       const int *__restrict__ soff, const int *__restrict__ swid,
                                                       const int *__restrict__ snb) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block(nbx) * kBlock + threadIdx.x;
   if (i >= N) return;
   double w[4], gx[4], gy[4];
   ld4(W, i, w);
@@ -200,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggcb(int N, int nfw, const doub
 // GGNB gradient (src/gradient_ggnb.f90:351-378):
 // grad_i = (coef0 q_i + sum_incidences (coefedg*coefnb) q_j) / vol
 template <int LIM>
-__global__ __launch_bounds__(kBlock) void k_grad_ggnb(int N, const double *__restrict__ W,
+__global__ __launch_bounds__(kBlock) void k_grad_ggnb(int nbx, int N, const double *__restrict__ W,
                                                       const double2 *__restrict__ c0, const int *__restrict__ goff,
                                                       const int *__restrict__ gwid, const int *__restrict__ gnb,
                                                       const double2 *__restrict__ gc, const double *__restrict__ vol,
@@ -208,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggnb(int N, const double *__res
                                                       const int *__restrict__ nvrt, const double2 *__restrict__ loff,
                                                       const int *__restrict__ soff, const int *__restrict__ swid,
                                                       const int *__restrict__ snb) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block(nbx) * kBlock + threadIdx.x;
   if (i >= N) return;
   const int s = i / kSlice, l = i % kSlice;
   double w[4], gx[4], gy[4];
@@ -239,11 +254,11 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggnb(int N, const double *__res
 }
 
 // interior faces: reconstruction (src/residual.f90:87-92) + Roe (95)
-__global__ __launch_bounds__(kBlock) void k_flux_int(int Fi, Phys P, const int2 *__restrict__ fc,
+__global__ __launch_bounds__(kBlock) void k_flux_int(int nbx, int Fi, Phys P, const int2 *__restrict__ fc,
                                                      const double *__restrict__ geo, const double *__restrict__ W,
                                                      const double *__restrict__ grad, const double *__restrict__ phi,
                                                      double *__restrict__ FA, double *__restrict__ WSA) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = xcd_block(nbx) * kBlock + threadIdx.x;
   if (f >= Fi) return;
   const int2 c = fc[f];
   double g[8];
@@ -337,7 +352,7 @@ __device__ __forceinline__ void block_sum4(double v[4], double *red) {
 
 // gather (src/residual.f90:97-101,153-155 in the reference's per-cell order),
 // -R/vol (164-166), local time step (src/runge_kutta.f90:433) and stage update
-__global__ __launch_bounds__(kBlock) void k_update(int N, int gw, Upd A, const int *__restrict__ gat,
+__global__ __launch_bounds__(kBlock) void k_update(int nbx, int N, int gw, Upd A, const int *__restrict__ gat,
                                                    const double *__restrict__ FA, const double *__restrict__ WSA,
                                                    const double *__restrict__ vol, double *__restrict__ Ua,
                                                    double *__restrict__ Ub, double *__restrict__ Facc,
@@ -345,7 +360,9 @@ __global__ __launch_bounds__(kBlock) void k_update(int N, int gw, Upd A, const i
                                                    double *__restrict__ rout, double *__restrict__ wsout,
                                                    double *__restrict__ part) {
   __shared__ double red[4 * kBlock];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = nblk(N), lb = xcd_block(nbx);
+  if (lb >= nb) return;
+  const int i = lb * kBlock + threadIdx.x;
   double d2[4] = {0, 0, 0, 0};
   if (i < N) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0}, wacc = 0.0;
@@ -420,20 +437,29 @@ __global__ __launch_bounds__(kBlock) void k_update(int N, int gw, Upd A, const i
   if (A.mode != M_RESID && A.last) {
     block_sum4(d2, red);
     if (threadIdx.x == 0)
-      for (int v = 0; v < 4; ++v) part[4 * blockIdx.x + v] = d2[v];
+      for (int v = 0; v < 4; ++v) part[4 * lb + v] = d2[v];
   }
 }
 
 // residual norms of one step: sqrt(sum_i (U-U0)^2 / N) per variable
 // (src/runge_kutta.f90:169-184), partials summed in block order
-__global__ void k_norm_final(int nparts, int N, const double *__restrict__ part, double *__restrict__ out) {
-  __shared__ double red[4 * kBlock];
+constexpr int kNormThreads = 1024;
+__global__ __launch_bounds__(kNormThreads) void k_norm_final(int nparts, int N, const double *__restrict__ part,
+                                                             double *__restrict__ out) {
+  __shared__ double red[4 * kNormThreads];
+  const int t = threadIdx.x;
   double s[4] = {0, 0, 0, 0};
-  for (int p = threadIdx.x; p < nparts; p += kBlock)
+  for (int p = t; p < nparts; p += kNormThreads)
     for (int v = 0; v < 4; ++v) s[v] = s[v] + part[4 * p + v];
-  block_sum4(s, red);
-  if (threadIdx.x == 0)
-    for (int v = 0; v < 4; ++v) out[v] = sqrt(s[v] / double(N));
+  for (int v = 0; v < 4; ++v) red[v * kNormThreads + t] = s[v];
+  __syncthreads();
+  for (int h = kNormThreads / 2; h > 0; h >>= 1) {
+    if (t < h)
+      for (int v = 0; v < 4; ++v) red[v * kNormThreads + t] = red[v * kNormThreads + t] + red[v * kNormThreads + t + h];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int v = 0; v < 4; ++v) out[v] = sqrt(red[v * kNormThreads] / double(N));
 }
 
 // isentropic-vortex error over interior cells (src/mms.f90:315-361)
@@ -548,11 +574,7 @@ __global__ void k_fill(long n, double v, double *__restrict__ p) {
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-template <class T>
-struct DBuf {
-  T *p = nullptr;
-  size_t n = 0;
-};
+#include "fused.hpp"
 
 }  // namespace
 
@@ -588,6 +610,21 @@ struct fvs_ctx {
   int *goff = nullptr, *gwid = nullptr, *gnb = nullptr;
   int *cell_intr = nullptr, *wall = nullptr;
   int ncell_intr = 0, nwall = 0;
+  // centroid-based LSQ gradient (k_grad_lsqc)
+  bool lsqc = false;
+  int pmode = 0;
+  double2 *cxy = nullptr;
+  double *ginv = nullptr;
+  unsigned *sid = nullptr;
+  ShiftTable shifts{};
+  // fused flux + update (k_flux_update): per-block face lists, local gather
+  bool fused = false;
+  int maxf = 0;
+  int *boff = nullptr, *bfaces = nullptr, *lgat = nullptr;
+  long nlisted = 0;
+  double *Walt = nullptr;  // second primitive-state buffer
+  // XCD-aware block numbering per kernel class (fvs_autotune picks)
+  bool xcd_grad = false, xcd_flux = false, xcd_upd = false, xcd_fused = true;
   int nparts = 0;
   bool have_state = false;
   double last_ms = 0.0;
@@ -762,6 +799,44 @@ int setup_device(fvs_ctx &C) {
     for (size_t k = 0; k < v.size(); ++k) gat[k * N + i] = v[k];
   }
   if (int rc = C.upload(C.gat, gat)) return rc;
+  // per-block face lists for the fused flux + update kernel: every face slot
+  // touching block b's cells, ascending (interior slots, then boundary slots),
+  // and each cell's faces as signed local indices in the gather order
+  if (C.opt.kernel_path == 0) {
+    const int nbk = nblk(N);
+    std::vector<int> boff(nbk + 1, 0), bfaces, lgat(size_t(gw) * N, 0), loc;
+    int maxf = 0;
+    for (int b = 0; b < nbk; ++b) {
+      loc.clear();
+      const int i0 = b * kBlock, i1 = std::min(N, i0 + kBlock);
+      for (int i = i0; i < i1; ++i)
+        for (int k = 0; k < gw; ++k) {
+          const int sg = gat[size_t(k) * N + i];
+          if (sg) loc.push_back((sg > 0 ? sg : -sg) - 1);
+        }
+      std::sort(loc.begin(), loc.end());
+      loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
+      for (int i = i0; i < i1; ++i)
+        for (int k = 0; k < gw; ++k) {
+          const int sg = gat[size_t(k) * N + i];
+          if (!sg) continue;
+          const int slot = (sg > 0 ? sg : -sg) - 1;
+          const int li = int(std::lower_bound(loc.begin(), loc.end(), slot) - loc.begin()) + 1;
+          lgat[size_t(k) * N + i] = sg > 0 ? li : -li;
+        }
+      bfaces.insert(bfaces.end(), loc.begin(), loc.end());
+      boff[b + 1] = int(bfaces.size());
+      maxf = std::max(maxf, int(loc.size()));
+    }
+    if (size_t(maxf) * 5 * sizeof(double) + 4 * kBlock * sizeof(double) <= 64 * 1024) {
+      C.maxf = maxf;
+      C.nlisted = long(bfaces.size());
+      if (int rc = C.upload(C.boff, boff)) return rc;
+      if (int rc = C.upload(C.bfaces, bfaces)) return rc;
+      if (int rc = C.upload(C.lgat, lgat)) return rc;
+      C.fused = true;
+    }
+  }
   // limiter face offsets (xf - xc) per local edge, in the cell's own frame
   if (C.opt.limiter > 0) {
     std::vector<double2> lo(4 * size_t(N), make_double2(0.0, 0.0));
@@ -790,6 +865,79 @@ int setup_device(fvs_ctx &C) {
     if (int rc = C.upload(C.swid, wid)) return rc;
     if (int rc = C.upload(C.snb, nb)) return rc;
     if (int rc = C.upload(C.scw, c2)) return rc;
+    // centroid form: neighbour id | shift code << 27, per-cell G^-1
+    if (C.opt.grad_method == 3 && C.opt.kernel_path == 0 && N < (1 << 27)) {
+      std::vector<double> dsx, dsy;  // distinct periodic shifts
+      std::vector<unsigned> code(nb.size(), 0u);
+      bool ok = true;
+      for (int s = 0; s < int(wid.size()) && ok; ++s)
+        for (int k = 0; k < wid[s] && ok; ++k)
+          for (int l = 0; l < kSlice && ok; ++l) {
+            const long e = off[s] + long(k) * kSlice + l;
+            const int i = s * kSlice + l;
+            unsigned sh = 0;
+            if (i < N && m.periodic) {
+              const int ci = P[i], cj = P[nb[e]];
+              double sx = 0.0, sy = 0.0;
+              for (int a = 0; a < m.nvrt[ci] && sx == 0.0 && sy == 0.0; ++a)
+                for (int b = 0; b < m.nvrt[cj]; ++b)
+                  if (m.cnodec[4 * ci + a] == m.cnodec[4 * cj + b]) {
+                    sx = m.nxy[2 * m.cnode[4 * ci + a]] - m.nxy[2 * m.cnode[4 * cj + b]];
+                    sy = m.nxy[2 * m.cnode[4 * ci + a] + 1] - m.nxy[2 * m.cnode[4 * cj + b] + 1];
+                    break;
+                  }
+              if (sx != 0.0 || sy != 0.0) {
+                size_t q = 0;
+                while (q < dsx.size() && !(dsx[q] == sx && dsy[q] == sy)) ++q;
+                if (q == dsx.size()) {
+                  if (q == 15) { ok = false; break; }
+                  dsx.push_back(sx);
+                  dsy.push_back(sy);
+                }
+                sh = unsigned(q + 1);
+              }
+            }
+            code[e] = unsigned(nb[e]) | (sh << 27);
+          }
+      if (ok) {
+        for (size_t q = 0; q < dsx.size(); ++q) C.shifts.sx[q] = dsx[q], C.shifts.sy[q] = dsy[q];
+        std::vector<double2> cxy(N);
+        std::vector<double> gi(4 * size_t(N), 0.0);
+        const double p = C.opt.lsq_p;
+        for (int i = 0; i < N; ++i) {
+          const int c = P[i];
+          cxy[i] = make_double2(m.cx[c], m.cy[c]);
+          // G = D^T D with rows w*(dx,dy), w = 1/d^p (src/gradient_lsq.f90:159-189)
+          double g11 = 0, g12 = 0, g22 = 0;
+          for (int k = m.lsq_ptr[c]; k < m.lsq_ptr[c + 1]; ++k) {
+            const int j = m.lsq_cell[k];
+            const double w = m.lsq_w[k];
+            double sx = 0.0, sy = 0.0;
+            if (m.periodic)
+              for (int a = 0; a < m.nvrt[c] && sx == 0.0 && sy == 0.0; ++a)
+                for (int b = 0; b < m.nvrt[j]; ++b)
+                  if (m.cnodec[4 * c + a] == m.cnodec[4 * j + b]) {
+                    sx = m.nxy[2 * m.cnode[4 * c + a]] - m.nxy[2 * m.cnode[4 * j + b]];
+                    sy = m.nxy[2 * m.cnode[4 * c + a] + 1] - m.nxy[2 * m.cnode[4 * j + b] + 1];
+                    break;
+                  }
+            const double d1 = w * ((m.cx[j] + sx) - m.cx[c]), d2 = w * ((m.cy[j] + sy) - m.cy[c]);
+            g11 = g11 + d1 * d1;
+            g12 = g12 + d1 * d2;
+            g22 = g22 + d2 * d2;
+          }
+          const double det = g11 * g22 - g12 * g12;
+          gi[4 * size_t(i)] = 1.0 / det * g22;
+          gi[4 * size_t(i) + 1] = -(1.0 / det * g12);
+          gi[4 * size_t(i) + 2] = 1.0 / det * g11;
+        }
+        C.pmode = (p == 0.0) ? 0 : (p == 1.0 ? 1 : (p == 2.0 ? 2 : 3));
+        if (int rc = C.upload(C.cxy, cxy)) return rc;
+        if (int rc = C.upload(C.ginv, gi)) return rc;
+        if (int rc = C.upload(C.sid, code)) return rc;
+        C.lsqc = true;
+      }
+    }
   }
   if (C.opt.grad_method == 1) {
     std::vector<double2> c0(N), cn(size_t(C.NFW) * N, make_double2(0.0, 0.0));
@@ -833,6 +981,7 @@ int setup_device(fvs_ctx &C) {
   if (int rc = C.alloc(C.Ub, n4)) return rc;
   if (int rc = C.alloc(C.Facc, n4)) return rc;
   if (int rc = C.alloc(C.W, n4)) return rc;
+  if (int rc = C.alloc(C.Walt, n4)) return rc;
   if (int rc = C.alloc(C.grad, 2 * n4)) return rc;
   if (int rc = C.alloc(C.phi, N)) return rc;
   if (int rc = C.alloc(C.dtl, N)) return rc;
@@ -854,7 +1003,7 @@ int setup_device(fvs_ctx &C) {
 }
 
 // kernel classes timed by fvs_kernel_times
-enum { KC_GRAD = 0, KC_FLUX_INT = 1, KC_FLUX_BND = 2, KC_UPDATE = 3, KC_NORM = 4, KC_N = 5 };
+enum { KC_GRAD = 0, KC_FLUX_INT = 1, KC_FLUX_BND = 2, KC_UPDATE = 3, KC_NORM = 4, KC_FUSED = 5, KC_N = 6 };
 
 struct ProfScope {
   fvs_ctx &C;
@@ -883,28 +1032,45 @@ int launch_gradient(fvs_ctx &C, bool with_limiter) {
   const int N = C.N;
   const bool lim = with_limiter && C.opt.limiter > 0 && C.opt.recon != 1;
   const int lt = C.opt.limiter;
-  dim3 g(nblk(N)), b(kBlock);
+  dim3 g(xcd_grid(nblk(N))), b(kBlock);
   ProfScope ps(C, KC_GRAD);
-  if (C.opt.grad_method == 3) {
+  if (C.lsqc) {
+    const int nb = nblk(N);
+#define FVS_LSQC(L, PM)                                                                                          \
+  hipLaunchKernelGGL((k_grad_lsqc<L, PM>), g, b, 0, C.stream, N, xcd_arg(nb, C.xcd_grad), C.W, C.cxy, C.ginv, C.soff, C.swid, C.sid,  \
+                     C.shifts, C.opt.lsq_p, C.grad, C.phi, lt, C.nvrt, C.loff, C.vol)
+    if (lim) {
+      if (C.pmode == 0) FVS_LSQC(1, 0);
+      else if (C.pmode == 1) FVS_LSQC(1, 1);
+      else if (C.pmode == 2) FVS_LSQC(1, 2);
+      else FVS_LSQC(1, 3);
+    } else {
+      if (C.pmode == 0) FVS_LSQC(0, 0);
+      else if (C.pmode == 1) FVS_LSQC(0, 1);
+      else if (C.pmode == 2) FVS_LSQC(0, 2);
+      else FVS_LSQC(0, 3);
+    }
+#undef FVS_LSQC
+  } else if (C.opt.grad_method == 3) {
     if (lim)
-      hipLaunchKernelGGL(k_grad_lsq<1>, g, b, 0, C.stream, N, C.W, C.soff, C.swid, C.snb, C.scw, C.grad, C.phi, lt,
+      hipLaunchKernelGGL(k_grad_lsq<1>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.W, C.soff, C.swid, C.snb, C.scw, C.grad, C.phi, lt,
                          C.nvrt, C.loff, C.vol);
     else
-      hipLaunchKernelGGL(k_grad_lsq<0>, g, b, 0, C.stream, N, C.W, C.soff, C.swid, C.snb, C.scw, C.grad, C.phi, lt,
+      hipLaunchKernelGGL(k_grad_lsq<0>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.W, C.soff, C.swid, C.snb, C.scw, C.grad, C.phi, lt,
                          C.nvrt, C.loff, C.vol);
   } else if (C.opt.grad_method == 1) {
     if (lim)
-      hipLaunchKernelGGL(k_grad_ggcb<1>, g, b, 0, C.stream, N, C.NFW, C.W, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
+      hipLaunchKernelGGL(k_grad_ggcb<1>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.NFW, C.W, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
                          C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
     else
-      hipLaunchKernelGGL(k_grad_ggcb<0>, g, b, 0, C.stream, N, C.NFW, C.W, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
+      hipLaunchKernelGGL(k_grad_ggcb<0>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.NFW, C.W, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
                          C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
   } else {
     if (lim)
-      hipLaunchKernelGGL(k_grad_ggnb<1>, g, b, 0, C.stream, N, C.W, C.ggnb0, C.goff, C.gwid, C.gnb, C.ggnbc, C.vol,
+      hipLaunchKernelGGL(k_grad_ggnb<1>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.W, C.ggnb0, C.goff, C.gwid, C.gnb, C.ggnbc, C.vol,
                          C.grad, C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
     else
-      hipLaunchKernelGGL(k_grad_ggnb<0>, g, b, 0, C.stream, N, C.W, C.ggnb0, C.goff, C.gwid, C.gnb, C.ggnbc, C.vol,
+      hipLaunchKernelGGL(k_grad_ggnb<0>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.W, C.ggnb0, C.goff, C.gwid, C.gnb, C.ggnbc, C.vol,
                          C.grad, C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
   }
   HIPC(hipGetLastError());
@@ -917,7 +1083,8 @@ int launch_fluxes(fvs_ctx &C, double t) {
     if (int rc = launch_gradient(C, true)) return rc;
   if (C.Fi > 0) {
     ProfScope ps(C, KC_FLUX_INT);
-    hipLaunchKernelGGL(k_flux_int, dim3(nblk(C.Fi)), dim3(kBlock), 0, C.stream, C.Fi, C.P, C.fi_cells, C.fi_geo,
+    hipLaunchKernelGGL(k_flux_int, dim3(xcd_grid(nblk(C.Fi))), dim3(kBlock), 0, C.stream,
+                       xcd_arg(nblk(C.Fi), C.xcd_flux), C.Fi, C.P, C.fi_cells, C.fi_geo,
                        C.W, C.grad, C.phi, C.FA, C.WSA);
   }
   if (C.Fb > 0) {
@@ -931,9 +1098,28 @@ int launch_fluxes(fvs_ctx &C, double t) {
 
 int launch_update(fvs_ctx &C, const Upd &A) {
   ProfScope ps(C, KC_UPDATE);
-  hipLaunchKernelGGL(k_update, dim3(nblk(C.N)), dim3(kBlock), 0, C.stream, C.N, C.GW, A, C.gat, C.FA, C.WSA, C.vol,
+  hipLaunchKernelGGL(k_update, dim3(xcd_grid(nblk(C.N))), dim3(kBlock), 0, C.stream, xcd_arg(nblk(C.N), C.xcd_upd),
+                     C.N, C.GW, A, C.gat, C.FA, C.WSA, C.vol,
                      C.U0, C.Ub, C.Facc, C.W, C.dtl, C.rout, C.wsout, C.part);
   HIPC(hipGetLastError());
+  return 0;
+}
+
+// fused flux + gather + update of one stage; swaps the primitive buffers
+int launch_flux_update(fvs_ctx &C, const Upd &A, double t) {
+  if (C.opt.recon != 1)
+    if (int rc = launch_gradient(C, true)) return rc;
+  {
+    ProfScope ps(C, KC_FUSED);
+    const int nb = nblk(C.N);
+    const size_t lds = size_t(C.maxf) * 5 * sizeof(double);
+    hipLaunchKernelGGL(k_flux_update, dim3(xcd_grid(nb)), dim3(kBlock), lds, C.stream, C.N, xcd_arg(nb, C.xcd_fused),
+                       C.GW, C.maxf, A,
+                       C.P, t, C.boff, C.bfaces, C.lgat, C.Fi, C.fi_cells, C.fi_geo, C.fb_cell, C.fb_type, C.fb_geo,
+                       C.W, C.grad, C.phi, C.vol, C.U0, C.Ub, C.Facc, C.Walt, C.dtl, C.part);
+  }
+  HIPC(hipGetLastError());
+  std::swap(C.W, C.Walt);
   return 0;
 }
 
@@ -1370,13 +1556,17 @@ int fvs_time_integration(fvs_ctx *C, double t1, int nsteps, double *norms, doubl
         const double tst[4] = {told, t12, t12, tnew};
         ts = tst[k];
       }
-      rc = launch_fluxes(*C, ts);
-      if (!rc) rc = launch_update(*C, A);
+      if (C->fused) {
+        rc = launch_flux_update(*C, A, ts);
+      } else {
+        rc = launch_fluxes(*C, ts);
+        if (!rc) rc = launch_update(*C, A);
+      }
     }
     if (rc) break;
     {
       ProfScope ps(*C, KC_NORM);
-      hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, C->stream, nblk(C->N), C->N, C->part, dn + 4 * s);
+      hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kNormThreads), 0, C->stream, nblk(C->N), C->N, C->part, dn + 4 * s);
     }
     if (dv) rc = launch_vortex_err(*C, told + dt, dv + 13 * size_t(s));
   }
@@ -1438,8 +1628,8 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   int rc = fvs_time_integration(C, t1, nsteps, nullptr, nullptr);
   C->profiling = false;
   if (rc) return rc;
-  double tot[KC_N] = {0, 0, 0, 0, 0};
-  int cnt[KC_N] = {0, 0, 0, 0, 0};
+  double tot[KC_N] = {0, 0, 0, 0, 0, 0};
+  int cnt[KC_N] = {0, 0, 0, 0, 0, 0};
   for (size_t k = 0; k < C->evclass.size(); ++k) {
     float t = 0.f;
     HIPC(hipEventElapsedTime(&t, C->evpool[2 * k], C->evpool[2 * k + 1]));
@@ -1452,7 +1642,8 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   const double N = C->N, Fi = C->Fi, Fb = C->Fb;
   const bool lim = C->opt.limiter > 0 && C->opt.recon != 1;
   double bg = 0.0;
-  if (C->opt.grad_method == 3) bg = N * (32 + 64) + 20.0 * m.lsq_ptr.back();
+  if (C->lsqc) bg = N * (32 + 16 + 32 + 64) + 4.0 * m.lsq_ptr.back();
+  else if (C->opt.grad_method == 3) bg = N * (32 + 64) + 20.0 * m.lsq_ptr.back();
   else if (C->opt.grad_method == 1) bg = N * (32 + 16 + 8 + 64) + N * C->NFW * 20.0;
   else bg = N * (32 + 16 + 8 + 64) + 20.0 * m.ggnb_ptr.back();
   if (lim) {
@@ -1471,12 +1662,59 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   if (C->opt.steady) state += (8.0 + 8.0 * (nst - 1)) / nst;
   const double bup = N * (C->GW * 4.0 + 8 + 32 + state) + (Fi + Fb) * 40.0;
   const double bno = 4.0 * 8 * nblk(C->N) + 32;
-  const double b5[KC_N] = {bg, bfi, bfb, bup, bno};
+  // fused: face records and block lists once, the stage's W/grad/phi once,
+  // the cell-side traffic of the update kernel without the face-flux buffer
+  const double bfu = (Fi + Fb) * (8 + 56) + 4.0 * double(C->nlisted) + N * (32 + 64 + 8) +
+                     N * (C->GW * 4.0 + 8 + 32 + state);
+  const double b5[KC_N] = {bg, bfi, bfb, bup, bno, bfu};
   for (int k = 0; k < KC_N; ++k) {
     ms[k] = cnt[k] ? tot[k] / cnt[k] : 0.0;
     bytes[k] = b5[k];
     launches[k] = cnt[k];
   }
+  return 0;
+}
+
+int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) {
+  if (int rc = need_state(C)) return rc;
+  if (nsteps <= 0) return fail(FVS_EINVAL, "nsteps must be positive");
+  // the trials advance the state: keep a copy and restore it afterwards
+  const size_t n4 = 4 * size_t(C->N);
+  double *save = nullptr;
+  HIPC(hipMalloc(&save, n4 * sizeof(double)));
+  int rc = 0;
+  if (hipMemcpyAsync(save, C->U0, n4 * 8, hipMemcpyDeviceToDevice, C->stream) != hipSuccess)
+    rc = fail(FVS_EHIP, "autotune: state copy failed");
+  auto trial = [&](double &ms) -> int {
+    if (hipMemcpyAsync(C->U0, save, n4 * 8, hipMemcpyDeviceToDevice, C->stream) != hipSuccess)
+      return fail(FVS_EHIP, "autotune: state restore failed");
+    hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+    if (int r = fvs_time_integration(C, 0.0, nsteps, nullptr, nullptr)) return r;
+    ms = C->last_ms / nsteps;
+    return 0;
+  };
+  bool *knobs[4] = {&C->xcd_grad, &C->xcd_flux, &C->xcd_upd, &C->xcd_fused};
+  const bool used[4] = {C->opt.recon != 1, !C->fused, !C->fused, C->fused};
+  double best = 0.0;
+  if (!rc) rc = trial(best);  // warm-up
+  if (!rc) rc = trial(best);
+  for (int k = 0; k < 4 && !rc; ++k) {
+    if (!used[k]) continue;
+    *knobs[k] = !*knobs[k];
+    double ms = 0.0;
+    rc = trial(ms);
+    if (!rc && ms < best) best = ms;
+    else *knobs[k] = !*knobs[k];
+  }
+  if (hipMemcpyAsync(C->U0, save, n4 * 8, hipMemcpyDeviceToDevice, C->stream) != hipSuccess && !rc)
+    rc = fail(FVS_EHIP, "autotune: state restore failed");
+  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  if (hipStreamSynchronize(C->stream) != hipSuccess && !rc) rc = fail(FVS_EHIP, "autotune: sync failed");
+  (void)hipFree(save);
+  if (rc) return rc;
+  if (choice4)
+    for (int k = 0; k < 4; ++k) choice4[k] = *knobs[k] ? 1 : 0;
+  if (best_ms) *best_ms = best;
   return 0;
 }
 

@@ -59,7 +59,8 @@ class Scheme:
     steady: int = 0
     dt: float = 0.01
     cfl: float = 1.0
-    reorder: int = 0
+    reorder: int = 0          # device cell order: 0 reference, 1 Morton
+    kernel_path: int = 0      # 0 fastest available, 1 face-flux buffer kernels
 
     def to_c(self) -> _lib.Options:
         o = _lib.Options()
@@ -74,6 +75,7 @@ class Scheme:
     def oracle_kwargs(self) -> dict:
         d = dataclasses.asdict(self)
         d.pop("reorder")
+        d.pop("kernel_path")
         return d
 
 
@@ -308,15 +310,25 @@ class Solver:
         check(self._L.fvs_set_timestep(self._h, dt, cfl))
         self.scheme.dt, self.scheme.cfl = dt, cfl
 
-    KERNEL_CLASSES = ("gradient", "flux_interior", "flux_boundary", "gather_update", "norm_reduce")
+    KERNEL_CLASSES = ("gradient", "flux_interior", "flux_boundary", "gather_update", "norm_reduce",
+                      "flux_update_fused")
 
     def kernel_times(self, t1: float, nsteps: int):
         """Per kernel class: (mean ms per launch, algorithmic bytes per launch,
         launches) over nsteps steps (the state advances)."""
-        ms, by = np.zeros(5), np.zeros(5)
-        n = np.zeros(5, np.int32)
+        ms, by = np.zeros(6), np.zeros(6)
+        n = np.zeros(6, np.int32)
         check(self._L.fvs_kernel_times(self._h, t1, nsteps, ptr(ms), ptr(by), ptr(n)))
         return {k: (float(ms[i]), float(by[i]), int(n[i])) for i, k in enumerate(self.KERNEL_CLASSES)}
+
+    def autotune(self, nsteps: int = 10):
+        """Pick plain or XCD-aware workgroup numbering per kernel class; the
+        state is left as it was.  Returns (choices, best ms per step)."""
+        ch = np.zeros(4, np.int32)
+        best = np.zeros(1)
+        check(self._L.fvs_autotune(self._h, nsteps, ptr(ch), ptr(best)))
+        names = ("gradient", "flux", "update", "fused")
+        return {n: int(c) for n, c in zip(names, ch)}, float(best[0])
 
     @property
     def last_elapsed_ms(self) -> float:

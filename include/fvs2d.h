@@ -55,6 +55,8 @@ typedef struct {
   int steady;           /* local time stepping                                  */
   double dt, cfl;
   int reorder;          /* device cell order: 0 reference order, 1 Morton blocks */
+  int kernel_path;      /* 0: fastest available (fused LDS flux+update, centroid LSQ);
+                           1: face-flux buffer kernels only */
 } fvs_options;
 
 const char *fvs_last_error(void);
@@ -148,10 +150,17 @@ int fvs_set_timestep(fvs_ctx *ctx, double dt, double cfl);
 /* Measurement helper, no reference counterpart: runs nsteps steps like
  * fvs_time_integration (advancing the state) with HIP events around every
  * launch and returns, per kernel class [gradient(+limiter), interior-face
- * flux, boundary-face flux, gather+update, norm reduction], the mean device
+ * flux, boundary-face flux, gather+update, norm reduction, fused
+ * flux+gather+update], the mean device
  * time per launch (ms), the algorithmic bytes per launch and the number of
  * launches timed. */
-int fvs_kernel_times(fvs_ctx *ctx, double t1, int nsteps, double *ms5, double *bytes5, int *launches5);
+int fvs_kernel_times(fvs_ctx *ctx, double t1, int nsteps, double *ms6, double *bytes6, int *launches6);
+/* Launch autotuner, no reference counterpart: times nsteps steps per trial
+ * and keeps, per kernel class [gradient, face flux, gather+update, fused
+ * flux+update], the faster of plain and XCD-aware workgroup numbering.  The
+ * state is restored afterwards.  choice4 (may be NULL) receives the choices
+ * (1 = XCD-aware), best_ms (may be NULL) the best time per step. */
+int fvs_autotune(fvs_ctx *ctx, int nsteps, int *choice4, double *best_ms);
 /* wall-clock of the last fvs_time_integration call's device work (ms) */
 double fvs_last_elapsed_ms(const fvs_ctx *ctx);
 

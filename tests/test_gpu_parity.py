@@ -5,7 +5,9 @@ not, so results agree to rounding, not bitwise):
   * one residual evaluation: max|d| <= 1e-12 * max|ref| per variable;
   * trajectories: |d| / |ref| <= 1e-10 per cell and variable after the
     stated number of steps (the north-star parity bar);
-  * limiter values: |d| <= 1e-12 (branches taken identically).
+  * limiter values: |d| <= 1e-10 (Barth divides by a face increment that
+    carries the rounding difference), scaled by max|phi| where the Van Albada
+    form returns values far outside [0, 1].
 """
 import numpy as np
 import pytest
@@ -90,11 +92,16 @@ def test_residual_reconstruction(recon, kappa):
     assert rel_err(ws, orc.ws) <= 1e-13
 
 
+PATHS = [dict(kernel_path=0, reorder=0), dict(kernel_path=0, reorder=1), dict(kernel_path=1, reorder=0)]
+
+
 @pytest.mark.parametrize("kind", ["dirichlet", "periodic"])
 @pytest.mark.parametrize("scheme", ["ssp3", "rk4", "ssp42"])
 def test_trajectory_config1(kind, scheme):
     """64x64 vortex square, LSQ-nn, unlimited, 100 steps: cell states within
-    1e-10 relative, per-step residual norms and vortex errors alike."""
+    1e-10 relative, per-step residual norms and vortex errors alike, for the
+    fused LDS kernel path (reference and Morton cell order) and the face-flux
+    buffer path."""
     m = mesh(kind, 64)
     rk = dict(ssp3=dict(rk_scheme=2), rk4=dict(rk_scheme=0, rk_order=4), ssp42=dict(rk_scheme=1, rk_order=2))[scheme]
     sch = vortex_scheme(**rk)
@@ -106,14 +113,16 @@ def test_trajectory_config1(kind, scheme):
     orc.initialize(1, 0.0)
     nsteps = 100
     n_ref, v_ref = orc.time_integration(0.0, nsteps, want_verr=True)
-    gpu = Solver(m, sch)
-    gpu.initialize_solution(1, 0.0)
-    n_gpu, v_gpu = gpu.time_integration(0.0, nsteps, vortex_error=True)
-    err = rel_err(gpu.cvar, orc.cvar)
-    print(f"{kind} {scheme} dt={dt:.6g} max rel err after {nsteps} steps: {err:.3e}")
-    assert err <= 1e-10
-    assert rel_err(n_gpu, n_ref) <= 1e-9
-    assert rel_err(v_gpu, v_ref) <= 1e-9
+    ref = orc.cvar
+    for path in PATHS:
+        gpu = Solver(m, sch, **path)
+        gpu.initialize_solution(1, 0.0)
+        n_gpu, v_gpu = gpu.time_integration(0.0, nsteps, vortex_error=True)
+        err = rel_err(gpu.cvar, ref)
+        print(f"{kind} {scheme} {path} dt={dt:.6g} max rel err after {nsteps} steps: {err:.3e}")
+        assert err <= 1e-10, path
+        assert rel_err(n_gpu, n_ref) <= 1e-9, path
+        assert rel_err(v_gpu, v_ref) <= 1e-9, path
 
 
 @pytest.mark.parametrize("order", [1, 2, 3])
@@ -191,19 +200,23 @@ def test_gradient_1var():
 
 
 def test_determinism_and_reorder():
-    """Bitwise run-to-run equality (no float atomics), and the Morton device
-    order gives the same cell states as the reference order."""
+    """Bitwise run-to-run equality (no float atomics); the Morton device order
+    gives bitwise the same states as the reference order (each face flux and
+    each cell sum is computed by the same arithmetic in either order); the
+    face-buffer path agrees to rounding."""
     m = mesh("periodic", 48)
-    sch = vortex_scheme(dt=0.01)
-    out = []
-    for reorder in (0, 0, 1):
-        sch.reorder = reorder
-        gpu = Solver(m, sch)
+    out = {}
+    for key, path in (("a", dict(kernel_path=0, reorder=0)), ("b", dict(kernel_path=0, reorder=0)),
+                      ("c", dict(kernel_path=0, reorder=1)), ("d", dict(kernel_path=1, reorder=0)),
+                      ("e", dict(kernel_path=1, reorder=1))):
+        gpu = Solver(m, vortex_scheme(dt=0.01, limiter=1), **path)
         gpu.initialize_solution(1, 0.0)
         gpu.time_integration(0.0, 30)
-        out.append(gpu.cvar)
-    assert np.array_equal(out[0], out[1])
-    assert np.array_equal(out[0], out[2])
+        out[key] = gpu.cvar
+    assert np.array_equal(out["a"], out["b"])
+    assert np.array_equal(out["a"], out["c"])
+    assert np.array_equal(out["d"], out["e"])
+    assert rel_err(out["a"], out["d"]) <= 1e-12
 
 
 def test_vortex_error_call():
@@ -215,3 +228,19 @@ def test_vortex_error_call():
     gpu = Solver(m, sch)
     gpu.cvar = cv
     assert rel_err(gpu.error_isentropic_vortex(0.25), orc.vortex_error(0.25)) <= 1e-11
+
+
+def test_autotune_keeps_state_and_results():
+    """The autotuner restores the state, and workgroup numbering changes no
+    arithmetic: results are bitwise those of the untuned solver."""
+    m = mesh("periodic", 48)
+    a = Solver(m, vortex_scheme(dt=0.01), reorder=1)
+    a.initialize_solution(1, 0.0)
+    before = a.cvar
+    a.autotune(3)
+    assert np.array_equal(a.cvar, before)
+    a.time_integration(0.0, 10)
+    b = Solver(m, vortex_scheme(dt=0.01), reorder=1)
+    b.initialize_solution(1, 0.0)
+    b.time_integration(0.0, 10)
+    assert np.array_equal(a.cvar, b.cvar)
