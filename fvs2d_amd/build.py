@@ -53,5 +53,41 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FLANG = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+FORTRAN = os.path.join(HERE, "fortran")
+TESTS_F = os.path.join(os.path.dirname(HERE), "tests", "fortran")
+FDRIVER = os.path.join(TESTS_F, "libfvs2d_fdriver.so")
+# the reference is compiled with default REAL = 8 bytes (-r8)
+F_FLAGS = ["-O2", "-fPIC", "-fdefault-real-8", "-fdefault-double-8"]
+
+
+def build_fortran_harness(verbose: bool = False) -> str:
+    """Compile the iso_c_binding modules (fvs2d_amd/fortran) with the test
+    harness (tests/fortran) into tests/fortran/libfvs2d_fdriver.so."""
+    if not os.path.exists(FLANG):
+        return ""
+    mod = os.path.join(BUILD, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    srcs = [os.path.join(TESTS_F, "harness_modules.f90"), os.path.join(FORTRAN, "fvs2d_capi.f90"),
+            os.path.join(FORTRAN, "fvs2d_dropin.f90"), os.path.join(TESTS_F, "driver.f90")]
+    objs = []
+    for src in srcs:
+        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+        cmd = [FLANG, *F_FLAGS, "-module-dir", mod, "-I", mod, "-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run(cmd)
+        objs.append(obj)
+    tmp = FDRIVER + ".tmp"
+    link = [FLANG, "-shared", "-fPIC", *objs, "-L", HERE, "-lfvs2d_hip", f"-Wl,-rpath,{HERE}",
+            "-Wl,-rpath,$ORIGIN/../../fvs2d_amd", "-o", tmp]
+    if verbose:
+        print(" ".join(link), flush=True)
+    _run(link)
+    os.replace(tmp, FDRIVER)
+    return FDRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran_harness(verbose=True))

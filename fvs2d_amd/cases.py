@@ -1,0 +1,86 @@
+"""The named workloads, as (mesh, scheme) pairs, and a small command line.
+
+* ``vortex(n, periodic)`` -- isentropic vortex on the jittered square
+  (n = 64: 8,192 triangles; n = 1024: 2,097,152), LSQ-nn, SSP-RK3, CFL 0.5.
+* ``naca(n_around, n_layers, mach, aoa, thickness)`` -- steady Euler on the
+  NACA 00tt O-grid, slip wall + free stream, LSQ-nn + Venkatakrishnan,
+  SSP-RK(4,2) with local time stepping.
+
+``python -m fvs2d_amd.cases steady --n 512 --layers 256`` runs the steady case
+to a 1e-8 density-residual drop and prints C_L, C_D and the iteration count as
+JSON.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import meshgen
+from ._lib import RK_SSP3, RK_SSP42
+from .solver import Scheme, Solver
+
+
+def vortex(n: int = 64, periodic: bool = True, **kw):
+    m = meshgen.jittered_square(n, periodic=periodic, seed=0)
+    d = dict(grad_method=3, lsq_nn=1, limiter=0, recon=2, gamma=1.4, lvortex=1, vortex_x0=0.0, vortex_y0=0.0,
+             vortex_K=5.0, vortex_inf=(1.0, 1.0, 1.0, 1.0), rk_scheme=RK_SSP3, cfl=0.5, reorder=1)
+    d.update(kw)
+    return m, Scheme(**d)
+
+
+def naca(n_around: int = 512, n_layers: int = 256, mach: float = 0.8, aoa: float = 1.25, thickness: float = 0.12,
+         cfl: float = 1.25, **kw):
+    m = meshgen.naca_ogrid(n_around, n_layers, thickness=thickness)
+    d = dict(grad_method=3, lsq_nn=1, limiter=1, recon=2, gamma=1.4, mach=mach, aoa_deg=aoa, lvortex=0,
+             rk_scheme=RK_SSP42, rk_order=2, steady=1, cfl=cfl, dt=1.0, reorder=1)
+    d.update(kw)
+    return m, Scheme(**d)
+
+
+def run_steady(n_around, n_layers, mach, aoa, thickness, cfl, max_steps, drop, limiter=1, chunk=500,
+               freeze_at=None):
+    m, sch = naca(n_around, n_layers, mach, aoa, thickness, cfl, limiter=limiter)
+    t0 = time.perf_counter()
+    s = Solver(m, sch)
+    s.initialize_solution(1, 0.0)
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = s.steady_solve(max_steps, drop=drop, chunk=chunk, freeze_at=freeze_at)
+    sec = time.perf_counter() - t0
+    cl, cd, cl1, cd1 = s.wall_forces()
+    h = res["history"]
+    return dict(cells=m.ncells, mach=mach, aoa=aoa, thickness=thickness, cfl=cfl, limiter=limiter,
+                iterations=res["iterations"], steps_run=res["steps_run"], diverged=res["diverged"],
+                limiter_frozen_at=res["limiter_frozen_at"],
+                final_drop=res["drop"], seconds=sec, setup_seconds=setup, cl=cl, cd=cd, cl_cell=cl1, cd_cell=cd1,
+                ms_per_iteration=1e3 * sec / max(res["steps_run"], 1),
+                drop_every_1000=[float(h[min(k, len(h) - 1), 0] / h[0, 0]) for k in range(0, len(h), 1000)])
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m fvs2d_amd.cases")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    st = sub.add_parser("steady")
+    st.add_argument("--n", type=int, default=512)
+    st.add_argument("--layers", type=int, default=256)
+    st.add_argument("--mach", type=float, default=0.8)
+    st.add_argument("--aoa", type=float, default=1.25)
+    st.add_argument("--thickness", type=float, default=0.12)
+    st.add_argument("--cfl", type=float, default=1.25)
+    st.add_argument("--max-steps", type=int, default=100000)
+    st.add_argument("--drop", type=float, default=1e-8)
+    st.add_argument("--limiter", type=int, default=1)
+    st.add_argument("--freeze-at", type=float, default=None)
+    a = ap.parse_args(argv)
+    out = run_steady(a.n, a.layers, a.mach, a.aoa, a.thickness, a.cfl, a.max_steps, a.drop, a.limiter,
+                     freeze_at=a.freeze_at)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

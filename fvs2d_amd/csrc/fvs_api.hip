@@ -625,6 +625,8 @@ struct fvs_ctx {
   double *Walt = nullptr;  // second primitive-state buffer
   // XCD-aware block numbering per kernel class (fvs_autotune picks)
   bool xcd_grad = false, xcd_flux = false, xcd_upd = false, xcd_fused = true;
+  // limiter values frozen at their last evaluation (steady convergence aid)
+  bool limiter_frozen = false;
   int nparts = 0;
   bool have_state = false;
   double last_ms = 0.0;
@@ -1030,7 +1032,7 @@ struct ProfScope {
 
 int launch_gradient(fvs_ctx &C, bool with_limiter) {
   const int N = C.N;
-  const bool lim = with_limiter && C.opt.limiter > 0 && C.opt.recon != 1;
+  const bool lim = with_limiter && C.opt.limiter > 0 && C.opt.recon != 1 && !C.limiter_frozen;
   const int lt = C.opt.limiter;
   dim3 g(xcd_grid(nblk(N))), b(kBlock);
   ProfScope ps(C, KC_GRAD);
@@ -1672,6 +1674,12 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
     bytes[k] = b5[k];
     launches[k] = cnt[k];
   }
+  return 0;
+}
+
+int fvs_freeze_limiter(fvs_ctx *C, int frozen) {
+  if (!C) return fail(FVS_EINVAL, "null context");
+  C->limiter_frozen = frozen != 0;
   return 0;
 }
 

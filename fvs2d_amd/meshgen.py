@@ -156,76 +156,66 @@ def naca_ogrid(n_around: int = 512, n_layers: int = 256, *, thickness: float = 0
                first_spacing: float = 2e-3, far_radius: float = 50.0) -> Mesh:
     """O-grid around a symmetric NACA 4-digit airfoil with a closed trailing edge.
 
-    ``n_around`` surface nodes, cosine-clustered in chord (clockwise from the
-    trailing edge over the lower... i.e. ordered so that cells are CCW),
-    ``n_layers`` cell layers.  Layer k (0 at the wall) sits at wall distance
-    ``first*(r^k-1)/(r-1)`` along a blend of the surface normal and the ray to
-    a circle of radius ``far_radius`` around mid-chord; the ratio r is solved
-    so the last layer reaches the far field.  Each quad is split into two
-    triangles.  Boundaries: ``slip_wall`` (cells on the airfoil) and
-    ``freestream`` (cells on the outer circle), each cell with one boundary
-    edge."""
+    ``n_around`` surface nodes at x = (1 + cos s)/2, s = 2 pi i / n (clustered
+    at both edges), ordered trailing edge -> upper surface -> leading edge ->
+    lower surface.  Node k of the ray from surface node i sits at distance
+    d_k = first * (r^k - 1)/(r - 1) along a direction that turns from the
+    surface normal (at the wall) to the ray from mid-chord (far away); r is
+    solved so that d reaches ``far_radius``, and each ray's distances are
+    scaled so its last node lies on the circle of that radius about mid-chord.
+    Each quad is split into two counter-clockwise triangles; ``slip_wall`` lists
+    the triangles on the airfoil, ``freestream`` those on the outer circle, one
+    boundary edge each."""
     m = n_around
-    # surface: parameter s in [0, 2pi): x = 0.5(1+cos s); upper for s in (0,pi)
     s = 2.0 * math.pi * np.arange(m) / m
     xs = 0.5 * (1.0 + np.cos(s))
     ys = naca_thickness(xs, thickness) * np.where(s <= math.pi, 1.0, -1.0)
     ys[0] = 0.0
-    # counter-clockwise circle around mid-chord matching each surface point
+    if m % 2 == 0:
+        ys[m // 2] = 0.0
     xc0 = 0.5
     ang = np.arctan2(ys, xs - xc0)
-    ang = np.unwrap(ang)
-    r_lay = _growth_ratio(first_spacing, n_layers, far_radius)
-    dist = first_spacing * (r_lay ** np.arange(n_layers + 1) - 1.0) / (r_lay - 1.0)
-    # surface normals (outward) from central differences along the surface
-    dx = np.roll(xs, -1) - np.roll(xs, 1)
-    dy = np.roll(ys, -1) - np.roll(ys, 1)
-    ln = np.hypot(dx, dy)
-    nxs, nys = dy / ln, -dx / ln  # surface runs CCW (upper first, TE->LE): outward = right normal
-    # make sure normals point away from the chord centre
-    sgn = np.sign((xs - xc0) * nxs + ys * nys)
-    sgn[sgn == 0] = 1.0
-    nxs, nys = nxs * sgn, nys * sgn
+    # outward unit normals from central differences along the surface
+    tx = np.roll(xs, -1) - np.roll(xs, 1)
+    ty = np.roll(ys, -1) - np.roll(ys, 1)
+    ln = np.hypot(tx, ty)
+    nxs, nys = ty / ln, -tx / ln
+    flip = (nxs * (xs - xc0) + nys * ys) < 0
+    nxs[flip], nys[flip] = -nxs[flip], -nys[flip]
+    rx, ry = np.cos(ang), np.sin(ang)
+    cx, cy = xc0 + far_radius * rx, far_radius * ry
+    span = np.hypot(cx - xs, cy - ys)
+    r = _growth_ratio(first_spacing, n_layers, far_radius)
+    d = first_spacing * (r ** np.arange(n_layers + 1) - 1.0) / (r - 1.0)
     xy = np.empty(((n_layers + 1) * m, 2))
     for k in range(n_layers + 1):
-        d = dist[k]
-        blend = min(1.0, d / far_radius) ** 0.5
-        # normal-extruded point and far-circle point
-        px = xs + d * nxs
-        py = ys + d * nys
-        fx = xc0 + (far_radius * (d / far_radius)) * np.cos(ang) if k else xs
-        fy = (far_radius * (d / far_radius)) * np.sin(ang) if k else ys
+        beta = (d[k] / far_radius) ** 0.5
+        dx = (1.0 - beta) * nxs + beta * rx
+        dy = (1.0 - beta) * nys + beta * ry
+        dn = np.hypot(dx, dy)
+        dist = d[k] * span / far_radius
         if k == n_layers:
-            px, py = xc0 + far_radius * np.cos(ang), far_radius * np.sin(ang)
-            blend = 1.0
-        xy[k * m:(k + 1) * m, 0] = (1 - blend) * px + blend * fx
-        xy[k * m:(k + 1) * m, 1] = (1 - blend) * py + blend * fy
-    # quads (k, i) -> (k, i+1) -> (k+1, i+1) -> (k+1, i); orientation fixed below
-    tris = []
+            xy[k * m:(k + 1) * m, 0], xy[k * m:(k + 1) * m, 1] = cx, cy
+        else:
+            xy[k * m:(k + 1) * m, 0] = xs + dist * dx / dn
+            xy[k * m:(k + 1) * m, 1] = ys + dist * dy / dn
     kk, ii = np.meshgrid(np.arange(n_layers), np.arange(m), indexing="ij")
     kk, ii = kk.ravel(), ii.ravel()
     a = kk * m + ii
     b = kk * m + (ii + 1) % m
     c = (kk + 1) * m + (ii + 1) % m
-    d = (kk + 1) * m + ii
-    t1 = np.stack([a, b, c], 1)
-    t2 = np.stack([a, c, d], 1)
+    dd = (kk + 1) * m + ii
+    # (a, b, c, d) runs clockwise (the fluid lies right of the surface
+    # direction), so the triangles are (c, b, a) and (d, c, a)
     tri = np.empty((2 * a.size, 3), np.int64)
-    tri[0::2] = t1
-    tri[1::2] = t2
+    tri[0::2] = np.stack([c, b, a], 1)
+    tri[1::2] = np.stack([dd, c, a], 1)
     area = _tri_area(xy, tri)
-    if np.all(area < 0):
-        tri = tri[:, ::-1].copy()
-        area = -area
     if not np.all(area > 0):
-        raise RuntimeError("O-grid has inverted cells")
-    # boundary cells: wall edges are (a,b) at k=0 -> triangle t1 of layer 0;
-    # far-field edges (c,d) at k=n_layers-1 -> triangle t2 of the last layer
-    ncq = m
-    wall = np.arange(0, 2 * ncq, 2, dtype=np.int32)
-    far = (2 * (n_layers - 1) * m + 1 + np.arange(0, 2 * ncq, 2)).astype(np.int32)
-    return Mesh(xy, tri.shape[0], 0, tri.astype(np.int32).ravel(),
-                [("slip_wall", wall), ("freestream", far)])
+        raise RuntimeError(f"O-grid has {int(np.sum(area <= 0))} inverted cells")
+    wall = np.arange(0, 2 * m, 2, dtype=np.int32)
+    far = (2 * (n_layers - 1) * m + 1 + np.arange(0, 2 * m, 2)).astype(np.int32)
+    return Mesh(xy, tri.shape[0], 0, tri.astype(np.int32).ravel(), [("slip_wall", wall), ("freestream", far)])
 
 
 # ---------------------------------------------------------------------------

@@ -295,6 +295,51 @@ class Solver:
         check(self._L.fvs_time_integration(self._h, t1, nsteps, ptr(norms), ptr(verr)))
         return (norms, verr) if vortex_error else norms
 
+    def freeze_limiter(self, frozen: bool = True):
+        check(self._L.fvs_freeze_limiter(self._h, int(frozen)))
+
+    def steady_solve(self, max_steps: int, drop: float = 1e-8, chunk: int = 200, t1: float = 0.0,
+                     freeze_at: Optional[float] = None):
+        """Pseudo-time stepping until the density residual of
+        src/runge_kutta.f90:396-398 has dropped by ``drop`` relative to the first
+        step (the reference runs a fixed step count; the stopping rule is this
+        library's).  Norms come back after every ``chunk`` steps; the state may
+        run up to chunk-1 steps past the crossing.  With ``freeze_at`` the
+        limiter is frozen once the drop passes that level (limiter chatter
+        otherwise stalls convergence).  Returns a dict with the iteration of the
+        crossing (or None), the norm history, the iteration at which the limiter
+        was frozen, and whether a non-finite norm (divergence) stopped the loop."""
+        hist = []
+        done = 0
+        conv_at = None
+        frozen_at = None
+        diverged = False
+        r1 = None
+        while done < max_steps:
+            n = min(chunk, max_steps - done)
+            norms = self.time_integration(t1, n)
+            hist.append(norms)
+            if r1 is None:
+                r1 = float(norms[0, 0])
+            if not np.all(np.isfinite(norms)):
+                diverged = True
+                done += n
+                break
+            if freeze_at is not None and frozen_at is None and norms[-1, 0] <= freeze_at * r1:
+                self.freeze_limiter(True)
+                frozen_at = done + n
+            hit = np.nonzero(norms[:, 0] <= drop * r1)[0]
+            if hit.size:
+                conv_at = done + int(hit[0]) + 1
+                done += n
+                break
+            done += n
+        if frozen_at is not None:
+            self.freeze_limiter(False)
+        h = np.concatenate(hist) if hist else np.zeros((0, 4))
+        return dict(iterations=conv_at, steps_run=done, history=h, diverged=diverged, limiter_frozen_at=frozen_at,
+                    drop=float(h[-1, 0] / r1) if len(h) else None)
+
     def error_isentropic_vortex(self, t: float) -> np.ndarray:
         out = np.zeros(13)
         check(self._L.fvs_vortex_error(self._h, t, ptr(out)))

@@ -155,6 +155,10 @@ int fvs_set_timestep(fvs_ctx *ctx, double dt, double cfl);
  * time per launch (ms), the algorithmic bytes per launch and the number of
  * launches timed. */
 int fvs_kernel_times(fvs_ctx *ctx, double t1, int nsteps, double *ms6, double *bytes6, int *launches6);
+/* Steady-state convergence aid, no reference counterpart: while frozen != 0
+ * the limiter is not re-evaluated and every residual uses the limiter values
+ * of the last evaluation. */
+int fvs_freeze_limiter(fvs_ctx *ctx, int frozen);
 /* Launch autotuner, no reference counterpart: times nsteps steps per trial
  * and keeps, per kernel class [gradient, face flux, gather+update, fused
  * flux+update], the faster of plain and XCD-aware workgroup numbering.  The

@@ -13,16 +13,9 @@ def pytest_configure(config):
 
 
 def _has_gpu():
-    import ctypes
-    try:
-        hip = ctypes.CDLL("libamdhip64.so")
-    except OSError:
-        try:
-            hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
-        except OSError:
-            return False
-    n = ctypes.c_int(0)
-    return hip.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+    """A GPU box exposes /dev/kfd; probing through HIP would initialise the
+    runtime in the test process, so only the device node is checked."""
+    return os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK | os.W_OK)
 
 
 def pytest_collection_modifyitems(config, items):
