@@ -19,7 +19,8 @@ RK_CLASSIC, RK_SSP42, RK_SSP3 = 0, 1, 2
 
 # every entry point of include/fvs2d.h
 EXPORTS = (
-    "fvs_last_error", "fvs_create", "fvs_destroy", "fvs_sizes", "fvs_mesh_create", "fvs_mesh_destroy",
+    "fvs_last_error", "fvs_create", "fvs_create_batch", "fvs_batch_time_integration", "fvs_batch_set_active",
+    "fvs_batch_wall_forces", "fvs_set_graph", "fvs_destroy", "fvs_sizes", "fvs_mesh_create", "fvs_mesh_destroy",
     "fvs_ctx_mesh", "fvs_mesh_sizes", "fvs_mesh_topology", "fvs_mesh_lists", "fvs_mesh_lsq_nnz", "fvs_mesh_lsq",
     "fvs_set_state", "fvs_get_state", "fvs_initialize", "fvs_primitives", "fvs_gradient", "fvs_gradient_1var",
     "fvs_limiter", "fvs_compute_residual", "fvs_roe_flux_batch", "fvs_local_dt", "fvs_time_integration",
@@ -59,6 +60,11 @@ def load():
     V, D, I = C.c_void_p, C.c_double, C.c_int
     L.fvs_last_error.restype = C.c_char_p
     L.fvs_create.argtypes = [C.POINTER(V), I, V, I, I, V, V, I, V, V, V, C.POINTER(Options)]
+    L.fvs_create_batch.argtypes = [C.POINTER(V), I, V, V, I, V, I, I, V, I, V, V, V, C.POINTER(Options)]
+    L.fvs_batch_time_integration.argtypes = [V, D, I, V]
+    L.fvs_batch_set_active.argtypes = [V, V]
+    L.fvs_batch_wall_forces.argtypes = [V, V]
+    L.fvs_set_graph.argtypes = [V, I]
     L.fvs_mesh_create.argtypes = [C.POINTER(V), I, V, I, I, V, V, I, V, V, V, I, I, D, I]
     for n in ("fvs_destroy", "fvs_mesh_destroy"):
         getattr(L, n).argtypes = [V]

@@ -28,8 +28,9 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const doubl
                                                       ShiftTable st, double p, double *__restrict__ grad,
                                                       double *__restrict__ phi, int lt, const int *__restrict__ nvrt,
                                                       const double2 *__restrict__ loff,
-                                                      const double *__restrict__ vol) {
-  const int i = xcd_block(nb) * kBlock + threadIdx.x;
+                                                      const double *__restrict__ vol,
+                                                      const int *__restrict__ blist) {
+  const int i = (blist ? blist[blockIdx.x] : xcd_block(nb)) * kBlock + threadIdx.x;
   if (i >= N) return;
   const int s = i / kSlice, l = i % kSlice;
   const int base = soff[s], wid = swid[s];
@@ -128,7 +129,8 @@ __device__ __forceinline__ void interior_flux(const Phys &P, int2 c, const doubl
 
 __device__ __forceinline__ void boundary_flux(const Phys &P, double t, int c, int ty, const double g[8],
                                               const double *__restrict__ W, const double *__restrict__ grad,
-                                              const double *__restrict__ phi, double fa[4], double &wsa) {
+                                              const double *__restrict__ phi, double fa[4], double &wsa,
+                                              const double *__restrict__ pinf) {
   double w[4], gr[8];
   ld4(W, c, w);
   ld8(grad, c, gr);
@@ -137,7 +139,7 @@ __device__ __forceinline__ void boundary_flux(const Phys &P, double t, int c, in
   for (int v = 0; v < 4; ++v) pL[v] = w[v] + ph * (g[3] * gr[v] + g[4] * gr[4 + v]);
   const double nx = g[0], ny = g[1];
   if (ty == FVS_BC_FREESTREAM) {
-    for (int v = 0; v < 4; ++v) pR[v] = P.pinf[v];
+    for (int v = 0; v < 4; ++v) pR[v] = pinf[v];
   } else if (ty == FVS_BC_SLIP_WALL) {
     const double un = pL[1] * nx + pL[2] * ny;
     pR[0] = pL[0];
@@ -171,11 +173,14 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
                                                         const double *__restrict__ vol, double *__restrict__ Ua,
                                                         double *__restrict__ Ub, double *__restrict__ Facc,
                                                         double *__restrict__ Wout, double *__restrict__ dtl,
-                                                        double *__restrict__ part) {
+                                                        double *__restrict__ part, const int *__restrict__ blist,
+                                                        const int *__restrict__ binst,
+                                                        const double *__restrict__ pinf_inst,
+                                                        double *__restrict__ d2out) {
   extern __shared__ double lds[];
   double *lF = lds;             // [maxf][4] flux*area
   double *lS = lds + 4 * maxf;  // [maxf]    wave speed*area
-  const int lb = xcd_block(nb);
+  const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
   const int nbl = nb < 0 ? -nb : nb;
   if (lb >= nbl) return;  // the whole workgroup leaves before any barrier
   const int f0 = boff[lb], nf = boff[lb + 1] - f0;
@@ -188,7 +193,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
     } else {
       const int b = slot - Fi;
       ld8(bgeo, b, g);
-      boundary_flux(P, t, bcell[b], btype[b], g, Win, grad, phi, fa, wsa);
+      boundary_flux(P, t, bcell[b], btype[b], g, Win, grad, phi, fa, wsa,
+                    binst ? pinf_inst + 4 * binst[b] : P.pinf);
     }
     for (int v = 0; v < 4; ++v) lF[4 * f + v] = fa[v];
     lS[f] = wsa;
@@ -253,6 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
         const double d = fabs(un[v] - u0[v]);
         d2[v] = d * d;
       }
+      if (d2out) st4(d2out, i, d2);
     } else if (A.mode == M_SSP3) {
       st4(Ub, i, un);
     }
@@ -265,4 +272,62 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
     if (threadIdx.x == 0)
       for (int v = 0; v < 4; ++v) part[4 * lb + v] = d2[v];
   }
+}
+
+// per-instance residual norms of one step from the per-cell squared
+// increments: one workgroup per instance, fixed-shape tree (deterministic)
+__global__ __launch_bounds__(kBlock) void k_inst_norms(int ninst, const int *__restrict__ off,
+                                                       const double *__restrict__ d2, const int *__restrict__ ctr,
+                                                       double *__restrict__ out) {
+  __shared__ double red[4 * kBlock];
+  const int k = blockIdx.x;
+  if (k >= ninst) return;
+  const int a = off[k], b = off[k + 1];
+  double s[4] = {0, 0, 0, 0};
+  for (int i = a + threadIdx.x; i < b; i += kBlock) {
+    double q[4];
+    ld4(d2, i, q);
+    for (int v = 0; v < 4; ++v) s[v] = s[v] + q[v];
+  }
+  block_sum4(s, red);
+  if (threadIdx.x == 0) {
+    double *o = out + 4 * (long(ctr ? ctr[0] : 0) * ninst + k);
+    for (int v = 0; v < 4; ++v) o[v] = sqrt(s[v] / double(b - a));
+  }
+}
+
+__global__ void k_counter(int *__restrict__ ctr, int value, int add) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctr[0] = add ? ctr[0] + value : value;
+}
+
+// C_L / C_D per instance: thread k walks instance k's wall faces in order
+__global__ void k_wall_forces_inst(int ninst, const int *__restrict__ woff, const int *__restrict__ wf,
+                                   const int *__restrict__ bcell, const double *__restrict__ geo,
+                                   const double *__restrict__ W, const double *__restrict__ grad,
+                                   const double *__restrict__ mach_aoa, const double *__restrict__ pinf_inst,
+                                   double *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ninst) return;
+  const double M = mach_aoa[2 * k], al = mach_aoa[2 * k + 1] * (kPiD / 180.0);
+  const double pinf = pinf_inst[4 * k + 3];
+  double cn = 0, ca = 0, cn1 = 0, ca1 = 0;
+  for (int q = woff[k]; q < woff[k + 1]; ++q) {
+    const int b = wf[q], c = bcell[b];
+    double g[8], gr[8], w[4];
+    ld8(geo, b, g);
+    ld8(grad, c, gr);
+    ld4(W, c, w);
+    const double pw = w[3] + g[3] * gr[3] + g[4] * gr[7];
+    const double cp = 2.0 / (M * M) * (pw - pinf);
+    const double cp1 = 2.0 / (M * M) * (w[3] - pinf);
+    cn = cn + cp * g[1] * g[2];
+    ca = ca + cp * g[0] * g[2];
+    cn1 = cn1 + cp1 * g[1] * g[2];
+    ca1 = ca1 + cp1 * g[0] * g[2];
+  }
+  const double c = cos(al), sn = sin(al);
+  out[4 * k] = cn * c - ca * sn;
+  out[4 * k + 1] = cn * sn + ca * c;
+  out[4 * k + 2] = cn1 * c - ca1 * sn;
+  out[4 * k + 3] = cn1 * sn + ca1 * c;
 }

@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -292,7 +293,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_bnd(int Fb, int Fi, Phys P, dou
                                                      const int *__restrict__ btype, const double *__restrict__ geo,
                                                      const double *__restrict__ W, const double *__restrict__ grad,
                                                      const double *__restrict__ phi, double *__restrict__ FA,
-                                                     double *__restrict__ WSA) {
+                                                     double *__restrict__ WSA, const int *__restrict__ binst,
+                                                     const double *__restrict__ pinf_inst) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= Fb) return;
   const int c = bcell[b];
@@ -306,7 +308,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_bnd(int Fb, int Fi, Phys P, dou
   const int ty = btype[b];
   const double nx = g[0], ny = g[1];
   if (ty == FVS_BC_FREESTREAM) {
-    for (int v = 0; v < 4; ++v) pR[v] = P.pinf[v];
+    const double *pinf = binst ? pinf_inst + 4 * binst[b] : P.pinf;
+    for (int v = 0; v < 4; ++v) pR[v] = pinf[v];
   } else if (ty == FVS_BC_SLIP_WALL) {
     const double un = pL[1] * nx + pL[2] * ny;
     pR[0] = pL[0];
@@ -445,7 +448,7 @@ __global__ __launch_bounds__(kBlock) void k_update(int nbx, int N, int gw, Upd A
 // (src/runge_kutta.f90:169-184), partials summed in block order
 constexpr int kNormThreads = 1024;
 __global__ __launch_bounds__(kNormThreads) void k_norm_final(int nparts, int N, const double *__restrict__ part,
-                                                             double *__restrict__ out) {
+                                                             double *__restrict__ out, const int *__restrict__ ctr) {
   __shared__ double red[4 * kNormThreads];
   const int t = threadIdx.x;
   double s[4] = {0, 0, 0, 0};
@@ -458,8 +461,10 @@ __global__ __launch_bounds__(kNormThreads) void k_norm_final(int nparts, int N, 
       for (int v = 0; v < 4; ++v) red[v * kNormThreads + t] = red[v * kNormThreads + t] + red[v * kNormThreads + t + h];
     __syncthreads();
   }
-  if (t == 0)
-    for (int v = 0; v < 4; ++v) out[v] = sqrt(red[v * kNormThreads] / double(N));
+  if (t == 0) {
+    double *o = out + 4 * (ctr ? ctr[0] : 0);
+    for (int v = 0; v < 4; ++v) o[v] = sqrt(red[v * kNormThreads] / double(N));
+  }
 }
 
 // isentropic-vortex error over interior cells (src/mms.f90:315-361)
@@ -627,6 +632,17 @@ struct fvs_ctx {
   bool xcd_grad = false, xcd_flux = false, xcd_upd = false, xcd_fused = true;
   // limiter values frozen at their last evaluation (steady convergence aid)
   bool limiter_frozen = false;
+  // batched instances (fvs_create_batch): contiguous device ranges
+  int ninst = 0;
+  std::vector<int> inst_of_real;     // reference order
+  std::vector<int> inst_off_dev;     // device order, ninst+1
+  std::vector<double> mach_aoa;      // 2 per instance
+  int *dinst_off = nullptr, *dbinst = nullptr, *dwoff = nullptr, *blist = nullptr;
+  double *dpinf_inst = nullptr, *d2buf = nullptr, *dmach_aoa = nullptr, *dforces = nullptr;
+  int nact = 0;                      // active blocks (0 = all, no list)
+  // step counter for norm outputs (graph replays), graph mode switch
+  int *dctr = nullptr;
+  bool use_graph = false;
   int nparts = 0;
   bool have_state = false;
   double last_ms = 0.0;
@@ -664,7 +680,7 @@ struct fvs_ctx {
 namespace {
 
 // Morton order of cell centroids: compact blocks of consecutive cells
-std::vector<int> morton_order(const MeshPrep &m) {
+std::vector<int> morton_order(const MeshPrep &m, const std::vector<int> &inst) {
   const int N = m.ncells;
   double xa = *std::min_element(m.cx.begin(), m.cx.end()), xb = *std::max_element(m.cx.begin(), m.cx.end());
   double ya = *std::min_element(m.cy.begin(), m.cy.end()), yb = *std::max_element(m.cy.begin(), m.cy.end());
@@ -679,14 +695,15 @@ std::vector<int> morton_order(const MeshPrep &m) {
     v = (v | (v << 1)) & 0x5555555555555555ull;
     return v;
   };
-  std::vector<std::pair<uint64_t, int>> key(N);
+  // instances stay contiguous (instance-major), Morton order inside each
+  std::vector<std::tuple<int, uint64_t, int>> key(N);
   for (int c = 0; c < N; ++c) {
     uint64_t ix = uint64_t((m.cx[c] - xa) / s * 4294967295.0), iy = uint64_t((m.cy[c] - ya) / s * 4294967295.0);
-    key[c] = {spread(ix) | (spread(iy) << 1), c};
+    key[c] = {inst.empty() ? 0 : inst[c], spread(ix) | (spread(iy) << 1), c};
   }
   std::sort(key.begin(), key.end());
   std::vector<int> perm(N);
-  for (int c = 0; c < N; ++c) perm[c] = key[c].second;
+  for (int c = 0; c < N; ++c) perm[c] = std::get<2>(key[c]);
   return perm;
 }
 
@@ -729,7 +746,7 @@ int setup_device(fvs_ctx &C) {
   MeshPrep &m = C.mp;
   const int N = C.N = m.ncells;
   C.perm.resize(N);
-  if (C.opt.reorder == 1) C.perm = morton_order(m);
+  if (C.opt.reorder == 1) C.perm = morton_order(m, C.inst_of_real);
   else std::iota(C.perm.begin(), C.perm.end(), 0);
   C.iperm.resize(N);
   for (int i = 0; i < N; ++i) C.iperm[C.perm[i]] = i;
@@ -994,6 +1011,38 @@ int setup_device(fvs_ctx &C) {
   C.nparts = std::max(nblk(N), nblk(std::max(C.ncell_intr, 1)));
   if (int rc = C.alloc(C.part, 12 * size_t(C.nparts))) return rc;
   if (int rc = C.alloc(C.small, 64)) return rc;
+  if (int rc = C.alloc(C.dctr, 1)) return rc;
+  if (C.ninst > 0) {
+    // per-instance ranges in device order, free streams, wall-face ranges
+    std::vector<int> off(C.ninst + 1, 0), binst(Fb), woff(C.ninst + 1, 0);
+    for (int i = 0; i < N; ++i) off[C.inst_of_real[P[i]] + 1]++;
+    for (int k = 0; k < C.ninst; ++k) off[k + 1] += off[k];
+    for (int i = 1; i < N; ++i)
+      if (C.inst_of_real[P[i]] < C.inst_of_real[P[i - 1]]) return fail(FVS_EINVAL, "instances not contiguous");
+    C.inst_off_dev = off;
+    for (int q = 0; q < Fb; ++q) binst[q] = C.inst_of_real[P[fbc[q]]];
+    std::vector<double> pinf(4 * size_t(C.ninst));
+    const double kPi = 3.14159265358979323846;
+    for (int k = 0; k < C.ninst; ++k) {
+      pinf[4 * k] = 1.0;
+      pinf[4 * k + 1] = C.mach_aoa[2 * k] * std::cos(C.mach_aoa[2 * k + 1] * (kPi / 180.0));
+      pinf[4 * k + 2] = C.mach_aoa[2 * k] * std::sin(C.mach_aoa[2 * k + 1] * (kPi / 180.0));
+      pinf[4 * k + 3] = 1.0 / C.opt.gamma;
+    }
+    std::vector<int> wsorted(wall);
+    std::stable_sort(wsorted.begin(), wsorted.end(), [&](int a, int b2) { return binst[a] < binst[b2]; });
+    for (int w : wsorted) woff[binst[w] + 1]++;
+    for (int k = 0; k < C.ninst; ++k) woff[k + 1] += woff[k];
+    if (int rc = C.upload(C.dinst_off, off)) return rc;
+    if (int rc = C.upload(C.dbinst, binst)) return rc;
+    if (int rc = C.upload(C.dpinf_inst, pinf)) return rc;
+    if (int rc = C.upload(C.dmach_aoa, C.mach_aoa)) return rc;
+    if (int rc = C.upload(C.dwoff, woff)) return rc;
+    HIPC(hipMemcpyAsync(C.wall, wsorted.data(), wsorted.size() * sizeof(int), hipMemcpyHostToDevice, C.stream));
+    if (int rc = C.alloc(C.d2buf, n4)) return rc;
+    if (int rc = C.alloc(C.dforces, 4 * size_t(C.ninst))) return rc;
+    if (int rc = C.alloc(C.blist, size_t(nblk(N)))) return rc;
+  }
   // phi = 1 (no limiter) or 0 (first order); grad = 0 for first order
   const double phi0 = (C.opt.recon == 1) ? 0.0 : 1.0;
   hipLaunchKernelGGL(k_fill, dim3(nblk(N)), dim3(kBlock), 0, C.stream, long(N), phi0, C.phi);
@@ -1037,10 +1086,11 @@ int launch_gradient(fvs_ctx &C, bool with_limiter) {
   dim3 g(xcd_grid(nblk(N))), b(kBlock);
   ProfScope ps(C, KC_GRAD);
   if (C.lsqc) {
+    if (C.nact) g = dim3(C.nact);
     const int nb = nblk(N);
 #define FVS_LSQC(L, PM)                                                                                          \
   hipLaunchKernelGGL((k_grad_lsqc<L, PM>), g, b, 0, C.stream, N, xcd_arg(nb, C.xcd_grad), C.W, C.cxy, C.ginv, C.soff, C.swid, C.sid,  \
-                     C.shifts, C.opt.lsq_p, C.grad, C.phi, lt, C.nvrt, C.loff, C.vol)
+                     C.shifts, C.opt.lsq_p, C.grad, C.phi, lt, C.nvrt, C.loff, C.vol, C.nact ? C.blist : nullptr)
     if (lim) {
       if (C.pmode == 0) FVS_LSQC(1, 0);
       else if (C.pmode == 1) FVS_LSQC(1, 1);
@@ -1092,7 +1142,7 @@ int launch_fluxes(fvs_ctx &C, double t) {
   if (C.Fb > 0) {
     ProfScope ps(C, KC_FLUX_BND);
     hipLaunchKernelGGL(k_flux_bnd, dim3(nblk(C.Fb)), dim3(kBlock), 0, C.stream, C.Fb, C.Fi, C.P, t, C.fb_cell,
-                       C.fb_type, C.fb_geo, C.W, C.grad, C.phi, C.FA, C.WSA);
+                       C.fb_type, C.fb_geo, C.W, C.grad, C.phi, C.FA, C.WSA, C.dbinst, C.dpinf_inst);
   }
   HIPC(hipGetLastError());
   return 0;
@@ -1115,10 +1165,11 @@ int launch_flux_update(fvs_ctx &C, const Upd &A, double t) {
     ProfScope ps(C, KC_FUSED);
     const int nb = nblk(C.N);
     const size_t lds = size_t(C.maxf) * 5 * sizeof(double);
-    hipLaunchKernelGGL(k_flux_update, dim3(xcd_grid(nb)), dim3(kBlock), lds, C.stream, C.N, xcd_arg(nb, C.xcd_fused),
-                       C.GW, C.maxf, A,
-                       C.P, t, C.boff, C.bfaces, C.lgat, C.Fi, C.fi_cells, C.fi_geo, C.fb_cell, C.fb_type, C.fb_geo,
-                       C.W, C.grad, C.phi, C.vol, C.U0, C.Ub, C.Facc, C.Walt, C.dtl, C.part);
+    hipLaunchKernelGGL(k_flux_update, dim3(C.nact ? C.nact : xcd_grid(nb)), dim3(kBlock), lds, C.stream, C.N,
+                       xcd_arg(nb, C.xcd_fused), C.GW, C.maxf, A, C.P, t, C.boff, C.bfaces, C.lgat, C.Fi, C.fi_cells,
+                       C.fi_geo, C.fb_cell, C.fb_type, C.fb_geo, C.W, C.grad, C.phi, C.vol, C.U0, C.Ub, C.Facc,
+                       C.Walt, C.dtl, C.part, C.nact ? C.blist : nullptr, C.dbinst, C.dpinf_inst,
+                       A.last ? C.d2buf : nullptr);
   }
   HIPC(hipGetLastError());
   std::swap(C.W, C.Walt);
@@ -1164,9 +1215,12 @@ extern "C" {
 
 const char *fvs_last_error(void) { return g_err.c_str(); }
 
-int fvs_create(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
-               const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types, const int *bnd_cells,
-               const fvs_options *opt) {
+}  // extern "C"
+
+static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
+                       const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types,
+                       const int *bnd_cells, const fvs_options *opt, int ninst, const int *cell_offsets,
+                       const double *mach_aoa) {
   if (!out || !xy || !cell_nodes || !opt || (nb > 0 && (!bnd_ncells || !bnd_types)))
     return fail(FVS_EINVAL, "fvs_create: null argument");
   *out = nullptr;
@@ -1180,6 +1234,23 @@ int fvs_create(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad,
   if (hipGetDeviceCount(&devs) != hipSuccess || devs < 1) return fail(FVS_EHIP, "no HIP device available");
   auto *C = new fvs_ctx();
   C->opt = *opt;
+  if (ninst > 0) {
+    const int N = ntri + nquad;
+    if (!cell_offsets || !mach_aoa || cell_offsets[0] != 0 || cell_offsets[ninst] != N) {
+      delete C;
+      return fail(FVS_EINVAL, "fvs_create_batch: cell_offsets must run from 0 to ncells");
+    }
+    C->ninst = ninst;
+    C->inst_of_real.resize(N);
+    for (int k = 0; k < ninst; ++k) {
+      if (cell_offsets[k + 1] < cell_offsets[k]) {
+        delete C;
+        return fail(FVS_EINVAL, "fvs_create_batch: cell_offsets must not decrease");
+      }
+      for (int c = cell_offsets[k]; c < cell_offsets[k + 1]; ++c) C->inst_of_real[c] = k;
+    }
+    C->mach_aoa.assign(mach_aoa, mach_aoa + 2 * size_t(ninst));
+  }
   SchemeFlags fl;
   fl.grad_method = opt->grad_method;
   fl.lsq_nn = opt->lsq_nn;
@@ -1219,6 +1290,23 @@ int fvs_create(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad,
   }
   *out = C;
   return 0;
+}
+
+extern "C" {
+
+int fvs_create(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
+               const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types, const int *bnd_cells,
+               const fvs_options *opt) {
+  return create_impl(out, nnodes, xy, ntri, nquad, cell_nodes, node_canon, nb, bnd_ncells, bnd_types, bnd_cells, opt,
+                     0, nullptr, nullptr);
+}
+
+int fvs_create_batch(fvs_ctx **out, int ninst, const int *cell_offsets, const double *mach_aoa, int nnodes,
+                     const double *xy, int ntri, int nquad, const int *cell_nodes, int nb, const int *bnd_ncells,
+                     const int *bnd_types, const int *bnd_cells, const fvs_options *opt) {
+  if (ninst <= 0) return fail(FVS_EINVAL, "fvs_create_batch: ninst must be positive");
+  return create_impl(out, nnodes, xy, ntri, nquad, cell_nodes, nullptr, nb, bnd_ncells, bnd_types, bnd_cells, opt,
+                     ninst, cell_offsets, mach_aoa);
 }
 
 int fvs_destroy(fvs_ctx *ctx) {
@@ -1504,74 +1592,140 @@ int fvs_roe_flux_batch(int n, double gamma, const double *pL, const double *pR, 
   return rc;
 }
 
-int fvs_time_integration(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr) {
-  if (int rc = need_state(C)) return rc;
-  if (nsteps < 0) return fail(FVS_EINVAL, "nsteps < 0");
-  if (nsteps == 0) return 0;
-  const fvs_options &o = C->opt;
+}  // extern "C"
+
+namespace {
+
+// The stage sequence of one time step (src/runge_kutta.f90:120-418 and the
+// SSP-RK3 of this library).  Norm outputs go to dn / dinst at the step index
+// held in the device counter, which the step advances; so the same launch
+// sequence can be captured once in a HIP graph and replayed.
+int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {
+  const fvs_options &o = C.opt;
   const double dt = o.dt;
-  double *dn = nullptr, *dv = nullptr;
-  HIPC(hipMalloc(&dn, std::max(1, nsteps) * 4 * sizeof(double)));
-  if (verr && hipMalloc(&dv, size_t(nsteps) * 13 * sizeof(double)) != hipSuccess) {
-    (void)hipFree(dn);
-    return fail(FVS_EHIP, "hipMalloc failed");
-  }
-  // runge_kutta_init coefficient tables (src/runge_kutta.f90:36-65)
-  double ak = 1, bk = 2, ck = 2, dk = 1;
+  double ak = 1, bk = 2, ck = 2, dk = 1;  // runge_kutta_init, src/runge_kutta.f90:36-51
   if (o.rk_order == 1) ak = 3.60897, bk = 2.04, ck = 0.34206, dk = 0.00897;
   if (o.rk_order == 2) ak = 0.11, bk = 3.92, ck = 1.86, dk = 0.11;
   if (o.rk_order == 3) ak = 0.65, bk = 2.7, ck = 2.0, dk = 0.65;
   const double rk_coef[4] = {ak, bk, ck, dk};
   const double h_rk[4] = {dt / 2.0, dt / 2.0, dt, dt / 6.0};
-  const double dts[4] = {0.0, dt / 3.0, dt * 2.0 / 3.0, dt};
+  const double dts[4] = {0.0, dt / 3.0, dt * 2.0 / 3.0, dt};  // src/runge_kutta.f90:56-65
   const double ssp3_a[3] = {0.0, 0.75, 1.0 / 3.0}, ssp3_b[3] = {1.0, 0.25, 2.0 / 3.0};
   const int nst = (o.rk_scheme == FVS_RK_SSP3) ? 3 : 4;
-  int rc = 0;
-  HIPC(hipEventRecord(C->ev0, C->stream));
-  for (int s = 0; s < nsteps && !rc; ++s) {
-    const double told = t1 + double(s) * dt;
-    for (int k = 0; k < nst && !rc; ++k) {
-      Upd A{};
-      A.steady = o.steady;
-      A.first = (k == 0);
-      A.last = (k == nst - 1);
-      A.cfl = o.cfl;
-      A.gamma = o.gamma;
-      double ts = told;
-      if (o.rk_scheme == FVS_RK_SSP3) {
-        A.mode = M_SSP3;
-        A.a = ssp3_a[k];
-        A.b = ssp3_b[k];
-        A.h = o.steady ? 1.0 : dt;
-        ts = (k == 0) ? told : (k == 1 ? told + dt : told + 0.5 * dt);
-      } else if (o.rk_scheme == FVS_RK_SSP42) {
-        A.mode = M_SSP42;
-        A.coef = 1.0;
-        A.h = o.steady ? (k == 3 ? 1.0 / 4.0 : 1.0 / 3.0) : (k == 3 ? dt / 4.0 : dt / 3.0);
-        ts = told + dts[k];
-      } else {
-        A.mode = M_CLASSIC;
-        A.coef = rk_coef[k];
-        A.use_acc = (k == 3);
-        A.h = o.steady ? (k == 2 ? 1.0 : (k == 3 ? 1.0 / 6.0 : 1.0 / 2.0)) : h_rk[k];
-        const double t12 = told + 0.5 * dt, tnew = told + dt;
-        const double tst[4] = {told, t12, t12, tnew};
-        ts = tst[k];
-      }
-      if (C->fused) {
-        rc = launch_flux_update(*C, A, ts);
-      } else {
-        rc = launch_fluxes(*C, ts);
-        if (!rc) rc = launch_update(*C, A);
-      }
+  for (int k = 0; k < nst; ++k) {
+    Upd A{};
+    A.steady = o.steady;
+    A.first = (k == 0);
+    A.last = (k == nst - 1);
+    A.cfl = o.cfl;
+    A.gamma = o.gamma;
+    double ts = told;
+    if (o.rk_scheme == FVS_RK_SSP3) {
+      A.mode = M_SSP3;
+      A.a = ssp3_a[k];
+      A.b = ssp3_b[k];
+      A.h = o.steady ? 1.0 : dt;
+      ts = (k == 0) ? told : (k == 1 ? told + dt : told + 0.5 * dt);
+    } else if (o.rk_scheme == FVS_RK_SSP42) {
+      A.mode = M_SSP42;
+      A.coef = 1.0;
+      A.h = o.steady ? (k == 3 ? 1.0 / 4.0 : 1.0 / 3.0) : (k == 3 ? dt / 4.0 : dt / 3.0);
+      ts = told + dts[k];
+    } else {
+      A.mode = M_CLASSIC;
+      A.coef = rk_coef[k];
+      A.use_acc = (k == 3);
+      A.h = o.steady ? (k == 2 ? 1.0 : (k == 3 ? 1.0 / 6.0 : 1.0 / 2.0)) : h_rk[k];
+      const double t12 = told + 0.5 * dt, tnew = told + dt;
+      const double tst[4] = {told, t12, t12, tnew};
+      ts = tst[k];
     }
-    if (rc) break;
-    {
-      ProfScope ps(*C, KC_NORM);
-      hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kNormThreads), 0, C->stream, nblk(C->N), C->N, C->part, dn + 4 * s);
+    int rc;
+    if (C.fused) {
+      rc = launch_flux_update(C, A, ts);
+    } else {
+      rc = launch_fluxes(C, ts);
+      if (!rc) rc = launch_update(C, A);
     }
-    if (dv) rc = launch_vortex_err(*C, told + dt, dv + 13 * size_t(s));
+    if (rc) return rc;
   }
+  if (!C.nact) {
+    ProfScope ps(C, KC_NORM);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kNormThreads), 0, C.stream, nblk(C.N), C.N, C.part, dn, C.dctr);
+  }
+  if (dinst && C.ninst > 0)
+    hipLaunchKernelGGL(k_inst_norms, dim3(C.ninst), dim3(kBlock), 0, C.stream, C.ninst, C.dinst_off, C.d2buf, C.dctr,
+                       dinst);
+  hipLaunchKernelGGL(k_counter, dim3(1), dim3(1), 0, C.stream, C.dctr, 1, 1);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+bool time_dependent(const fvs_ctx &C) {
+  if (!C.opt.lvortex) return false;
+  for (int b = 0; b < C.mp.nb; ++b)
+    if (C.mp.bnd_type[b] == FVS_BC_DIRICHLET) return true;
+  return false;
+}
+
+int run_steps(fvs_ctx &C, double t1, int nsteps, double *dn, double *dv, double *dinst) {
+  const double dt = C.opt.dt;
+  const int nst = (C.opt.rk_scheme == FVS_RK_SSP3) ? 3 : 4;
+  hipLaunchKernelGGL(k_counter, dim3(1), dim3(1), 0, C.stream, C.dctr, 0, 0);
+  HIPC(hipGetLastError());
+  int s = 0;
+  // graph replay: one captured graph covers an even number of stages so the
+  // primitive-state buffers are back in place after each replay
+  if (C.use_graph && !C.profiling && !dv && !time_dependent(C)) {
+    const int gsteps = (nst % 2 == 0) ? 1 : 2;
+    if (nsteps >= 2 * gsteps) {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      HIPC(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+      int rc = 0;
+      for (int g = 0; g < gsteps && !rc; ++g) rc = launch_step(C, t1, dn, dinst);
+      hipError_t e = hipStreamEndCapture(C.stream, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(FVS_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(FVS_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+      }
+      for (; s + gsteps <= nsteps; s += gsteps) {
+        e = hipGraphLaunch(exec, C.stream);
+        if (e != hipSuccess) break;
+      }
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return fail(FVS_EHIP, std::string("graph launch: ") + hipGetErrorString(e));
+    }
+  }
+  for (; s < nsteps; ++s) {
+    const double told = t1 + double(s) * dt;
+    if (int rc = launch_step(C, told, dn, dinst)) return rc;
+    if (dv)
+      if (int rc = launch_vortex_err(C, told + dt, dv + 13 * size_t(s))) return rc;
+  }
+  return 0;
+}
+
+int integrate(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr, double *inst_norms) {
+  if (int rc = need_state(C)) return rc;
+  if (nsteps < 0) return fail(FVS_EINVAL, "nsteps < 0");
+  if (nsteps == 0) return 0;
+  double *dn = nullptr, *dv = nullptr, *di = nullptr;
+  HIPC(hipMalloc(&dn, size_t(nsteps) * 4 * sizeof(double)));
+  int rc = 0;
+  if (verr && hipMalloc(&dv, size_t(nsteps) * 13 * sizeof(double)) != hipSuccess) rc = fail(FVS_EHIP, "hipMalloc");
+  if (!rc && inst_norms && C->ninst > 0 &&
+      hipMalloc(&di, size_t(nsteps) * C->ninst * 4 * sizeof(double)) != hipSuccess)
+    rc = fail(FVS_EHIP, "hipMalloc");
+  if (!rc && hipEventRecord(C->ev0, C->stream) != hipSuccess) rc = fail(FVS_EHIP, "hipEventRecord");
+  if (!rc) rc = run_steps(*C, t1, nsteps, dn, dv, di);
   if (!rc) {
     hipError_t e = hipEventRecord(C->ev1, C->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(C->stream);
@@ -1584,9 +1738,81 @@ int fvs_time_integration(fvs_ctx *C, double t1, int nsteps, double *norms, doubl
     rc = fail(FVS_EHIP, "copy of residual norms failed");
   if (!rc && verr && hipMemcpy(verr, dv, size_t(nsteps) * 13 * 8, hipMemcpyDeviceToHost) != hipSuccess)
     rc = fail(FVS_EHIP, "copy of vortex errors failed");
+  if (!rc && di && hipMemcpy(inst_norms, di, size_t(nsteps) * C->ninst * 4 * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(FVS_EHIP, "copy of instance norms failed");
   (void)hipFree(dn);
   if (dv) (void)hipFree(dv);
+  if (di) (void)hipFree(di);
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvs_time_integration(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr) {
+  if (C && C->nact && norms) {
+    // with inactive instances the whole-mesh norm would mix stale partials
+    std::fill(norms, norms + 4 * size_t(std::max(nsteps, 0)), std::nan(""));
+    return integrate(C, t1, nsteps, nullptr, verr, nullptr);
+  }
+  return integrate(C, t1, nsteps, norms, verr, nullptr);
+}
+
+int fvs_set_graph(fvs_ctx *C, int on) {
+  if (!C) return fail(FVS_EINVAL, "null context");
+  C->use_graph = on != 0;
+  return 0;
+}
+
+int fvs_batch_time_integration(fvs_ctx *C, double t1, int nsteps, double *inst_norms) {
+  if (!C) return fail(FVS_EINVAL, "null context");
+  if (C->ninst <= 0) return fail(FVS_ESTATE, "not a batch context (use fvs_create_batch)");
+  return integrate(C, t1, nsteps, nullptr, nullptr, inst_norms);
+}
+
+int fvs_batch_set_active(fvs_ctx *C, const int *active) {
+  if (!C || !active) return fail(FVS_EINVAL, "null argument");
+  if (C->ninst <= 0) return fail(FVS_ESTATE, "not a batch context (use fvs_create_batch)");
+  if (!C->fused || !(C->lsqc || C->opt.recon == 1))
+    return fail(FVS_ESTATE, "instance masking needs the fused kernels and the centroid-form LSQ gradient");
+  const int nb = nblk(C->N);
+  std::vector<int> list;
+  bool all = true;
+  for (int k = 0; k < C->ninst; ++k) all = all && active[k];
+  if (!all) {
+    for (int b = 0; b < nb; ++b) {
+      const int i0 = b * kBlock, i1 = std::min(C->N, i0 + kBlock) - 1;
+      const int k0 = int(std::upper_bound(C->inst_off_dev.begin(), C->inst_off_dev.end(), i0) - C->inst_off_dev.begin()) - 1;
+      const int k1 = int(std::upper_bound(C->inst_off_dev.begin(), C->inst_off_dev.end(), i1) - C->inst_off_dev.begin()) - 1;
+      bool on = false;
+      for (int k = k0; k <= k1 && !on; ++k) on = active[k] != 0;
+      if (on) list.push_back(b);
+    }
+    if (list.empty()) list.push_back(0);
+    HIPC(hipMemcpyAsync(C->blist, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+    HIPC(hipStreamSynchronize(C->stream));
+  }
+  C->nact = all ? 0 : int(list.size());
+  return 0;
+}
+
+int fvs_batch_wall_forces(fvs_ctx *C, double *out) {
+  if (int rc = need_state(C)) return rc;
+  if (C->ninst <= 0) return fail(FVS_ESTATE, "not a batch context (use fvs_create_batch)");
+  if (!out) return fail(FVS_EINVAL, "null argument");
+  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  const int save = C->nact;
+  C->nact = 0;
+  int rc = launch_gradient(*C, false);
+  C->nact = save;
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_wall_forces_inst, dim3(nblk(C->ninst)), dim3(kBlock), 0, C->stream, C->ninst, C->dwoff,
+                     C->wall, C->fb_cell, C->fb_geo, C->W, C->grad, C->dmach_aoa, C->dpinf_inst, C->dforces);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(out, C->dforces, 4 * size_t(C->ninst) * 8, hipMemcpyDeviceToHost, C->stream));
+  HIPC(hipStreamSynchronize(C->stream));
+  return 0;
 }
 
 int fvs_vortex_error(fvs_ctx *C, double t, double *out13) {

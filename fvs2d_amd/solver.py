@@ -298,15 +298,20 @@ class Solver:
     def freeze_limiter(self, frozen: bool = True):
         check(self._L.fvs_freeze_limiter(self._h, int(frozen)))
 
+    def set_graph(self, on: bool = True):
+        """Replay each time step from a captured HIP graph (fvs_set_graph)."""
+        check(self._L.fvs_set_graph(self._h, int(on)))
+
     def steady_solve(self, max_steps: int, drop: float = 1e-8, chunk: int = 200, t1: float = 0.0,
-                     freeze_at: Optional[float] = None):
+                     freeze_at: Optional[float] = None, freeze_after: Optional[int] = None):
         """Pseudo-time stepping until the density residual of
         src/runge_kutta.f90:396-398 has dropped by ``drop`` relative to the first
         step (the reference runs a fixed step count; the stopping rule is this
         library's).  Norms come back after every ``chunk`` steps; the state may
         run up to chunk-1 steps past the crossing.  With ``freeze_at`` the
-        limiter is frozen once the drop passes that level (limiter chatter
-        otherwise stalls convergence).  Returns a dict with the iteration of the
+        limiter is frozen once the drop passes that level, with
+        ``freeze_after`` after that many steps (limiter chatter otherwise
+        stalls convergence).  Returns a dict with the iteration of the
         crossing (or None), the norm history, the iteration at which the limiter
         was frozen, and whether a non-finite norm (divergence) stopped the loop."""
         hist = []
@@ -317,6 +322,12 @@ class Solver:
         r1 = None
         while done < max_steps:
             n = min(chunk, max_steps - done)
+            if freeze_after is not None and frozen_at is None:
+                if done >= freeze_after:
+                    self.freeze_limiter(True)
+                    frozen_at = done
+                else:
+                    n = min(n, freeze_after - done)
             norms = self.time_integration(t1, n)
             hist.append(norms)
             if r1 is None:
@@ -378,6 +389,103 @@ class Solver:
     @property
     def last_elapsed_ms(self) -> float:
         return float(self._L.fvs_last_elapsed_ms(self._h))
+
+
+class BatchSolver(Solver):
+    """Many independent meshes in one context (fvs_create_batch): every RK
+    stage is one launch over all instances, each instance with its own free
+    stream, residual norms, convergence and forces."""
+
+    def __init__(self, meshes, mach_aoa, scheme: Optional[Scheme] = None, **overrides):
+        from .meshgen import concat
+        self.scheme = dataclasses.replace(scheme or Scheme(), **overrides)
+        mesh, coff, _ = concat(meshes)
+        self.ninst = len(meshes)
+        self.cell_offsets = coff.astype(np.int32)
+        self.mach_aoa = np.ascontiguousarray(np.asarray(mach_aoa, np.float64).reshape(self.ninst, 2))
+        L = _lib.load()
+        self._L = L
+        xy, cn, nb, bt, bc, _ = _mesh_arrays(mesh)
+        self._opts = self.scheme.to_c()
+        h = C.c_void_p()
+        check(L.fvs_create_batch(C.byref(h), self.ninst, ptr(self.cell_offsets), ptr(self.mach_aoa), mesh.nnodes,
+                                 ptr(xy), mesh.ntri, mesh.nquad, ptr(cn), len(nb), ptr(nb), ptr(bt), ptr(bc),
+                                 C.byref(self._opts)))
+        self._h = h
+        s = np.zeros(8, np.int32)
+        check(L.fvs_sizes(h, ptr(s)))
+        (self.ncells, self.nedges, self.nnodes, self.nedges_intr, self.nedges_bndr, self.ncells_intr,
+         self.ncells_bndr, self.nb) = (int(v) for v in s)
+        self.mesh = HostMesh._view(L, L.fvs_ctx_mesh(h))
+        self.active = np.ones(self.ninst, np.int32)
+
+    def initialize_freestream(self):
+        """Each instance at its own free stream (src/initialize.f90:36-43)."""
+        g = self.scheme.gamma
+        cv = np.zeros((self.ncells, 4))
+        for k in range(self.ninst):
+            M, a = self.mach_aoa[k]
+            u, v = M * np.cos(np.radians(a)), M * np.sin(np.radians(a))
+            p = 1.0 / g
+            cv[self.cell_offsets[k]:self.cell_offsets[k + 1]] = [1.0, u, v, p / (g - 1) + 0.5 * (u * u + v * v)]
+        self.cvar = cv
+
+    def instance_state(self, k: int) -> np.ndarray:
+        return self.cvar[self.cell_offsets[k]:self.cell_offsets[k + 1]]
+
+    def batch_time_integration(self, t1: float, nsteps: int) -> np.ndarray:
+        out = np.zeros((nsteps, self.ninst, 4))
+        check(self._L.fvs_batch_time_integration(self._h, t1, nsteps, ptr(out)))
+        return out
+
+    def set_active(self, flags):
+        self.active = np.ascontiguousarray(flags, np.int32)
+        check(self._L.fvs_batch_set_active(self._h, ptr(self.active)))
+
+    def wall_forces_batch(self) -> np.ndarray:
+        out = np.zeros((self.ninst, 4))
+        check(self._L.fvs_batch_wall_forces(self._h, ptr(out)))
+        return out
+
+    def steady_solve_batch(self, max_steps: int, drop: float = 1e-8, chunk: int = 200,
+                           freeze_after: Optional[int] = None):
+        """Pseudo-time stepping of all instances until each one's density
+        residual has dropped by ``drop`` from its first step; converged
+        instances are masked out at the next chunk boundary.  The limiter is
+        frozen for all instances after ``freeze_after`` steps."""
+        self.set_active(np.ones(self.ninst, np.int32))
+        iters = np.full(self.ninst, -1, np.int64)
+        r1 = None
+        done = 0
+        frozen = False
+        diverged = np.zeros(self.ninst, bool)
+        while done < max_steps and np.any(self.active):
+            n = min(chunk, max_steps - done)
+            if freeze_after is not None and not frozen:
+                if done >= freeze_after:
+                    self.freeze_limiter(True)
+                    frozen = True
+                else:
+                    n = min(n, freeze_after - done)
+            nr = self.batch_time_integration(0.0, n)[:, :, 0]
+            if r1 is None:
+                r1 = nr[0].copy()
+            for k in np.nonzero(self.active)[0]:
+                if not np.all(np.isfinite(nr[:, k])):
+                    diverged[k] = True
+                    self.active[k] = 0
+                    continue
+                hit = np.nonzero(nr[:, k] <= drop * r1[k])[0]
+                if hit.size:
+                    iters[k] = done + int(hit[0]) + 1
+                    self.active[k] = 0
+            done += n
+            if np.any(self.active):
+                self.set_active(self.active)
+        if frozen:
+            self.freeze_limiter(False)
+        self.set_active(np.ones(self.ninst, np.int32))
+        return dict(iterations=iters, steps_run=done, diverged=diverged)
 
 
 def compute_flux_invscid(pL, pR, nx, ny, gamma=1.4):

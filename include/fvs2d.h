@@ -74,6 +74,28 @@ const char *fvs_last_error(void);
 int fvs_create(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
                const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types,
                const int *bnd_cells, const fvs_options *opt);
+/* Batched instances (no reference counterpart; the reference solves one
+ * case per run): ninst independent meshes concatenated into one (instance k
+ * owns cells [cell_offsets[k], cell_offsets[k+1]) of the concatenation, its
+ * boundaries listed with the others), each with its own free stream
+ * (mach_aoa[2k], mach_aoa[2k+1] in degrees).  Every RK stage is one launch over
+ * all instances; per-instance residual norms, masks and forces below. */
+int fvs_create_batch(fvs_ctx **out, int ninst, const int *cell_offsets, const double *mach_aoa, int nnodes,
+                     const double *xy, int ntri, int nquad, const int *cell_nodes, int nb, const int *bnd_ncells,
+                     const int *bnd_types, const int *bnd_cells, const fvs_options *opt);
+/* nsteps steps of all active instances; inst_norms (nsteps*ninst*4, may be
+ * NULL) receives each instance's log_res.plt norms per step */
+int fvs_batch_time_integration(fvs_ctx *ctx, double t1, int nsteps, double *inst_norms);
+/* active[k] != 0 keeps instance k stepping; the others are left out of the
+ * launches (blocks holding only inactive instances are skipped) */
+int fvs_batch_set_active(fvs_ctx *ctx, const int *active);
+/* C_L, C_D, C_L(cell), C_D(cell) per instance (4*ninst), src/io.f90:340-449 */
+int fvs_batch_wall_forces(fvs_ctx *ctx, double *out);
+/* Launch mode, no reference counterpart: on != 0 captures the launches of one
+ * time step (two for three-stage schemes) in a HIP graph once per
+ * fvs_time_integration call and replays it; time-dependent boundary data
+ * (the Dirichlet vortex) always use stream launches. */
+int fvs_set_graph(fvs_ctx *ctx, int on);
 /* replaces data_sol_close / data_grid_close (src/data_solution.f90:130-140) */
 int fvs_destroy(fvs_ctx *ctx);
 
