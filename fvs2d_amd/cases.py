@@ -8,7 +8,8 @@
 
 ``python -m fvs2d_amd.cases steady --n 512 --layers 256`` runs the steady case
 to a 1e-8 density-residual drop and prints C_L, C_D and the iteration count as
-JSON.
+JSON; ``python -m fvs2d_amd.cases batch --instances 16`` runs drawn instances
+as one batch and reports solves/hr and iteration percentiles.
 """
 from __future__ import annotations
 
@@ -61,6 +62,62 @@ def run_steady(n_around, n_layers, mach, aoa, thickness, cfl, max_steps, drop, l
                 drop_every_1000=[float(h[min(k, len(h) - 1), 0] / h[0, 0]) for k in range(0, len(h), 1000)])
 
 
+RESOLUTIONS = ((128, 64), (256, 128), (384, 192), (512, 256))
+
+
+def draw_instances(n: int, seed: int = 0, resolutions=RESOLUTIONS):
+    """Batch conditions: resolution uniform over ``resolutions``, Mach
+    U[0.5, 0.85], angle of attack U[-2, 4] deg, thickness U[0.10, 0.15]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    res = [resolutions[i] for i in rng.integers(0, len(resolutions), n)]
+    mach = rng.uniform(0.5, 0.85, n)
+    aoa = rng.uniform(-2.0, 4.0, n)
+    thick = rng.uniform(0.10, 0.15, n)
+    return [dict(n_around=r[0], n_layers=r[1], mach=float(m), aoa=float(a), thickness=float(t))
+            for r, m, a, t in zip(res, mach, aoa, thick)]
+
+
+def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, freeze_after: int,
+              resolutions=RESOLUTIONS, sequential: int = 0, chunk: int = 250, graph: bool = True):
+    """Batched steady solves of drawn instances; optionally the first
+    ``sequential`` of them again one at a time for comparison."""
+    from .solver import BatchSolver
+    inst = draw_instances(ninst, seed, resolutions)
+    meshes = [meshgen.naca_ogrid(d["n_around"], d["n_layers"], thickness=d["thickness"]) for d in inst]
+    _, sch = naca(16, 8, cfl=cfl)
+    t0 = time.perf_counter()
+    b = BatchSolver(meshes, [(d["mach"], d["aoa"]) for d in inst], sch)
+    b.set_graph(graph)
+    b.initialize_freestream()
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res = b.steady_solve_batch(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after)
+    sec = time.perf_counter() - t0
+    forces = b.wall_forces_batch()
+    it = res["iterations"]
+    ok = it > 0
+    out = dict(instances=ninst, cells=int(b.ncells), seconds=sec, setup_seconds=setup,
+               converged=int(ok.sum()), diverged=int(res["diverged"].sum()),
+               solves_per_hour=3600.0 * ok.sum() / sec,
+               iterations_p50=float(np.percentile(it[ok], 50)) if ok.any() else None,
+               iterations_p99=float(np.percentile(it[ok], 99)) if ok.any() else None,
+               cl_cd=[[float(f[0]), float(f[1])] for f in forces[:8]])
+    if sequential:
+        t0 = time.perf_counter()
+        n_ok = 0
+        for d, m in list(zip(inst, meshes))[:sequential]:
+            _, sc = naca(16, 8, mach=d["mach"], aoa=d["aoa"], cfl=cfl)
+            s = Solver(m, sc)
+            s.set_graph(graph)
+            s.initialize_solution(1, 0.0)
+            r = s.steady_solve(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after)
+            n_ok += r["iterations"] is not None
+            s.close()
+        sq = time.perf_counter() - t0
+        out["sequential"] = dict(instances=sequential, seconds=sq, solves_per_hour=3600.0 * n_ok / sq)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m fvs2d_amd.cases")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -75,9 +132,24 @@ def main(argv=None):
     st.add_argument("--drop", type=float, default=1e-8)
     st.add_argument("--limiter", type=int, default=1)
     st.add_argument("--freeze-at", type=float, default=None)
+    bt = sub.add_parser("batch")
+    bt.add_argument("--instances", type=int, default=16)
+    bt.add_argument("--seed", type=int, default=0)
+    bt.add_argument("--max-steps", type=int, default=200000)
+    bt.add_argument("--drop", type=float, default=1e-8)
+    bt.add_argument("--cfl", type=float, default=1.25)
+    bt.add_argument("--freeze-after", type=int, default=1000)
+    bt.add_argument("--resolutions", type=str, default="128x64,256x128,384x192,512x256")
+    bt.add_argument("--sequential", type=int, default=0)
+    bt.add_argument("--graph", type=int, default=1)
     a = ap.parse_args(argv)
-    out = run_steady(a.n, a.layers, a.mach, a.aoa, a.thickness, a.cfl, a.max_steps, a.drop, a.limiter,
-                     freeze_at=a.freeze_at)
+    if a.cmd == "steady":
+        out = run_steady(a.n, a.layers, a.mach, a.aoa, a.thickness, a.cfl, a.max_steps, a.drop, a.limiter,
+                         freeze_at=a.freeze_at)
+    else:
+        res = tuple(tuple(int(v) for v in r.split("x")) for r in a.resolutions.split(","))
+        out = run_batch(a.instances, a.seed, a.max_steps, a.drop, a.cfl, a.freeze_after, res, a.sequential,
+                        graph=bool(a.graph))
     print(json.dumps(out))
     return 0
 
