@@ -17,7 +17,8 @@ BUILD = os.path.join(HERE, "_build")
 
 ARCH = os.environ.get("FVS_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-# packed-FP32 instructions are kept off: all arithmetic here is FP64 anyway
+# packed-FP32 instructions are kept off (the solver is FP64; the network's
+# float32 epilogues do not need them)
 HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
@@ -36,16 +37,18 @@ def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     obj_host = os.path.join(BUILD, "mesh_prep.o")
     obj_dev = os.path.join(BUILD, "fvs_api.o")
+    obj_mlp = os.path.join(BUILD, "mlp.o")
     cmds = [
         ["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "mesh_prep.cpp"), "-o", obj_host],
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "fvs_api.hip"), "-o", obj_dev],
+        [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "mlp.hip"), "-o", obj_mlp],
     ]
     for c in cmds:
         if verbose:
             print(" ".join(c), flush=True)
         _run(c)
     tmp = LIB + ".tmp"
-    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", obj_dev, obj_host, "-o", tmp]
+    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", obj_dev, obj_mlp, obj_host, "-o", tmp]
     if verbose:
         print(" ".join(link), flush=True)
     _run(link)

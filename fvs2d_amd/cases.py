@@ -78,7 +78,8 @@ def draw_instances(n: int, seed: int = 0, resolutions=RESOLUTIONS):
 
 
 def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, freeze_after: int,
-              resolutions=RESOLUTIONS, sequential: int = 0, chunk: int = 250, graph: bool = True):
+              resolutions=RESOLUTIONS, sequential: int = 0, chunk: int = 250, graph: bool = True,
+              freeze_drop=None):
     """Batched steady solves of drawn instances; optionally the first
     ``sequential`` of them again one at a time for comparison."""
     from .solver import BatchSolver
@@ -91,7 +92,7 @@ def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, fr
     b.initialize_freestream()
     setup = time.perf_counter() - t0
     t0 = time.perf_counter()
-    res = b.steady_solve_batch(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after)
+    res = b.steady_solve_batch(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after, freeze_drop=freeze_drop)
     sec = time.perf_counter() - t0
     forces = b.wall_forces_batch()
     it = res["iterations"]
@@ -101,7 +102,9 @@ def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, fr
                solves_per_hour=3600.0 * ok.sum() / sec,
                iterations_p50=float(np.percentile(it[ok], 50)) if ok.any() else None,
                iterations_p99=float(np.percentile(it[ok], 99)) if ok.any() else None,
-               cl_cd=[[float(f[0]), float(f[1])] for f in forces[:8]])
+               per_instance=[dict(d, iterations=int(i), diverged=bool(v), frozen_at=int(z), cl=float(f[0]),
+                                  cd=float(f[1]))
+                             for d, i, v, z, f in zip(inst, it, res["diverged"], res["frozen_at"], forces)])
     if sequential:
         t0 = time.perf_counter()
         n_ok = 0
@@ -110,7 +113,7 @@ def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, fr
             s = Solver(m, sc)
             s.set_graph(graph)
             s.initialize_solution(1, 0.0)
-            r = s.steady_solve(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after)
+            r = s.steady_solve(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after, freeze_at=freeze_drop)
             n_ok += r["iterations"] is not None
             s.close()
         sq = time.perf_counter() - t0
@@ -139,6 +142,7 @@ def main(argv=None):
     bt.add_argument("--drop", type=float, default=1e-8)
     bt.add_argument("--cfl", type=float, default=1.25)
     bt.add_argument("--freeze-after", type=int, default=1000)
+    bt.add_argument("--freeze-drop", type=float, default=None)
     bt.add_argument("--resolutions", type=str, default="128x64,256x128,384x192,512x256")
     bt.add_argument("--sequential", type=int, default=0)
     bt.add_argument("--graph", type=int, default=1)
@@ -149,7 +153,7 @@ def main(argv=None):
     else:
         res = tuple(tuple(int(v) for v in r.split("x")) for r in a.resolutions.split(","))
         out = run_batch(a.instances, a.seed, a.max_steps, a.drop, a.cfl, a.freeze_after, res, a.sequential,
-                        graph=bool(a.graph))
+                        graph=bool(a.graph), freeze_drop=a.freeze_drop)
     print(json.dumps(out))
     return 0
 

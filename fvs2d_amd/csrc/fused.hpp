@@ -29,7 +29,9 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const doubl
                                                       double *__restrict__ phi, int lt, const int *__restrict__ nvrt,
                                                       const double2 *__restrict__ loff,
                                                       const double *__restrict__ vol,
-                                                      const int *__restrict__ blist) {
+                                                      const int *__restrict__ blist,
+                                                      const int *__restrict__ ioff,
+                                                      const int *__restrict__ frz, int ninst) {
   const int i = (blist ? blist[blockIdx.x] : xcd_block(nb)) * kBlock + threadIdx.x;
   if (i >= N) return;
   const int s = i / kSlice, l = i % kSlice;
@@ -98,7 +100,21 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const doubl
   }
   double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
   st8(grad, i, g8);
-  if (LIM) phi[i] = cell_limiter(lt, nvrt[i], N, i, loff, vol[i], w, mn, mx, gx, gy);
+  if (LIM) {
+    // batch mode with some instances frozen: find the cell's instance in the
+    // offset table (ioff[0] = 0 <= i < ioff[ninst] = N) and keep its value
+    bool keep = false;
+    if (ioff) {
+      int lo = 0, hi = ninst;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (ioff[mid] <= i) lo = mid;
+        else hi = mid;
+      }
+      keep = frz[lo] != 0;
+    }
+    if (!keep) phi[i] = cell_limiter(lt, nvrt[i], N, i, loff, vol[i], w, mn, mx, gx, gy);
+  }
 }
 
 // interior face flux with L = c1, R = c2 (shared by both flux kernels)

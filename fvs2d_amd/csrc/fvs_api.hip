@@ -632,6 +632,9 @@ struct fvs_ctx {
   bool xcd_grad = false, xcd_flux = false, xcd_upd = false, xcd_fused = true;
   // limiter values frozen at their last evaluation (steady convergence aid)
   bool limiter_frozen = false;
+  // batch mode: per-instance frozen flags (dfrz) while only some are frozen
+  int *dfrz = nullptr;
+  bool frz_some = false;
   // batched instances (fvs_create_batch): contiguous device ranges
   int ninst = 0;
   std::vector<int> inst_of_real;     // reference order
@@ -1090,7 +1093,8 @@ int launch_gradient(fvs_ctx &C, bool with_limiter) {
     const int nb = nblk(N);
 #define FVS_LSQC(L, PM)                                                                                          \
   hipLaunchKernelGGL((k_grad_lsqc<L, PM>), g, b, 0, C.stream, N, xcd_arg(nb, C.xcd_grad), C.W, C.cxy, C.ginv, C.soff, C.swid, C.sid,  \
-                     C.shifts, C.opt.lsq_p, C.grad, C.phi, lt, C.nvrt, C.loff, C.vol, C.nact ? C.blist : nullptr)
+                     C.shifts, C.opt.lsq_p, C.grad, C.phi, lt, C.nvrt, C.loff, C.vol, C.nact ? C.blist : nullptr,      \
+                     C.frz_some ? C.dinst_off : nullptr, C.dfrz, C.ninst)
     if (lim) {
       if (C.pmode == 0) FVS_LSQC(1, 0);
       else if (C.pmode == 1) FVS_LSQC(1, 1);
@@ -1216,6 +1220,11 @@ extern "C" {
 const char *fvs_last_error(void) { return g_err.c_str(); }
 
 }  // extern "C"
+
+// error text for the other translation units of the library (mlp.hip)
+namespace fvs {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace fvs
 
 static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
                        const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types,
@@ -1906,6 +1915,26 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
 int fvs_freeze_limiter(fvs_ctx *C, int frozen) {
   if (!C) return fail(FVS_EINVAL, "null context");
   C->limiter_frozen = frozen != 0;
+  C->frz_some = false;
+  return 0;
+}
+
+int fvs_batch_freeze_limiter(fvs_ctx *C, const int *frozen) {
+  if (!C || !frozen) return fail(FVS_EINVAL, "null argument");
+  if (C->ninst <= 0) return fail(FVS_ESTATE, "not a batch context (use fvs_create_batch)");
+  if (!C->lsqc) return fail(FVS_ESTATE, "per-instance freezing needs the centroid-form LSQ gradient");
+  int nf = 0;
+  for (int k = 0; k < C->ninst; ++k) nf += frozen[k] != 0;
+  C->limiter_frozen = nf == C->ninst;
+  C->frz_some = nf > 0 && nf < C->ninst;
+  if (C->frz_some) {
+    if (!C->dfrz)
+      if (int rc = C->alloc(C->dfrz, size_t(C->ninst))) return rc;
+    std::vector<int> f(frozen, frozen + C->ninst);
+    for (int &v : f) v = v != 0;
+    HIPC(hipMemcpyAsync(C->dfrz, f.data(), f.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+    HIPC(hipStreamSynchronize(C->stream));
+  }
   return 0;
 }
 

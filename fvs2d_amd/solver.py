@@ -447,45 +447,98 @@ class BatchSolver(Solver):
         check(self._L.fvs_batch_wall_forces(self._h, ptr(out)))
         return out
 
+    def freeze_limiter_batch(self, flags):
+        """Freeze the limiter of the instances with flags[k] != 0 only
+        (fvs_batch_freeze_limiter)."""
+        f = np.ascontiguousarray(flags, np.int32)
+        check(self._L.fvs_batch_freeze_limiter(self._h, ptr(f)))
+
+    def freestream_state(self, k: int) -> np.ndarray:
+        """Conservative free-stream state of instance k (src/initialize.f90:36-43)."""
+        g = self.scheme.gamma
+        M, a = self.mach_aoa[k]
+        u, v = M * np.cos(np.radians(a)), M * np.sin(np.radians(a))
+        return np.array([1.0, u, v, (1.0 / g) / (g - 1) + 0.5 * (u * u + v * v)])
+
+    def reference_norms(self) -> np.ndarray:
+        """Density-residual norm of one step from each instance's free stream,
+        i.e. the first-step norm of a cold start.  The state is restored."""
+        save = self.cvar
+        self.initialize_freestream()
+        r = self.batch_time_integration(0.0, 1)[0, :, 0].copy()
+        self.cvar = save
+        return r
+
     def steady_solve_batch(self, max_steps: int, drop: float = 1e-8, chunk: int = 200,
-                           freeze_after: Optional[int] = None):
-        """Pseudo-time stepping of all instances until each one's density
-        residual has dropped by ``drop`` from its first step; converged
-        instances are masked out at the next chunk boundary.  The limiter is
-        frozen for all instances after ``freeze_after`` steps."""
-        self.set_active(np.ones(self.ninst, np.int32))
-        iters = np.full(self.ninst, -1, np.int64)
-        r1 = None
+                           freeze_after: Optional[int] = None, freeze_drop: Optional[float] = None,
+                           r_ref=None, diverge: Optional[float] = None, on_diverge=None):
+        """Pseudo-time stepping of all instances, each until its density
+        residual norm is at most ``drop * r_ref[k]`` (``r_ref`` defaults to
+        each instance's first-step norm); converged instances are masked out
+        at the next chunk boundary.  Instance k's limiter is frozen once its
+        norm is at most ``freeze_drop * r_ref[k]`` or after ``freeze_after`` of
+        its own steps since its (re)start.  It has diverged when a norm is
+        non-finite or above ``diverge * r_ref[k]``; ``on_diverge(k)`` may then
+        reset its state and return True to restart it, its step count running
+        on, otherwise it is masked out.  Returns per-instance iterations (-1
+        when not converged), steps run, divergence and restart flags, and the
+        step at which each limiter was frozen (-1: never)."""
+        n = self.ninst
+        self.set_active(np.ones(n, np.int32))
+        iters = np.full(n, -1, np.int64)
+        r_ref = None if r_ref is None else np.asarray(r_ref, np.float64)
         done = 0
-        frozen = False
-        diverged = np.zeros(self.ninst, bool)
+        since = np.zeros(n, np.int64)
+        frozen = np.zeros(n, np.int32)
+        frozen_at = np.full(n, -1, np.int64)
+        diverged = np.zeros(n, bool)
+        restarted = np.zeros(n, bool)
+        self.freeze_limiter_batch(frozen)
         while done < max_steps and np.any(self.active):
-            n = min(chunk, max_steps - done)
-            if freeze_after is not None and not frozen:
-                if done >= freeze_after:
-                    self.freeze_limiter(True)
-                    frozen = True
-                else:
-                    n = min(n, freeze_after - done)
-            nr = self.batch_time_integration(0.0, n)[:, :, 0]
-            if r1 is None:
-                r1 = nr[0].copy()
+            m = min(chunk, max_steps - done)
+            if freeze_after is not None:
+                left = [freeze_after - since[k] for k in np.nonzero(self.active & (frozen == 0))[0]]
+                left = [v for v in left if v > 0]
+                if left:
+                    m = min(m, min(left))
+            nr = self.batch_time_integration(0.0, m)[:, :, 0]
+            if r_ref is None:
+                r_ref = nr[0].copy()
+            reset = []
             for k in np.nonzero(self.active)[0]:
-                if not np.all(np.isfinite(nr[:, k])):
-                    diverged[k] = True
-                    self.active[k] = 0
+                bad = ~np.isfinite(nr[:, k])
+                if diverge is not None:
+                    bad |= nr[:, k] > diverge * r_ref[k]
+                if np.any(bad):
+                    if on_diverge is not None and not restarted[k] and on_diverge(k):
+                        restarted[k] = True
+                        reset.append(k)
+                    else:
+                        diverged[k] = True
+                        self.active[k] = 0
                     continue
-                hit = np.nonzero(nr[:, k] <= drop * r1[k])[0]
+                hit = np.nonzero(nr[:, k] <= drop * r_ref[k])[0]
                 if hit.size:
                     iters[k] = done + int(hit[0]) + 1
                     self.active[k] = 0
-            done += n
+                    continue
+                since[k] += m
+                if not frozen[k] and ((freeze_drop is not None and nr[-1, k] <= freeze_drop * r_ref[k])
+                                      or (freeze_after is not None and since[k] >= freeze_after)):
+                    frozen[k] = 1
+                    frozen_at[k] = done + m
+            for k in reset:
+                since[k] = 0
+                frozen[k] = 0
+                frozen_at[k] = -1
+            done += m
+            self.freeze_limiter_batch(frozen)
             if np.any(self.active):
                 self.set_active(self.active)
-        if frozen:
-            self.freeze_limiter(False)
-        self.set_active(np.ones(self.ninst, np.int32))
-        return dict(iterations=iters, steps_run=done, diverged=diverged)
+        self.freeze_limiter(False)
+        self.set_active(np.ones(n, np.int32))
+        return dict(iterations=iters, steps_run=done, diverged=diverged, restarted=restarted, frozen_at=frozen_at,
+                    r_ref=r_ref)
 
 
 def compute_flux_invscid(pL, pR, nx, ny, gamma=1.4):

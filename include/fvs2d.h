@@ -181,6 +181,10 @@ int fvs_kernel_times(fvs_ctx *ctx, double t1, int nsteps, double *ms6, double *b
  * the limiter is not re-evaluated and every residual uses the limiter values
  * of the last evaluation. */
 int fvs_freeze_limiter(fvs_ctx *ctx, int frozen);
+/* Per-instance form of fvs_freeze_limiter for batch contexts, no reference
+ * counterpart: frozen[k] != 0 keeps instance k's limiter values as last
+ * evaluated while the other instances re-evaluate theirs every stage. */
+int fvs_batch_freeze_limiter(fvs_ctx *ctx, const int *frozen);
 /* Launch autotuner, no reference counterpart: times nsteps steps per trial
  * and keeps, per kernel class [gradient, face flux, gather+update, fused
  * flux+update], the faster of plain and XCD-aware workgroup numbering.  The

@@ -13,11 +13,12 @@ from fvs2d_amd.solver import Scheme, Solver, read_input, read_vortex
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HDR = os.path.join(ROOT, "include", "fvs2d.h")
+HDR_MLP = os.path.join(ROOT, "include", "fvs2d_mlp.h")
 GOLD = os.path.join(ROOT, "tests", "golden")
 
 
-def declared():
-    src = open(HDR).read()
+def declared(path=HDR):
+    src = open(path).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(fvs_\w+)\s*\(", src)))
 
@@ -29,6 +30,24 @@ def test_header_symbols_exported():
     for n in names:
         assert hasattr(L, n), f"{n} declared in include/fvs2d.h but not exported"
     assert sorted(_lib.EXPORTS) == names
+    from fvs2d_amd import mlp
+    names = declared(HDR_MLP)
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in include/fvs2d_mlp.h but not exported"
+    assert sorted(mlp.MLP_EXPORTS) == names
+
+
+def test_mlp_bad_arguments():
+    from fvs2d_amd import mlp
+    L = mlp._load()
+    h = C.c_void_p()
+    for depth, batch, word in ((0, 64, b"depth"), (9, 64, b"depth"), (2, 100, b"batch"), (2, 0, b"batch")):
+        o = mlp.MLPOptions(depth, batch, 1e-3, 0.9, 0.999, 1e-8, 0)
+        assert L.fvs_mlp_create(C.byref(h), C.byref(o)) == -1
+        assert word in L.fvs_last_error()
+    o = mlp.MLPOptions(2, 64, -1.0, 0.9, 0.999, 1e-8, 0)
+    assert L.fvs_mlp_create(C.byref(h), C.byref(o)) == -1 and b"Adam" in L.fvs_last_error()
+    assert L.fvs_mlp_train(None, 1, None) == -1
 
 
 def test_bad_arguments_return_codes():
@@ -49,6 +68,9 @@ def _gpu_present():
 def test_no_device_fails_loudly():
     with pytest.raises(_lib.FvsError, match="no HIP device"):
         Solver(meshgen.jittered_square(4), Scheme())
+    from fvs2d_amd.mlp import WarmStartNet
+    with pytest.raises(_lib.FvsError, match="no HIP device"):
+        WarmStartNet(depth=1, batch=64)
 
 
 def test_input_decks_of_the_reference():

@@ -18,10 +18,12 @@ from tests.common import rel_err, vortex_scheme
 pytestmark = pytest.mark.gpu
 
 INST = [((48, 24), (0.75, 1.0)), ((64, 32), (0.8, 1.25)), ((32, 16), (0.6, -2.0))]
+# instance sizes 1600, 2704, 1296 cells: boundaries inside 256-cell blocks
+INST_RAGGED = [((40, 20), (0.75, 1.0)), ((52, 26), (0.8, 1.25)), ((36, 18), (0.6, -2.0))]
 
 
-def single(k, steps, freeze_after=None):
-    (n, l), (M, a) = INST[k]
+def single(k, steps, freeze_after=None, inst=INST):
+    (n, l), (M, a) = inst[k]
     m, sch = cases.naca(n, l, mach=M, aoa=a, cfl=1.0)
     s = Solver(m, sch)
     s.initialize_solution(1, 0.0)
@@ -34,10 +36,10 @@ def single(k, steps, freeze_after=None):
     return s
 
 
-def batch():
-    meshes = [cases.naca(n, l, cfl=1.0)[0] for (n, l), _ in INST]
+def batch(inst=INST):
+    meshes = [cases.naca(n, l, cfl=1.0)[0] for (n, l), _ in inst]
     _, sch = cases.naca(16, 8, cfl=1.0)
-    b = BatchSolver(meshes, [ma for _, ma in INST], sch)
+    b = BatchSolver(meshes, [ma for _, ma in inst], sch)
     b.initialize_freestream()
     return b
 
@@ -105,3 +107,50 @@ def test_steady_solve_batch_converges():
         s.initialize_solution(1, 0.0)
         r = s.steady_solve(20000, drop=1e-6, chunk=100, freeze_after=300)
         assert r["iterations"] == res["iterations"][k], (k, r["iterations"], res["iterations"])
+
+
+def test_per_instance_limiter_freeze():
+    """Freezing the limiter of one instance (fvs_batch_freeze_limiter) leaves
+    the others re-evaluating theirs: instance 1 frozen after 30 steps, 0 and 2
+    never, each bit-identical to its own single solve.  Instance 1's cells
+    share launch blocks with instances 0 and 2 (the block boundaries do not
+    fall on instance boundaries), so the per-cell instance lookup is used."""
+    b = batch(INST_RAGGED)
+    o = b.cell_offsets
+    assert o[1] % 256 != 0 and o[2] % 256 != 0
+    b.batch_time_integration(0.0, 30)
+    b.freeze_limiter_batch([0, 1, 0])
+    b.batch_time_integration(0.0, 25)
+    b.freeze_limiter(False)
+    for k in range(len(INST_RAGGED)):
+        s = single(k, 55, freeze_after=30 if k == 1 else None, inst=INST_RAGGED)
+        assert np.array_equal(b.instance_state(k), s.cvar), k
+
+
+def test_steady_batch_restart_on_divergence():
+    """A diverging instance (forced by a threshold below its own residual)
+    is reset through on_diverge and restarted once, then masked out; the
+    others converge as before."""
+    b = batch()
+    b.initialize_freestream()
+    r_ref = b.reference_norms()
+    calls = []
+
+    def reset(k):
+        calls.append(k)
+        cv = b.cvar
+        cv[b.cell_offsets[k]:b.cell_offsets[k + 1]] = b.freestream_state(k)
+        b.cvar = cv
+        return True
+
+    res = b.steady_solve_batch(3000, drop=1e-3, chunk=50, freeze_after=300, r_ref=r_ref, on_diverge=reset)
+    assert not np.any(res["diverged"]) and np.all(res["iterations"] > 0) and calls == []
+    # divergence at 1e3 reference norms; instance 2's reference is scaled by
+    # 1e-6, so its first-step norm trips the test at once, it restarts from
+    # the free stream and trips again
+    b.initialize_freestream()
+    res2 = b.steady_solve_batch(3000, drop=1e-3, chunk=50, freeze_after=300, r_ref=r_ref * np.array([1, 1, 1e-6]),
+                                diverge=1e3, on_diverge=reset)
+    assert calls == [2]
+    assert res2["restarted"][2] and res2["diverged"][2] and res2["iterations"][2] == -1
+    assert np.array_equal(res2["iterations"][:2], res["iterations"][:2])
