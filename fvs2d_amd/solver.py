@@ -303,7 +303,8 @@ class Solver:
         check(self._L.fvs_set_graph(self._h, int(on)))
 
     def steady_solve(self, max_steps: int, drop: float = 1e-8, chunk: int = 200, t1: float = 0.0,
-                     freeze_at: Optional[float] = None, freeze_after: Optional[int] = None):
+                     freeze_at: Optional[float] = None, freeze_after: Optional[int] = None,
+                     freeze_stall: Optional[Tuple[int, float]] = None):
         """Pseudo-time stepping until the density residual of
         src/runge_kutta.f90:396-398 has dropped by ``drop`` relative to the first
         step (the reference runs a fixed step count; the stopping rule is this
@@ -311,15 +312,19 @@ class Solver:
         run up to chunk-1 steps past the crossing.  With ``freeze_at`` the
         limiter is frozen once the drop passes that level, with
         ``freeze_after`` after that many steps (limiter chatter otherwise
-        stalls convergence).  Returns a dict with the iteration of the
-        crossing (or None), the norm history, the iteration at which the limiter
-        was frozen, and whether a non-finite norm (divergence) stopped the loop."""
+        stalls convergence); with ``freeze_stall = (window, factor)`` once the
+        norm at a chunk end is above ``factor`` times the norm ``window`` steps
+        earlier (the same rule as steady_solve_batch).  Returns a dict with the
+        iteration of the crossing (or None), the norm history, the iteration at
+        which the limiter was frozen, and whether a non-finite norm
+        (divergence) stopped the loop."""
         hist = []
         done = 0
         conv_at = None
         frozen_at = None
         diverged = False
         r1 = None
+        ends = []
         while done < max_steps:
             n = min(chunk, max_steps - done)
             if freeze_after is not None and frozen_at is None:
@@ -336,15 +341,17 @@ class Solver:
                 diverged = True
                 done += n
                 break
-            if freeze_at is not None and frozen_at is None and norms[-1, 0] <= freeze_at * r1:
-                self.freeze_limiter(True)
-                frozen_at = done + n
             hit = np.nonzero(norms[:, 0] <= drop * r1)[0]
             if hit.size:
                 conv_at = done + int(hit[0]) + 1
                 done += n
                 break
             done += n
+            ends.append((done, float(norms[-1, 0])))
+            if frozen_at is None and ((freeze_at is not None and norms[-1, 0] <= freeze_at * r1)
+                                      or (freeze_stall is not None and _stalled(ends, freeze_stall))):
+                self.freeze_limiter(True)
+                frozen_at = done
         if frozen_at is not None:
             self.freeze_limiter(False)
         h = np.concatenate(hist) if hist else np.zeros((0, 4))
@@ -471,13 +478,16 @@ class BatchSolver(Solver):
 
     def steady_solve_batch(self, max_steps: int, drop: float = 1e-8, chunk: int = 200,
                            freeze_after: Optional[int] = None, freeze_drop: Optional[float] = None,
-                           r_ref=None, diverge: Optional[float] = None, on_diverge=None):
+                           r_ref=None, diverge: Optional[float] = None, on_diverge=None,
+                           freeze_stall: Optional[Tuple[int, float]] = None):
         """Pseudo-time stepping of all instances, each until its density
         residual norm is at most ``drop * r_ref[k]`` (``r_ref`` defaults to
         each instance's first-step norm); converged instances are masked out
         at the next chunk boundary.  Instance k's limiter is frozen once its
         norm is at most ``freeze_drop * r_ref[k]`` or after ``freeze_after`` of
-        its own steps since its (re)start.  It has diverged when a norm is
+        its own steps since its (re)start, or, with ``freeze_stall = (window,
+        factor)``, once its norm at a chunk end is above ``factor`` times its
+        norm ``window`` steps earlier (it stalls).  It has diverged when a norm is
         non-finite or above ``diverge * r_ref[k]``; ``on_diverge(k)`` may then
         reset its state and return True to restart it, its step count running
         on, otherwise it is masked out.  Returns per-instance iterations (-1
@@ -493,6 +503,7 @@ class BatchSolver(Solver):
         frozen_at = np.full(n, -1, np.int64)
         diverged = np.zeros(n, bool)
         restarted = np.zeros(n, bool)
+        ends = [[] for _ in range(n)]
         self.freeze_limiter_batch(frozen)
         while done < max_steps and np.any(self.active):
             m = min(chunk, max_steps - done)
@@ -523,14 +534,17 @@ class BatchSolver(Solver):
                     self.active[k] = 0
                     continue
                 since[k] += m
+                ends[k].append((int(since[k]), float(nr[-1, k])))
                 if not frozen[k] and ((freeze_drop is not None and nr[-1, k] <= freeze_drop * r_ref[k])
-                                      or (freeze_after is not None and since[k] >= freeze_after)):
+                                      or (freeze_after is not None and since[k] >= freeze_after)
+                                      or (freeze_stall is not None and _stalled(ends[k], freeze_stall))):
                     frozen[k] = 1
                     frozen_at[k] = done + m
             for k in reset:
                 since[k] = 0
                 frozen[k] = 0
                 frozen_at[k] = -1
+                ends[k] = []
             done += m
             self.freeze_limiter_batch(frozen)
             if np.any(self.active):
@@ -539,6 +553,17 @@ class BatchSolver(Solver):
         self.set_active(np.ones(n, np.int32))
         return dict(iterations=iters, steps_run=done, diverged=diverged, restarted=restarted, frozen_at=frozen_at,
                     r_ref=r_ref)
+
+
+def _stalled(ends, rule) -> bool:
+    """ends: (step, norm) at chunk ends since the (re)start.  True when the
+    latest norm is above factor times the norm at least window steps before."""
+    window, factor = rule
+    step, r = ends[-1]
+    for s0, r0 in reversed(ends[:-1]):
+        if step - s0 >= window:
+            return r > factor * r0
+    return False
 
 
 def compute_flux_invscid(pL, pR, nx, ny, gamma=1.4):
