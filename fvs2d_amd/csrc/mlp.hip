@@ -71,9 +71,19 @@ __host__ __device__ inline u16 f2bf(float x) {
 }
 __host__ __device__ inline float bf2f(u16 h) { return __builtin_bit_cast(float, unsigned(h) << 16); }
 
-__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun 7.1.26 (Hastings' rational approximation,
+// |error| <= 1.5e-7): branch-free, so the 64 evaluations per thread of a GEMM
+// epilogue do not split the wavefront; far below bfloat16 resolution (3.9e-3)
+__device__ inline float erf_as(float x) {
+  const float ax = fabsf(x);
+  const float t = 1.0f / (1.0f + 0.3275911f * ax);
+  const float poly = ((((1.061405429f * t - 1.453152027f) * t + 1.421413741f) * t - 0.284496736f) * t +
+                      0.254829592f) * t;
+  return copysignf(1.0f - poly * __expf(-ax * ax), x);
+}
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
 __device__ inline float gelu_grad(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * expf(-0.5f * z * z) * 0.39894228040143268f;
+  return 0.5f * (1.0f + erf_as(z * 0.70710678118654752f)) + z * __expf(-0.5f * z * z) * 0.39894228040143268f;
 }
 
 // splitmix64 finaliser (Steele, Lea, Flood 2014): row sampling and weight init
@@ -85,7 +95,10 @@ __host__ __device__ inline unsigned long long mix64(unsigned long long z) {
 }
 
 // ---------------------------------------------------------------- features
-// 16 threads per row, thread q writes input columns 8q..8q+7 (16 bytes).
+// 16 threads per row, thread q writes input columns 8q..8q+7 (16 bytes):
+// q = 0..3 the radial harmonics 4q+1..4q+4, q = 4..7 the angular harmonics
+// 4(q-4)+1..+4 (one sincos of the base angle and one of the first harmonic,
+// then the angle-addition recurrence), q = 8 the four scalar channels.
 // sample != 0: row m takes record hash(seed, step, m) mod ndata; otherwise
 // record m, and rows m >= nvalid are zero.
 __global__ __launch_bounds__(256) void k_features(int M, long nvalid, const float *__restrict__ raw,
@@ -108,33 +121,33 @@ __global__ __launch_bounds__(256) void k_features(int M, long nvalid, const floa
     src = m;
   }
   float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (valid) {
+  if (valid && q <= 8) {
     const float *r = raw + src * kRaw;
-    const double x = r[0], y = r[1], d = r[2], mach = r[3], aoa = r[4], th = r[5];
-    const double dx = x - 0.5;
-    const double rr = sqrt(dx * dx + y * y);
-    const double ang = atan2(y, dx);
-    const double s = log1p(4.0 * rr) / log1p(200.0);
-    const double kPi = 3.14159265358979323846;
-    for (int c = 0; c < 8; ++c) {
-      const int j = 8 * q + c;
-      double v = 0.0;
-      if (j < 32) {
-        const double a = double(j / 2 + 1) * kPi * s;
-        v = (j & 1) ? cos(a) : sin(a);
-      } else if (j < 64) {
-        const double a = double((j - 32) / 2 + 1) * ang;
-        v = (j & 1) ? cos(a) : sin(a);
-      } else if (j == 64) {
-        v = log1p(100.0 * d) / log1p(5000.0);
-      } else if (j == 65) {
-        v = (mach - 0.675) / 0.175;
-      } else if (j == 66) {
-        v = (aoa - 1.0) / 3.0;
-      } else if (j == 67) {
-        v = (th - 0.125) / 0.025;
+    if (q < 8) {
+      const double dx = double(r[0]) - 0.5, y = r[1];
+      double a;
+      if (q < 4) {
+        const double kPi = 3.14159265358979323846;
+        a = kPi * (log1p(4.0 * sqrt(dx * dx + y * y)) / log1p(200.0));
+      } else {
+        a = atan2(y, dx);
       }
-      f[c] = float(v);
+      const int k0 = 4 * (q & 3) + 1;
+      double s1, c1, sk, ck;
+      sincos(a, &s1, &c1);
+      sincos(double(k0) * a, &sk, &ck);
+      for (int h = 0; h < 4; ++h) {
+        f[2 * h] = float(sk);
+        f[2 * h + 1] = float(ck);
+        const double sn = sk * c1 + ck * s1, cn = ck * c1 - sk * s1;
+        sk = sn;
+        ck = cn;
+      }
+    } else {
+      f[0] = float(log1p(100.0 * double(r[2])) / log1p(5000.0));
+      f[1] = float((double(r[3]) - 0.675) / 0.175);
+      f[2] = float((double(r[4]) - 1.0) / 3.0);
+      f[3] = float((double(r[5]) - 0.125) / 0.025);
     }
   }
   unsigned pk[4];
@@ -148,9 +161,13 @@ __global__ __launch_bounds__(256) void k_features(int M, long nvalid, const floa
 }
 
 // ------------------------------------------------------------- GEMM core
-// acc[i][j] += A_lds[16i.., :] * B_lds[64w + 16j.., :]^T over one kBK slab.
-// Fragment layout of v_mfma_f32_16x16x32_bf16 (lane l): A row l%16, k
-// 8(l/16)..+7; B column l%16, same k; D rows 4(l/16)..+3, column l%16.
+// One kBK slab of C[m][n] += sum_k A[m][k] B[n][k] with A_lds rows 16i..
+// (4 blocks) and B_lds rows 64w + 16j.. (4 blocks).  Fragment layout of
+// v_mfma_f32_16x16x32_bf16 for lane l: first operand row l%16, k
+// 8(l/16)..+7; second operand column l%16, same k; result rows
+// 4(l/16)..+3, column l%16.  The B fragment goes first, so the result is
+// C transposed: lane l holds C[16i + l%16][64w + 16j + 4(l/16) + e], four
+// consecutive columns, stored as one 8- or 16-byte vector.
 __device__ __forceinline__ void mma_slab(const u16 *__restrict__ As, const u16 *__restrict__ Bs, int lane, int w,
                                          f32x4 (&acc)[4][4]) {
   bf16x8 a[4], b[4];
@@ -162,12 +179,18 @@ __device__ __forceinline__ void mma_slab(const u16 *__restrict__ As, const u16 *
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
+}
+
+__device__ __forceinline__ unsigned pack2(float a, float b) {
+  return unsigned(f2bf(a)) | (unsigned(f2bf(b)) << 16);
 }
 
 // C[M][256] = A[M][K] * B[256][K]^T (both row-major, K a multiple of 32).
 // EPI 0 (forward): z = C + bias; Z = bf16(z), H = bf16(GELU(z)).
 // EPI 1 (backward data): dZ = bf16(C * GELU'(Zp)).
+// The next slab's global data is loaded into registers while the current
+// one is multiplied.
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm_nt(int K, const u16 *__restrict__ A, const u16 *__restrict__ B,
                                                  const float *__restrict__ bias, const u16 *__restrict__ Zp,
@@ -181,47 +204,76 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int K, const u16 *__restrict__ 
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int arow = tid >> 2, ac = (tid & 3) * 8;
+  const u16 *Ap = A + (r0 + arow) * K + ac;
+  uint4 ra = *reinterpret_cast<const uint4 *>(Ap), rb[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int ch = tid + 256 * q;
+    rb[q] = *reinterpret_cast<const uint4 *>(B + long(ch >> 2) * K + (ch & 3) * 8);
+  }
   for (int k0 = 0; k0 < K; k0 += kBK) {
-    {
-      const int row = tid >> 2, c = (tid & 3) * 8;
-      *reinterpret_cast<uint4 *>(As + row * kLds + c) =
-          *reinterpret_cast<const uint4 *>(A + (r0 + row) * K + k0 + c);
-    }
+    *reinterpret_cast<uint4 *>(As + arow * kLds + ac) = ra;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int ch = tid + 256 * q, row = ch >> 2, c = (ch & 3) * 8;
-      *reinterpret_cast<uint4 *>(Bs + row * kLds + c) = *reinterpret_cast<const uint4 *>(B + long(row) * K + k0 + c);
+      const int ch = tid + 256 * q;
+      *reinterpret_cast<uint4 *>(Bs + (ch >> 2) * kLds + (ch & 3) * 8) = rb[q];
     }
     __syncthreads();
+    if (k0 + kBK < K) {
+      ra = *reinterpret_cast<const uint4 *>(Ap + k0 + kBK);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int ch = tid + 256 * q;
+        rb[q] = *reinterpret_cast<const uint4 *>(B + long(ch >> 2) * K + k0 + kBK + (ch & 3) * 8);
+      }
+    }
     mma_slab(As, Bs, lane, w, acc);
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    const long row = r0 + 16 * i + (lane & 15);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int col = 64 * w + 16 * j + (lane & 15);
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const long row = r0 + 16 * i + 4 * (lane >> 4) + e;
-        const long idx = row * kW + col;
-        if (EPI == 0) {
-          const float z = acc[i][j][e] + bias[col];
-          O1[idx] = f2bf(z);
-          O2[idx] = f2bf(gelu(z));
-        } else {
-          O1[idx] = f2bf(acc[i][j][e] * gelu_grad(bf2f(Zp[idx])));
-        }
+      const int n0 = 64 * w + 16 * j + 4 * (lane >> 4);
+      const long idx = row * kW + n0;
+      if (EPI == 0) {
+        const float4 bv = *reinterpret_cast<const float4 *>(bias + n0);
+        const float z0 = acc[i][j][0] + bv.x, z1 = acc[i][j][1] + bv.y, z2 = acc[i][j][2] + bv.z,
+                    z3 = acc[i][j][3] + bv.w;
+        *reinterpret_cast<uint2 *>(O1 + idx) = make_uint2(pack2(z0, z1), pack2(z2, z3));
+        *reinterpret_cast<uint2 *>(O2 + idx) = make_uint2(pack2(gelu(z0), gelu(z1)), pack2(gelu(z2), gelu(z3)));
+      } else {
+        const uint2 zz = *reinterpret_cast<const uint2 *>(Zp + idx);
+        const float z0 = bf2f(u16(zz.x & 0xffffu)), z1 = bf2f(u16(zz.x >> 16)), z2 = bf2f(u16(zz.y & 0xffffu)),
+                    z3 = bf2f(u16(zz.y >> 16));
+        *reinterpret_cast<uint2 *>(O1 + idx) =
+            make_uint2(pack2(acc[i][j][0] * gelu_grad(z0), acc[i][j][1] * gelu_grad(z1)),
+                       pack2(acc[i][j][2] * gelu_grad(z2), acc[i][j][3] * gelu_grad(z3)));
       }
     }
+  }
+}
+
+// move rows (2mp, 2mp+1) x 8 columns, held as v0/v1, into a transposed LDS
+// tile T[col][m] as 32-bit pairs
+__device__ __forceinline__ void put_pairs(u16 *__restrict__ T, int col0, int mp, uint4 v0, uint4 v1) {
+  const unsigned a0[4] = {v0.x, v0.y, v0.z, v0.w}, a1[4] = {v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const unsigned lo = (a0[c >> 1] >> (16 * (c & 1))) & 0xffffu, hi = (a1[c >> 1] >> (16 * (c & 1))) & 0xffffu;
+    *reinterpret_cast<unsigned *>(T + (col0 + c) * kLds + 2 * mp) = lo | (hi << 16);
+  }
 }
 
 // Weight-gradient partials: for k-tile kt (64 rows of dW) and row slice s,
 // dW[k][n] = sum over m in slice of A[m][k] * D[m][n], A[M][K], D[M][256].
 // Both operands are transposed into LDS (m contiguous); a thread moves two
-// rows' worth of 8 columns and writes 32-bit pairs.  Workgroups of k-tile 0
-// also sum D's columns (the bias gradient).  Output rows k >= Kreal (input
-// padding) are not written.
+// rows' worth of 8 columns and writes 32-bit pairs; the next slab's global
+// data is loaded into registers while the current one is multiplied.
+// Workgroups of k-tile 0 also sum D's columns (the bias gradient).  Output
+// rows k >= Kreal (input padding) are not written.
 __global__ __launch_bounds__(256) void k_gemm_tn(int K, int Kreal, int chunk, const u16 *__restrict__ A,
                                                  const u16 *__restrict__ D, float *__restrict__ G, long P,
                                                  long offW, long offb) {
@@ -238,33 +290,36 @@ __global__ __launch_bounds__(256) void k_gemm_tn(int K, int Kreal, int chunk, co
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   float db = 0.f;
+  const int amp = tid & 15, akg = tid >> 4;  // A: threads < 128, rows 2amp, 2amp+1, columns 8akg..
+  const u16 *Ap = A + (m0 + 2 * amp) * K + kb + 8 * akg;
+  const int bmp0 = tid & 15, bng0 = tid >> 4, bng1 = (tid + 256) >> 4;  // D: two chunks per thread
+  const u16 *Dp0 = D + (m0 + 2 * bmp0) * kW + 8 * bng0;
+  const u16 *Dp1 = D + (m0 + 2 * bmp0) * kW + 8 * bng1;
+  uint4 ra0{}, ra1{}, rb00, rb01, rb10, rb11;
+  if (tid < 128) {
+    ra0 = *reinterpret_cast<const uint4 *>(Ap);
+    ra1 = *reinterpret_cast<const uint4 *>(Ap + K);
+  }
+  rb00 = *reinterpret_cast<const uint4 *>(Dp0);
+  rb01 = *reinterpret_cast<const uint4 *>(Dp0 + kW);
+  rb10 = *reinterpret_cast<const uint4 *>(Dp1);
+  rb11 = *reinterpret_cast<const uint4 *>(Dp1 + kW);
   for (int mm = 0; mm < chunk; mm += kBK) {
-    if (tid < 128) {
-      const int mp = tid & 15, kg = tid >> 4;  // rows 2mp, 2mp+1; columns 8kg..8kg+7
-      const long src = (m0 + mm + 2 * mp) * K + kb + 8 * kg;
-      const uint4 v0 = *reinterpret_cast<const uint4 *>(A + src);
-      const uint4 v1 = *reinterpret_cast<const uint4 *>(A + src + K);
-      const unsigned a0[4] = {v0.x, v0.y, v0.z, v0.w}, a1[4] = {v1.x, v1.y, v1.z, v1.w};
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const unsigned lo = (a0[c >> 1] >> (16 * (c & 1))) & 0xffffu, hi = (a1[c >> 1] >> (16 * (c & 1))) & 0xffffu;
-        *reinterpret_cast<unsigned *>(As + (8 * kg + c) * kLds + 2 * mp) = lo | (hi << 16);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int ch = tid + 256 * q, mp = ch & 15, ng = ch >> 4;  // ng 0..31
-      const long src = (m0 + mm + 2 * mp) * kW + 8 * ng;
-      const uint4 v0 = *reinterpret_cast<const uint4 *>(D + src);
-      const uint4 v1 = *reinterpret_cast<const uint4 *>(D + src + kW);
-      const unsigned a0[4] = {v0.x, v0.y, v0.z, v0.w}, a1[4] = {v1.x, v1.y, v1.z, v1.w};
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const unsigned lo = (a0[c >> 1] >> (16 * (c & 1))) & 0xffffu, hi = (a1[c >> 1] >> (16 * (c & 1))) & 0xffffu;
-        *reinterpret_cast<unsigned *>(Bs + (8 * ng + c) * kLds + 2 * mp) = lo | (hi << 16);
-      }
-    }
+    if (tid < 128) put_pairs(As, 8 * akg, amp, ra0, ra1);
+    put_pairs(Bs, 8 * bng0, bmp0, rb00, rb01);
+    put_pairs(Bs, 8 * bng1, bmp0, rb10, rb11);
     __syncthreads();
+    if (mm + kBK < chunk) {
+      const long o = long(mm + kBK);
+      if (tid < 128) {
+        ra0 = *reinterpret_cast<const uint4 *>(Ap + o * K);
+        ra1 = *reinterpret_cast<const uint4 *>(Ap + (o + 1) * K);
+      }
+      rb00 = *reinterpret_cast<const uint4 *>(Dp0 + o * kW);
+      rb01 = *reinterpret_cast<const uint4 *>(Dp0 + (o + 1) * kW);
+      rb10 = *reinterpret_cast<const uint4 *>(Dp1 + o * kW);
+      rb11 = *reinterpret_cast<const uint4 *>(Dp1 + (o + 1) * kW);
+    }
     if (kt == 0) {
       const u16 *row = Bs + tid * kLds;
       for (int c = 0; c < kBK; ++c) db += bf2f(row[c]);
@@ -274,30 +329,34 @@ __global__ __launch_bounds__(256) void k_gemm_tn(int K, int Kreal, int chunk, co
   }
   float *Gs = G + long(s) * P;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    const int k = kb + 16 * i + (lane & 15);
+    if (k >= Kreal) continue;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int n = 64 * w + 16 * j + (lane & 15);
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = kb + 16 * i + 4 * (lane >> 4) + e;
-        if (k < Kreal) Gs[offW + long(k) * kW + n] = acc[i][j][e];
-      }
+      const int n0 = 64 * w + 16 * j + 4 * (lane >> 4);
+      *reinterpret_cast<float4 *>(Gs + offW + long(k) * kW + n0) =
+          make_float4(acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
     }
+  }
   if (kt == 0) Gs[offb + tid] = db;
 }
 
 // ------------------------------------------------------------ output layer
-// Block s handles rows [s*chunk, (s+1)*chunk), 8 rows at a time, 32 lanes
-// per row, 8 hidden columns per lane.  y = H Wh + bh in float32.  With dZ
-// set: dy = 2 (y - t) / (nvalid * 4) for rows < nvalid (0 otherwise), the
-// last hidden layer's dZ = bf16((dy Wh^T) * GELU'(Z)), and this slice's
-// partial gradients of Wh, bh and the loss sum.
-__global__ __launch_bounds__(256) void k_head(int chunk, long nvalid, float inv_no, const u16 *__restrict__ H,
+constexpr int kHeadRows = 128;                 // rows per workgroup
+constexpr int kHeadPart = kW * kOut + kOut;    // output-layer parameters
+
+// Workgroup b handles rows [128b, min(128b + 128, M)), 8 rows at a time (M is
+// a multiple of 64, so every lane of a row pair runs the same trips), 32 lanes per
+// row, 8 hidden columns per lane.  y = H Wh + bh in float32.  With dZ set:
+// dy = 2 (y - t) / (nvalid * 4) for rows < nvalid (0 otherwise), the last
+// hidden layer's dZ = bf16((dy Wh^T) * GELU'(Z)), and this workgroup's
+// partial gradients of Wh and bh (Gh[b][...]) and loss sum (lossp[b]).
+__global__ __launch_bounds__(256) void k_head(long M, long nvalid, float inv_no, const u16 *__restrict__ H,
                                               const u16 *__restrict__ Z, const float *__restrict__ Wh,
                                               const float *__restrict__ bh, const float *__restrict__ T,
-                                              float *__restrict__ Y, u16 *__restrict__ dZ, float *__restrict__ G,
-                                              long P, long offW, long offb, float *__restrict__ lossp) {
+                                              float *__restrict__ Y, u16 *__restrict__ dZ, float *__restrict__ Gh,
+                                              float *__restrict__ lossp) {
   __shared__ float Ws[kW * kOut];
   __shared__ float red[4][32][8 * kOut + kOut + 1];
   const int tid = threadIdx.x, q = tid & 31, rs = tid >> 5, lane = tid & 63, wv = tid >> 6;
@@ -307,8 +366,8 @@ __global__ __launch_bounds__(256) void k_head(int chunk, long nvalid, float inv_
   for (int c = 0; c < 8; ++c)
     for (int o = 0; o < kOut; ++o) gw[c][o] = 0.f;
   for (int o = 0; o < kOut; ++o) gb[o] = 0.f;
-  const long mbeg = long(blockIdx.x) * chunk;
-  for (long m = mbeg + rs; m < mbeg + chunk; m += 8) {
+  const long mbeg = long(blockIdx.x) * kHeadRows, mend = mbeg + kHeadRows < M ? mbeg + kHeadRows : M;
+  for (long m = mbeg + rs; m < mend; m += 8) {
     const uint4 hv = *reinterpret_cast<const uint4 *>(H + m * kW + 8 * q);
     const unsigned hw[4] = {hv.x, hv.y, hv.z, hv.w};
     float h[8];
@@ -359,18 +418,37 @@ __global__ __launch_bounds__(256) void k_head(int chunk, long nvalid, float inv_
     red[wv][q][8 * kOut + kOut] = ls;
   }
   __syncthreads();
-  float *Gs = G + long(blockIdx.x) * P;
+  float *Gb = Gh + long(blockIdx.x) * kHeadPart;
   for (int e = tid; e < kW * kOut; e += 256) {
     const int k = e / kOut, o = e % kOut, qq = k / 8, c = k % 8;
-    Gs[offW + e] = red[0][qq][c * kOut + o] + red[1][qq][c * kOut + o] + red[2][qq][c * kOut + o] +
-                   red[3][qq][c * kOut + o];
+    Gb[e] = red[0][qq][c * kOut + o] + red[1][qq][c * kOut + o] + red[2][qq][c * kOut + o] +
+            red[3][qq][c * kOut + o];
   }
   if (tid < kOut)
-    Gs[offb + tid] = red[0][0][8 * kOut + tid] + red[1][0][8 * kOut + tid] + red[2][0][8 * kOut + tid] +
-                     red[3][0][8 * kOut + tid];
+    Gb[kW * kOut + tid] = red[0][0][8 * kOut + tid] + red[1][0][8 * kOut + tid] + red[2][0][8 * kOut + tid] +
+                          red[3][0][8 * kOut + tid];
   if (tid == 0)
     lossp[blockIdx.x] = red[0][0][8 * kOut + kOut] + red[1][0][8 * kOut + kOut] + red[2][0][8 * kOut + kOut] +
                         red[3][0][8 * kOut + kOut];
+}
+
+// Output-layer gradient: column c of the nhb partial rows, summed by 16
+// groups of rows (group g takes rows g, g+16, ...) and then the 16 group
+// sums in a fixed tree; the order does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_head_reduce(int nhb, const float *__restrict__ Gh, float *__restrict__ gh) {
+  __shared__ float red[16][16];
+  const int t = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int c = blockIdx.x * 16 + t;
+  float s = 0.f;
+  if (c < kHeadPart)
+    for (int b = g; b < nhb; b += 16) s += Gh[long(b) * kHeadPart + c];
+  red[g][t] = s;
+  __syncthreads();
+  for (int h = 8; h >= 1; h >>= 1) {
+    if (g < h) red[g][t] += red[g + h][t];
+    __syncthreads();
+  }
+  if (g == 0 && c < kHeadPart) gh[c] = red[0][t];
 }
 
 // --------------------------------------------------------------- optimiser
@@ -383,24 +461,38 @@ struct LayerTab {
 };
 
 // mode 0: publish the bfloat16 copies only; 1: sum partials into gout;
-// 2: sum partials, Adam update, publish.  Block 0 thread 0 also turns the
+// 2: sum partials, Adam update, publish.  Hidden-layer gradients come as
+// nsplit slices of G, added in slice order; the output layer's (parameters
+// from offHead on) already reduced in gh.  Workgroup 0 also reduces the nhb
 // loss partials into the step's loss (lossh may be null).
 __global__ __launch_bounds__(256) void k_adam(long P, int nsplit, int mode, const float *__restrict__ G,
+                                              const float *__restrict__ gh, int nhb, long offHead,
                                               float *__restrict__ w, float *__restrict__ m1,
                                               float *__restrict__ m2, float lr, float b1, float b2, float eps,
                                               float bc1, float bc2, LayerTab tab, float *__restrict__ gout,
                                               const float *__restrict__ lossp, double inv_no,
                                               double *__restrict__ lossh) {
+  __shared__ double lred[256];
   const long p = long(blockIdx.x) * 256 + threadIdx.x;
-  if (p == 0 && lossh && mode != 0) {
+  if (blockIdx.x == 0 && lossh && mode != 0) {
     double s = 0.0;
-    for (int k = 0; k < nsplit; ++k) s += double(lossp[k]);
-    *lossh = s * inv_no;
+    for (int k = threadIdx.x; k < nhb; k += 256) s += double(lossp[k]);
+    lred[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h >= 1; h >>= 1) {
+      if (int(threadIdx.x) < h) lred[threadIdx.x] += lred[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) *lossh = lred[0] * inv_no;
   }
   if (p >= P) return;
   if (mode != 0) {
     float g = 0.f;
-    for (int k = 0; k < nsplit; ++k) g += G[long(k) * P + p];
+    if (p >= offHead) {
+      g = gh[p - offHead];
+    } else {
+      for (int k = 0; k < nsplit; ++k) g += G[long(k) * P + p];
+    }
     if (gout) gout[p] = g;
     if (mode == 2) {
       const float a = b1 * m1[p] + (1.f - b1) * g;
@@ -432,7 +524,8 @@ struct fvs_mlp {
   long P = 0;  // parameters
   LayerTab tab{};
   hipStream_t stream = nullptr;
-  float *w = nullptr, *m1 = nullptr, *m2 = nullptr, *G = nullptr, *lossp = nullptr, *gtmp = nullptr;
+  float *w = nullptr, *m1 = nullptr, *m2 = nullptr, *G = nullptr, *Gh = nullptr, *gh = nullptr,
+        *lossp = nullptr, *gtmp = nullptr;
   double *lossh = nullptr;
   u16 *X0 = nullptr, *dZa = nullptr, *dZb = nullptr;
   u16 *Z[kMaxDepth] = {}, *H[kMaxDepth] = {};
@@ -508,9 +601,9 @@ int run_pass(fvs_mlp &m, int M, long nvalid, bool grad, float *Y) {
   {
     Prof pf(m, MC_HEAD);
     const float inv_no = 1.0f / float(double(nvalid) * kOut);
-    hipLaunchKernelGGL(k_head, dim3(S), dim3(256), 0, m.stream, chunk, nvalid, inv_no, m.H[L - 1], m.Z[L - 1],
-                       m.w + m.tab.offW[L], m.w + m.tab.offb[L], m.T, Y, grad ? m.dZa : nullptr, m.G, m.P,
-                       m.tab.offW[L], m.tab.offb[L], m.lossp);
+    hipLaunchKernelGGL(k_head, dim3((M + kHeadRows - 1) / kHeadRows), dim3(256), 0, m.stream, long(M), nvalid, inv_no,
+                       m.H[L - 1], m.Z[L - 1],
+                       m.w + m.tab.offW[L], m.w + m.tab.offb[L], m.T, Y, grad ? m.dZa : nullptr, m.Gh, m.lossp);
   }
   if (grad) {
     u16 *cur = m.dZa, *oth = m.dZb;
@@ -534,17 +627,19 @@ int run_pass(fvs_mlp &m, int M, long nvalid, bool grad, float *Y) {
   return 0;
 }
 
-int launch_adam(fvs_mlp &m, int mode, int S, double inv_no, double *lossh, float *gout) {
+int launch_adam(fvs_mlp &m, int mode, int S, int nhb, double inv_no, double *lossh, float *gout) {
   Prof pf(m, MC_ADAM);
+  if (mode != 0)
+    hipLaunchKernelGGL(k_head_reduce, dim3((kHeadPart + 15) / 16), dim3(256), 0, m.stream, nhb, m.Gh, m.gh);
   float bc1 = 1.f, bc2 = 1.f;
   if (mode == 2) {
     m.t += 1;
     bc1 = float(1.0 - std::pow(m.opt.beta1, double(m.t)));
     bc2 = float(1.0 - std::pow(m.opt.beta2, double(m.t)));
   }
-  hipLaunchKernelGGL(k_adam, dim3(unsigned((m.P + 255) / 256)), dim3(256), 0, m.stream, m.P, S, mode, m.G, m.w, m.m1,
-                     m.m2, float(m.opt.lr), float(m.opt.beta1), float(m.opt.beta2), float(m.opt.eps), bc1, bc2, m.tab,
-                     gout, m.lossp, inv_no, lossh);
+  hipLaunchKernelGGL(k_adam, dim3(unsigned((m.P + 255) / 256)), dim3(256), 0, m.stream, m.P, S, mode, m.G, m.gh,
+                     nhb, m.tab.offW[m.L], m.w, m.m1, m.m2, float(m.opt.lr), float(m.opt.beta1), float(m.opt.beta2),
+                     float(m.opt.eps), bc1, bc2, m.tab, gout, m.lossp, inv_no, lossh);
   MLPC(hipGetLastError());
   return 0;
 }
@@ -610,7 +705,9 @@ int fvs_mlp_create(fvs_mlp **out, const fvs_mlp_options *opt) {
   rc = rc ? rc : m->alloc(m->m2, m->P);
   rc = rc ? rc : m->alloc(m->gtmp, m->P);
   rc = rc ? rc : m->alloc(m->G, size_t(kMaxSplit) * m->P);
-  rc = rc ? rc : m->alloc(m->lossp, kMaxSplit);
+  rc = rc ? rc : m->alloc(m->Gh, (M + kHeadRows - 1) / kHeadRows * kHeadPart);
+  rc = rc ? rc : m->alloc(m->gh, kHeadPart);
+  rc = rc ? rc : m->alloc(m->lossp, (M + kHeadRows - 1) / kHeadRows);
   rc = rc ? rc : m->alloc(m->lossh, 1);
   rc = rc ? rc : m->alloc(m->X0, M * kInPad);
   rc = rc ? rc : m->alloc(m->dZa, M * kW);
@@ -672,7 +769,7 @@ int fvs_mlp_set_params(fvs_mlp *m, const float *p) {
   MLPC(hipMemsetAsync(m->m1, 0, size_t(m->P) * sizeof(float), m->stream));
   MLPC(hipMemsetAsync(m->m2, 0, size_t(m->P) * sizeof(float), m->stream));
   m->t = 0;
-  if (int rc = launch_adam(*m, 0, 0, 0.0, nullptr, nullptr)) return rc;
+  if (int rc = launch_adam(*m, 0, 0, 0, 0.0, nullptr, nullptr)) return rc;
   MLPC(hipStreamSynchronize(m->stream));
   return 0;
 }
@@ -722,7 +819,7 @@ int fvs_mlp_train(fvs_mlp *m, int steps, double *loss) {
   for (int s = 0; s < steps && !rc; ++s) {
     launch_features(*m, m->M, m->M, m->raw, m->tgt, m->ndata, m->t + 1, 1, nullptr);
     rc = run_pass(*m, m->M, m->M, true, nullptr);
-    if (!rc) rc = launch_adam(*m, 2, S, inv_no, dl + s, nullptr);
+    if (!rc) rc = launch_adam(*m, 2, S, (m->M + kHeadRows - 1) / kHeadRows, inv_no, dl + s, nullptr);
   }
   if (!rc) {
     hipError_t e = hipEventRecord(m->ev1, m->stream);
@@ -745,7 +842,8 @@ static int grad_or_step(fvs_mlp *m, int n, const float *raw, const float *target
   if (int rc = stage_rows(*m, n, raw, target)) return rc;
   launch_features(*m, M, n, m->iraw, m->itgt, 0, 0, 0, nullptr);
   if (int rc = run_pass(*m, M, n, true, nullptr)) return rc;
-  if (int rc = launch_adam(*m, update ? 2 : 1, S, 1.0 / (double(n) * kOut), m->lossh, update ? nullptr : m->gtmp))
+  if (int rc = launch_adam(*m, update ? 2 : 1, S, (M + kHeadRows - 1) / kHeadRows, 1.0 / (double(n) * kOut), m->lossh,
+                           update ? nullptr : m->gtmp))
     return rc;
   if (grad) MLPC(hipMemcpyAsync(grad, m->gtmp, size_t(m->P) * sizeof(float), hipMemcpyDeviceToHost, m->stream));
   double l = 0.0;
