@@ -9,9 +9,20 @@ mesh; value = cells * stages * steps / seconds (cell-stage updates/s).
 
 Prints ONE JSON line.  Extra fields: the per-kernel breakdown, the roofline
 object of the dominant kernel (algorithmic bytes per launch / mean HIP-event
-duration of that kernel on its stream), the other gradient / limiter variants
-of the same mesh, and the CPU baseline (the C oracle, single thread, on a few
-steps of the same mesh).
+duration of that kernel on its stream; HBM traffic per launch from the
+committed rocprofv3 counter summary of the same workload, when there is one),
+the other gradient / limiter variants of the same mesh, the CPU baseline (the
+C oracle, single thread, on a few steps of the same mesh), and two objects for
+the steady part of the metric:
+
+* "steady": steady solves/hr to a 1e-8 density-residual drop on NACA O-grids
+  (64x32 quads -> 4,096 triangles, Venkatakrishnan, SSP-RK(4,2), CFL 1.25):
+  a warm-start network trained on batched cold solves of seeded conditions,
+  then held-out conditions solved batched cold, batched warm and one at a time
+  cold (plain);
+* "mlp": training-step rate of the warm-start network (68 -> 5 x 256 -> 4,
+  65,536-row batches) with its per-kernel matrix-core rate against the bf16
+  dense peak.
 """
 from __future__ import annotations
 
@@ -27,6 +38,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 matrix cores
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic_vortex1024_fused.json")
 
 
 def model_bytes_per_cell_stage(grad_method: int, limiter: int) -> float:
@@ -71,6 +84,69 @@ def cpu_baseline(mesh, dt: float, steps: int):
     return mesh.ncells * 3 * steps / sec, sec
 
 
+def measured_traffic(n: int, reorder: int, path: int, kernel: str):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary, if it was taken on this workload."""
+    if not os.path.exists(TRAFFIC):
+        return None
+    with open(TRAFFIC) as f:
+        t = json.load(f)
+    if (t.get("n"), t.get("reorder"), t.get("path"), t.get("kernel_class")) != (n, reorder, path, kernel):
+        return None
+    return t.get("bytes_per_launch")
+
+
+def mlp_bench(steps: int):
+    """Training-step rate of the warm-start network on synthetic rows."""
+    from fvs2d_amd.mlp import WarmStartNet
+    rng = np.random.default_rng(0)
+    n = 1 << 20
+    r = np.exp(rng.uniform(np.log(0.01), np.log(50), n))
+    th = rng.uniform(-np.pi, np.pi, n)
+    raw = np.stack([0.5 + r * np.cos(th), r * np.sin(th), rng.uniform(0, 50, n), rng.uniform(0.5, 0.85, n),
+                    rng.uniform(-2, 4, n), rng.uniform(0.1, 0.15, n)], 1).astype(np.float32)
+    tgt = np.stack([np.sin(2 * th) * np.exp(-r), np.cos(th) / (1 + r), raw[:, 3] - 0.675, raw[:, 4] / 4],
+                   1).astype(np.float32)
+    net = WarmStartNet(depth=5, batch=65536, seed=0)
+    net.set_data(raw, tgt)
+    net.train(5)
+    loss = net.train(steps)
+    ms_step = net.last_elapsed_ms / steps
+    kt = net.kernel_times(10)
+    flops_step = sum(v[1] * v[2] for v in kt.values())
+    gemm = {k: v for k, v in kt.items() if k.startswith("gemm")}
+    dom = max(gemm, key=lambda k: gemm[k][0] * gemm[k][2])
+    ms, fl, _ = kt[dom]
+    ach = fl / (ms * 1e-3) / 1e12
+    return {"network": "68 -> 5 x 256 (GELU) -> 4, bf16 operands / fp32 accumulate, Adam",
+            "batch_rows": 65536, "params": net.nparams, "steps": steps, "ms_per_step": round(ms_step, 4),
+            "rows_per_s": round(65536 / (ms_step * 1e-3), 1),
+            "mfma_tflops_per_step": round(flops_step / (ms_step * 1e-3) / 1e12, 1),
+            "loss_first_last": [float(loss[0]), float(loss[-1])],
+            "kernels": {k: {"ms_per_launch": round(v[0], 5), "launches_per_step": v[2],
+                            "TFLOPs": round(v[1] / (v[0] * 1e-3) / 1e12, 1) if v[0] > 0 and v[1] > 0 else None}
+                        for k, v in kt.items()},
+            "roofline": {"bound": "mfma", "achieved": round(ach, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(ach / BF16_PEAK_TFLOPS, 4), "kernel": dom}}
+
+
+def steady_bench(n_train: int, n_test: int, n_seq: int):
+    """Warm versus cold steady solves/hr (see the module docstring)."""
+    from fvs2d_amd import warmstart
+    out = warmstart.run(n_train, n_test, (64, 32), seed=0, steps=4000, batch=65536, lr=1e-3, depth=5, cfl=1.25,
+                        max_steps=60000, drop=1e-8, chunk=250, freeze_after=20000, freeze_drop=None,
+                        freeze_stall=(2000, 0.1), diverge=1e3, fallback=1.0, group=256, n_seq=n_seq,
+                        log=lambda s: print(f"[bench] steady: {s}", file=sys.stderr, flush=True))
+    keep = ("n_train", "n_test", "train_converged", "cold", "warm", "plain_cold", "iteration_ratio_median",
+            "cl_abs_diff_p50", "cd_abs_diff_p50", "warm_batched_vs_plain_cold")
+    d = {k: out[k] for k in keep if k in out}
+    d["workload"] = ("NACA 4-digit O-grids 64x32 quads -> 4096 triangles, M U[0.5,0.85], AoA U[-2,4] deg, "
+                     "t U[0.10,0.15], Venkatakrishnan K=5 (frozen on stall), SSP-RK(4,2) CFL 1.25, "
+                     "tolerance 1e-8 x cold first-step density residual")
+    d["warm_vs_cold_batched"] = round(out["warm"]["solves_per_hour"] / out["cold"]["solves_per_hour"], 3)
+    return d
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +158,11 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--reorder", type=int, default=1, help="device cell order: 1 Morton, 0 reference")
     ap.add_argument("--path", type=int, default=0, help="0 fused LDS kernels, 1 face-flux buffer kernels")
+    ap.add_argument("--mlp-steps", type=int, default=200, help="warm-start network training steps (0 = skip)")
+    ap.add_argument("--steady", type=int, default=1, help="warm vs cold steady solves/hr (0 = skip)")
+    ap.add_argument("--steady-train", type=int, default=256)
+    ap.add_argument("--steady-test", type=int, default=64)
+    ap.add_argument("--steady-seq", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -147,6 +228,16 @@ def main():
                          f"{N}-cell mesh and scheme, {sec:.1f} s"}
         print(f"[bench] cpu oracle: {rate / 1e6:.3f} M cell-stage/s ({sec:.1f}s)", file=sys.stderr, flush=True)
 
+    mlp = None
+    if args.mlp_steps > 0:
+        mlp = mlp_bench(args.mlp_steps)
+        print(f"[bench] mlp: {mlp['ms_per_step']} ms/step, {mlp['mfma_tflops_per_step']} TFLOP/s",
+              file=sys.stderr, flush=True)
+    steady = None
+    if args.steady:
+        steady = steady_bench(args.steady_train, args.steady_test, args.steady_seq)
+
+    traffic = measured_traffic(args.n, args.reorder, args.path, dom)
     out = {
         "metric": "cell-stage updates/s (FP64)",
         "value": round(value, 1),
@@ -166,12 +257,14 @@ def main():
                    "kernels": "fused" if args.path == 0 else "face-buffer",
                    "cells": N, "stages_per_step": stages, "dt": dt, "xcd_numbering": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom},
         "model_bytes_per_cell_stage": model_b,
         "effective_GBps_vs_model": round(value * model_b / 1e9, 1),
         "kernels": kernels,
         "variants_cell_stage_per_s": variants,
         "cpu_baseline": cpu,
+        "steady": steady,
+        "mlp": mlp,
     }
     print(json.dumps(out))
 

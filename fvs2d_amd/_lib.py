@@ -25,7 +25,7 @@ EXPORTS = (
     "fvs_set_state", "fvs_get_state", "fvs_initialize", "fvs_primitives", "fvs_gradient", "fvs_gradient_1var",
     "fvs_limiter", "fvs_compute_residual", "fvs_roe_flux_batch", "fvs_local_dt", "fvs_time_integration",
     "fvs_vortex_error", "fvs_wall_forces", "fvs_set_timestep", "fvs_kernel_times", "fvs_autotune", "fvs_freeze_limiter",
-    "fvs_batch_freeze_limiter", "fvs_last_elapsed_ms",
+    "fvs_batch_freeze_limiter", "fvs_wall_cp", "fvs_last_elapsed_ms",
 )
 
 
@@ -94,6 +94,7 @@ def load():
     L.fvs_autotune.argtypes = [V, I, V, V]
     L.fvs_freeze_limiter.argtypes = [V, I]
     L.fvs_batch_freeze_limiter.argtypes = [V, V]
+    L.fvs_wall_cp.argtypes = [V, V, V, V]
     L.fvs_last_elapsed_ms.restype = D
     L.fvs_last_elapsed_ms.argtypes = [V]
     _lib = L

@@ -559,6 +559,36 @@ __global__ void k_wall_forces(int nw, const int *__restrict__ wf, const int *__r
   out[3] = cn1 * sa_ + ca1 * ca_;
 }
 
+// per wall face: x_f, cp (face-extrapolated pressure), cp (cell pressure),
+// V_n with face-extrapolated velocity (src/io.f90:380-418); in batch
+// contexts Mach number and free-stream pressure of the face's instance
+__global__ void k_wall_cp(int nw, const int *__restrict__ wf, const int *__restrict__ bcell,
+                          const double *__restrict__ geo, const double *__restrict__ W,
+                          const double *__restrict__ grad, double mach, double pinf, const int *__restrict__ binst,
+                          const double *__restrict__ mach_aoa, const double *__restrict__ pinf_inst,
+                          double *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nw) return;
+  const int b = wf[k], c = bcell[b];
+  double g[8], gr[8], w[4];
+  ld8(geo, b, g);
+  ld8(grad, c, gr);
+  ld4(W, c, w);
+  double M = mach, pi = pinf;
+  if (binst) {
+    const int q = binst[b];
+    M = mach_aoa[2 * q];
+    pi = pinf_inst[4 * q + 3];
+  }
+  const double pw = w[3] + g[3] * gr[3] + g[4] * gr[7];
+  const double uw = w[1] + g[3] * gr[1] + g[4] * gr[5];
+  const double vw = w[2] + g[3] * gr[2] + g[4] * gr[6];
+  out[4 * k] = g[5];
+  out[4 * k + 1] = 2.0 / (M * M) * (pw - pi);
+  out[4 * k + 2] = 2.0 / (M * M) * (w[3] - pi);
+  out[4 * k + 3] = uw * g[0] + vw * g[1];
+}
+
 __global__ void k_roe_batch(int n, double g, const double *__restrict__ pL, const double *__restrict__ pR,
                             const double *__restrict__ nxy, double *__restrict__ F, double *__restrict__ ws) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1845,6 +1875,48 @@ int fvs_wall_forces(fvs_ctx *C, double *out4) {
   HIPC(hipGetLastError());
   HIPC(hipMemcpyAsync(out4, C->small + 16, 4 * 8, hipMemcpyDeviceToHost, C->stream));
   HIPC(hipStreamSynchronize(C->stream));
+  return 0;
+}
+
+int fvs_wall_cp(fvs_ctx *C, int *nwall, double *out4, double *un3) {
+  if (int rc = need_state(C)) return rc;
+  if (!nwall) return fail(FVS_EINVAL, "null argument");
+  *nwall = C->nwall;
+  if (un3) un3[0] = un3[1] = un3[2] = 0.0;
+  if (C->nwall == 0 || (!out4 && !un3)) return 0;
+  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  const int save = C->nact;
+  C->nact = 0;
+  int rc = launch_gradient(*C, false);
+  C->nact = save;
+  if (rc) return rc;
+  const size_t n4 = 4 * size_t(C->nwall);
+  double *d = nullptr;
+  HIPC(hipMalloc(&d, n4 * sizeof(double)));
+  const bool batch = C->ninst > 0;
+  hipLaunchKernelGGL(k_wall_cp, dim3(nblk(C->nwall)), dim3(kBlock), 0, C->stream, C->nwall, C->wall, C->fb_cell,
+                     C->fb_geo, C->W, C->grad, C->opt.mach, C->P.pinf[3], batch ? C->dbinst : nullptr,
+                     batch ? C->dmach_aoa : nullptr, batch ? C->dpinf_inst : nullptr, d);
+  std::vector<double> h(n4);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, n4 * sizeof(double), hipMemcpyDeviceToHost, C->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(C->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(FVS_EHIP, std::string("wall cp: ") + hipGetErrorString(e));
+  if (out4) std::copy(h.begin(), h.end(), out4);
+  if (un3) {
+    // the reference's loop order and arithmetic (src/io.f90:413-422)
+    double mx = 0.0, l2 = 0.0, l1 = 0.0;
+    for (int k = 0; k < C->nwall; ++k) {
+      const double un = h[4 * size_t(k) + 3];
+      l2 = l2 + un * un;
+      l1 = l1 + std::fabs(un);
+      mx = std::max(mx, std::fabs(un));
+    }
+    un3[0] = mx;
+    un3[1] = std::sqrt(l2 / double(C->nwall));
+    un3[2] = l1 / double(C->nwall);
+  }
   return 0;
 }
 

@@ -147,6 +147,51 @@ module fvs2d_capi
       integer(c_int) :: rc
     end function
 
+    function fvs_wall_cp(ctx, nwall, out4, un3) bind(C, name="fvs_wall_cp") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: nwall
+      type(c_ptr), value :: out4, un3
+      integer(c_int) :: rc
+    end function
+
+    function fvs_initialize(ctx, mode, t0) bind(C, name="fvs_initialize") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+      real(c_double), value :: t0
+      integer(c_int) :: rc
+    end function
+
+    function fvs_set_graph(ctx, on) bind(C, name="fvs_set_graph") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+      integer(c_int) :: rc
+    end function
+
+    function fvs_freeze_limiter(ctx, frozen) bind(C, name="fvs_freeze_limiter") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: frozen
+      integer(c_int) :: rc
+    end function
+
+    function fvs_sizes(ctx, out8) bind(C, name="fvs_sizes") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: out8(8)
+      integer(c_int) :: rc
+    end function
+
+    function fvs_autotune(ctx, nsteps, choice4, best_ms) bind(C, name="fvs_autotune") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+      type(c_ptr), value :: choice4, best_ms
+      integer(c_int) :: rc
+    end function
+
     function fvs_set_timestep(ctx, dt, cfl) bind(C, name="fvs_set_timestep") result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -358,6 +358,18 @@ class Solver:
         return dict(iterations=conv_at, steps_run=done, history=h, diverged=diverged, limiter_frozen_at=frozen_at,
                     drop=float(h[-1, 0] / r1) if len(h) else None)
 
+    def write_save(self, base: str, itime: int, imach: int = 2):
+        """The reference's save file of the current state (``<base>.cd`` +
+        ``.s8``; src/io.f90:96-113, 166-178), readable by its restart."""
+        from . import restart
+        restart.write_save(base, self.cvar, self.nnodes, itime, imach)
+
+    def read_cont(self, base: str, imach: int = 2):
+        """Set the state from the reference's restart files
+        (src/initialize.f90:56-77)."""
+        from . import restart
+        self.cvar = restart.read_cont(base, self.ncells, self.nnodes, imach)
+
     def error_isentropic_vortex(self, t: float) -> np.ndarray:
         out = np.zeros(13)
         check(self._L.fvs_vortex_error(self._h, t, ptr(out)))
@@ -367,6 +379,17 @@ class Solver:
         out = np.zeros(4)
         check(self._L.fvs_wall_forces(self._h, ptr(out)))
         return out
+
+    def wall_cp(self):
+        """Per wall face x_f, cp, cp_cell and V_n, and the max / L2 / L1 of V_n
+        (fvs_wall_cp; src/io.f90:380-418)."""
+        n = np.zeros(1, np.int32)
+        check(self._L.fvs_wall_cp(self._h, ptr(n), None, None))
+        out = np.zeros((int(n[0]), 4))
+        un3 = np.zeros(3)
+        check(self._L.fvs_wall_cp(self._h, ptr(n), ptr(out), ptr(un3)))
+        return dict(x=out[:, 0], cp=out[:, 1], cp_cell=out[:, 2], un=out[:, 3], un_max=un3[0], un_l2=un3[1],
+                    un_l1=un3[2])
 
     def set_timestep(self, dt: float, cfl: Optional[float] = None):
         cfl = self.scheme.cfl if cfl is None else cfl

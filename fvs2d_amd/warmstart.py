@@ -166,6 +166,18 @@ def warm_solve(b: Batch, net: WarmStartNet, scale, cfl, max_steps, drop, chunk, 
     return res, sec, fell
 
 
+def write_warm_start(net: WarmStartNet, scale, mesh: meshgen.Mesh, n_around: int, cond: dict, base: str = "cont"):
+    """Warm start of one O-grid instance as the reference's restart files
+    (``<base>.cd`` / ``.s8``), so its Fortran driver can start from it with
+    ntstart > 1 (src/initialize.f90:56-77).  Returns the state (ncells, 4)."""
+    from . import restart
+    U = freestream(cond["mach"], cond["aoa"]) + net.predict(records(mesh, n_around, cond)).astype(np.float64) * scale
+    if not _physical(U):
+        U = np.broadcast_to(freestream(cond["mach"], cond["aoa"]), U.shape).copy()
+    restart.write_save(base, U, mesh.nnodes, 0)
+    return U
+
+
 def sequential_cold(conds, cfl, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, limiter=1):
     """Plain (one instance at a time) cold solves: their wall time and count."""
     t0 = time.perf_counter()

@@ -166,6 +166,14 @@ int fvs_vortex_error(fvs_ctx *ctx, double t, double *out13);
 /* write_inst_cp_un C_L/C_D (src/io.f90:340-449), summed over slip/solid wall
  * boundaries: out4 = cl, cd, cl1, cd1 */
 int fvs_wall_forces(fvs_ctx *ctx, double *out4);
+/* The per-face part of write_inst_cp_un (src/io.f90:380-418): *nwall receives
+ * the number of wall faces; out4 (may be NULL) receives per wall face [x_f,
+ * cp with the face-extrapolated pressure, cp with the cell pressure, V_n with
+ * the face-extrapolated velocity], faces in boundary-edge order (batch
+ * contexts: instance by instance, each with its own Mach number); un3 (may
+ * be NULL) receives max |V_n|, sqrt(mean V_n^2) and mean |V_n| over them
+ * (the reference's figures for a mesh with one wall boundary). */
+int fvs_wall_cp(fvs_ctx *ctx, int *nwall, double *out4, double *un3);
 /* Change dt / CFL after creation (the reference reads them from fvs2d.input,
  * module input) */
 int fvs_set_timestep(fvs_ctx *ctx, double dt, double cfl);

@@ -1324,6 +1324,51 @@ void orc_wall_forces(const orc_t *o, double *out) {
   free(pc); free(dp);
 }
 
+/* Wall pressure coefficient and normal velocity per wall face,
+ * src/io.f90:380-418: out[4k..4k+3] = x_f, cp (face-extrapolated pressure),
+ * cp (cell pressure), V_n = (u_f, v_f).n with face-extrapolated velocity;
+ * un3 = max |V_n|, sqrt(mean V_n^2), mean |V_n| over the wall faces in
+ * boundary order (the reference's per-boundary figures when there is one
+ * wall boundary).  Returns the number of wall faces. */
+int orc_wall_cp(const orc_t *o, double *out, double *un3) {
+  int N = o->ncells;
+  double *q = xcalloc(N, 8), *dp = xcalloc(2 * N, 8), *du = xcalloc(2 * N, 8), *dv = xcalloc(2 * N, 8);
+  for (int ic = 0; ic < N; ++ic) q[ic] = o->pvar[4 * ic + 3];
+  orc_gradient_1var(o, q, dp);
+  for (int ic = 0; ic < N; ++ic) q[ic] = o->pvar[4 * ic + 1];
+  orc_gradient_1var(o, q, du);
+  for (int ic = 0; ic < N; ++ic) q[ic] = o->pvar[4 * ic + 2];
+  orc_gradient_1var(o, q, dv);
+  double M = o->op.mach, pinf = o->pvar_inf[3];
+  double un_max = 0.0, un_l2 = 0.0, un_l1 = 0.0;
+  int k = 0;
+  for (int ib = 0; ib < o->nb; ++ib) {
+    if (o->bnd_type[ib] != ORC_BC_SLIP_WALL && o->bnd_type[ib] != ORC_BC_SOLID_WALL) continue;
+    for (int i = 0; i < o->bnd_nedges[ib]; ++i) {
+      int ie = o->bnd_edges[o->bnd_eptr[ib] + i];
+      int ic = o->bnd_cells[o->bnd_cptr[ib] + i];
+      double ddx = o->ex[ie] - o->cx[ic], ddy = o->ey[ie] - o->cy[ic];
+      double pw = o->pvar[4 * ic + 3] + ddx * dp[ic] + ddy * dp[N + ic];
+      double uw = o->pvar[4 * ic + 1] + ddx * du[ic] + ddy * du[N + ic];
+      double vw = o->pvar[4 * ic + 2] + ddx * dv[ic] + ddy * dv[N + ic];
+      double un = uw * o->enx[ie] + vw * o->eny[ie];
+      out[4 * k] = o->ex[ie];
+      out[4 * k + 1] = 2.0 / (M * M) * (pw - pinf);
+      out[4 * k + 2] = 2.0 / (M * M) * (o->pvar[4 * ic + 3] - pinf);
+      out[4 * k + 3] = un;
+      un_l2 = un_l2 + un * un;
+      un_l1 = un_l1 + fabs(un);
+      if (fabs(un) > un_max) un_max = fabs(un);
+      ++k;
+    }
+  }
+  un3[0] = un_max;
+  un3[1] = k ? sqrt(un_l2 / (double)k) : 0.0;
+  un3[2] = k ? un_l1 / (double)k : 0.0;
+  free(q); free(dp); free(du); free(dv);
+  return k;
+}
+
 /* mms_init source terms at cell centres: src/mms.f90:106-115 */
 void orc_mms_source(const orc_t *o, double *src) {
   for (int ic = 0; ic < o->ncells; ++ic) {

@@ -100,6 +100,7 @@ void orc_time_integration(orc_t *o, double t1, int nsteps, double *norms, double
 void orc_vortex_error(const orc_t *o, double t, double *out13);
 /* wall forces over slip/solid walls: cl, cd, cl1, cd1 (src/io.f90:340-449) */
 void orc_wall_forces(const orc_t *o, double *out4);
+int orc_wall_cp(const orc_t *o, double *out, double *un3);
 void orc_mms_source(const orc_t *o, double *src /* 4*ncells */);
 
 #ifdef __cplusplus

@@ -29,9 +29,18 @@ class Opts(C.Structure):
     ]
 
 
+REF_SRC = "/root/reference/src/ios_unstrc.f90"
+IOS_REF = os.path.join(HERE, "_ref", "ios_ref")
+FLANG = "/opt/rocm/bin/amdflang"
+
+
 def build() -> str:
-    """Compile the oracle library in place (gcc, via oracle/Makefile)."""
+    """Compile the oracle library in place (gcc, via oracle/Makefile), and,
+    where the reference sources and amdflang are present, the reference's
+    restart-file module under its test driver (oracle/_ref/ios_ref)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if os.path.exists(REF_SRC) and os.path.exists(FLANG):
+        subprocess.run(["make", "-s", "-B", "-C", HERE, "ref"], check=True)
     return LIB
 
 
@@ -66,6 +75,8 @@ def lib():
                                    dp, dp]
         L.orc_vortex_error.argtypes = [P, C.c_double, dp]
         L.orc_wall_forces.argtypes = [P, dp]
+        L.orc_wall_cp.argtypes = [P, dp, dp]
+        L.orc_wall_cp.restype = C.c_int
         L.orc_mms_source.argtypes = [P, dp]
         L.orc_gradient_1var.argtypes = [P, dp, dp]
         L.orc_lsq_nnz.restype = C.c_int
@@ -260,6 +271,16 @@ class Oracle:
         out = np.zeros(4)
         self.L.orc_wall_forces(C.c_void_p(self.h), out)
         return out
+
+    def wall_cp(self):
+        """Per wall face [x_f, cp, cp_cell, V_n] and [max, L2, L1] of V_n
+        (src/io.f90:380-418)."""
+        ls = self.lists()
+        nmax = int(np.sum(ls["bnd_nedges"]))
+        out = np.zeros(4 * max(nmax, 1))
+        un3 = np.zeros(3)
+        k = self.L.orc_wall_cp(C.c_void_p(self.h), out, un3)
+        return out[: 4 * k].reshape(k, 4), un3
 
     def mms_source(self):
         out = np.zeros((self.ncells, 4))

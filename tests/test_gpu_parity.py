@@ -265,3 +265,37 @@ def test_naca_steady_and_forces(limiter):
     f_ref = orc.wall_forces()
     f_gpu = gpu.wall_forces()
     assert np.allclose(f_gpu, f_ref, rtol=1e-10, atol=1e-13)
+    # the per-face cp / V_n output of the same state (src/io.f90:380-418)
+    faces_ref, un3_ref = orc.wall_cp()
+    w = gpu.wall_cp()
+    got = np.column_stack([w["x"], w["cp"], w["cp_cell"], w["un"]])
+    assert got.shape == faces_ref.shape
+    assert np.array_equal(got[:, 0], faces_ref[:, 0])
+    for k in (1, 2, 3):
+        assert np.max(np.abs(got[:, k] - faces_ref[:, k])) <= 1e-10 * np.max(np.abs(faces_ref[:, k]))
+    assert np.allclose([w["un_max"], w["un_l2"], w["un_l1"]], un3_ref, rtol=1e-9)
+
+
+def test_batch_wall_cp_matches_single():
+    """fvs_wall_cp on a batch context: each instance's faces, with its own
+    Mach number, equal the single-instance output of the same state."""
+    from fvs2d_amd import cases
+    from fvs2d_amd.solver import BatchSolver
+    inst = [((40, 20), (0.7, 1.0)), ((32, 16), (0.6, -2.0))]
+    meshes = [cases.naca(n, l, cfl=1.0)[0] for (n, l), _ in inst]
+    _, sch = cases.naca(16, 8, cfl=1.0)
+    b = BatchSolver(meshes, [ma for _, ma in inst], sch)
+    b.initialize_freestream()
+    b.batch_time_integration(0.0, 20)
+    wb = b.wall_cp()
+    k0 = 0
+    for k, ((n, l), (M, a)) in enumerate(inst):
+        m, sc = cases.naca(n, l, mach=M, aoa=a, cfl=1.0)
+        s = Solver(m, sc)
+        s.cvar = b.instance_state(k)
+        ws = s.wall_cp()
+        nw = ws["x"].size
+        for key in ("x", "cp", "cp_cell", "un"):
+            assert np.allclose(wb[key][k0:k0 + nw], ws[key], rtol=1e-12, atol=1e-14), (k, key)
+        k0 += nw
+    assert k0 == wb["x"].size

@@ -456,3 +456,42 @@ def test_wall_forces_linear_pressure():
     al = math.radians(1.25)
     assert math.isclose(out[0], cn * math.cos(al) - ca * math.sin(al), rel_tol=1e-9)
     assert math.isclose(out[1], cn * math.sin(al) + ca * math.cos(al), rel_tol=1e-9)
+
+
+def test_wall_cp_linear_fields():
+    """Linear pressure and velocity fields with exact (LSQ) gradients: at each
+    wall face midpoint cp = (2/M^2)(p(x_f) - p_inf) and V_n = (u(x_f),
+    v(x_f)).n exactly; the cell-pressure cp uses the centroid value; and the
+    face cp integrates to the forces of test_wall_forces_linear_pressure."""
+    m = load_example("naca0012_example")
+    M = 0.8
+    o = Oracle(m, grad_method=3, lsq_nn=1, mach=M, aoa_deg=1.25)
+    cl = o.cells()
+    pv = np.zeros((o.ncells, 4))
+    pv[:, 0] = 1.0
+    pv[:, 1] = 0.5 - 0.2 * cl["x"] + 0.1 * cl["y"]
+    pv[:, 2] = 0.05 + 0.3 * cl["x"] - 0.4 * cl["y"]
+    pv[:, 3] = 2.0 + 0.3 * cl["x"] - 0.7 * cl["y"]
+    o.pvar = pv
+    faces, un3 = o.wall_cp()
+    e, ls = o.edges(), o.lists()
+    we = ls["bnd_edges"][: ls["bnd_nedges"][0]]
+    assert faces.shape == (we.size, 4)
+    xf, yf = e["x"][we], e["y"][we]
+    pinf = 1.0 / 1.4
+    assert np.allclose(faces[:, 0], xf, rtol=0, atol=0)
+    assert np.allclose(faces[:, 1], 2 / M ** 2 * (2.0 + 0.3 * xf - 0.7 * yf - pinf), rtol=1e-12, atol=1e-12)
+    un = (0.5 - 0.2 * xf + 0.1 * yf) * e["nx"][we] + (0.05 + 0.3 * xf - 0.4 * yf) * e["ny"][we]
+    assert np.allclose(faces[:, 3], un, rtol=1e-12, atol=1e-12)
+    bc = m.boundaries[0][1]
+    assert np.allclose(faces[:, 2], 2 / M ** 2 * (pv[bc, 3] - pinf), rtol=1e-14, atol=1e-14)
+    assert math.isclose(un3[0], np.max(np.abs(un)), rel_tol=1e-12)
+    assert math.isclose(un3[1], math.sqrt(np.mean(un ** 2)), rel_tol=1e-12)
+    assert math.isclose(un3[2], np.mean(np.abs(un)), rel_tol=1e-12)
+    # forces from the face cp in the reference's C_N / C_A sums
+    A = e["area"][we]
+    cn, ca = np.sum(faces[:, 1] * e["ny"][we] * A), np.sum(faces[:, 1] * e["nx"][we] * A)
+    al = math.radians(1.25)
+    f = o.wall_forces()
+    assert math.isclose(f[0], cn * math.cos(al) - ca * math.sin(al), rel_tol=1e-10)
+    assert math.isclose(f[1], cn * math.sin(al) + ca * math.cos(al), rel_tol=1e-10)
