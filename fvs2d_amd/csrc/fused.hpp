@@ -32,13 +32,25 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const doubl
                                                       const int *__restrict__ blist,
                                                       const int *__restrict__ ioff,
                                                       const int *__restrict__ frz, int ninst) {
-  const int i = (blist ? blist[blockIdx.x] : xcd_block(nb)) * kBlock + threadIdx.x;
+  const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
+  if (lb >= (nb < 0 ? -nb : nb)) return;  // the whole workgroup leaves before the barrier
+  // the block's own states and centres in LDS: neighbours inside the block
+  // (most of them, in Morton order) are read from there
+  __shared__ double sW[4 * kBlock];
+  __shared__ double2 sX[kBlock];
+  const int i0 = lb * kBlock, i = i0 + threadIdx.x;
+  double w[4], sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, mn[4], mx[4];
+  double2 ci = make_double2(0.0, 0.0);
+  if (i < N) {
+    ld4(W, i, w);
+    ci = cxy[i];
+    st4(sW, threadIdx.x, w);
+    sX[threadIdx.x] = ci;
+  }
+  __syncthreads();
   if (i >= N) return;
   const int s = i / kSlice, l = i % kSlice;
   const int base = soff[s], wid = swid[s];
-  const double2 ci = cxy[i];
-  double w[4], sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, mn[4], mx[4];
-  ld4(W, i, w);
   for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
   // four neighbours per round: their ids, then their centres and states are
   // all in flight before the first one is used (memory-level parallelism for a
@@ -78,8 +90,14 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const doubl
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int j = int(code[u] & 0x07ffffffu);
-      cj[u] = cxy[j];
-      ld4(W, j, q[u]);
+      const unsigned lj = unsigned(j - i0);
+      if (lj < unsigned(kBlock)) {
+        cj[u] = sX[lj];
+        ld4(sW, lj, q[u]);
+      } else {
+        cj[u] = cxy[j];
+        ld4(W, j, q[u]);
+      }
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) accumulate(code[u], cj[u], q[u]);
@@ -87,9 +105,17 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const doubl
   for (; k < wid; ++k) {
     const unsigned code = sid[base + long(k) * kSlice + l];
     const int j = int(code & 0x07ffffffu);
+    const unsigned lj = unsigned(j - i0);
     double q[4];
-    ld4(W, j, q);
-    accumulate(code, cxy[j], q);
+    double2 cj;
+    if (lj < unsigned(kBlock)) {
+      cj = sX[lj];
+      ld4(sW, lj, q);
+    } else {
+      cj = cxy[j];
+      ld4(W, j, q);
+    }
+    accumulate(code, cj, q);
   }
   const double2 g01 = reinterpret_cast<const double2 *>(ginv)[2 * i];
   const double2 g2 = reinterpret_cast<const double2 *>(ginv)[2 * i + 1];
@@ -118,15 +144,9 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const doubl
 }
 
 // interior face flux with L = c1, R = c2 (shared by both flux kernels)
-__device__ __forceinline__ void interior_flux(const Phys &P, int2 c, const double g[8], const double *__restrict__ W,
-                                              const double *__restrict__ grad, const double *__restrict__ phi,
-                                              double fa[4], double &wsa) {
-  double wL[4], wR[4], gL[8], gR[8];
-  ld4(W, c.x, wL);
-  ld4(W, c.y, wR);
-  ld8(grad, c.x, gL);
-  ld8(grad, c.y, gR);
-  const double phL = phi[c.x], phR = phi[c.y];
+__device__ __forceinline__ void interior_flux_v(const Phys &P, const double g[8], const double wL[4],
+                                                const double wR[4], const double gL[8], const double gR[8],
+                                                double phL, double phR, double fa[4], double &wsa) {
   const double k2 = P.kappa / 2.0, k1 = 1.0 - P.kappa;
   double pL[4], pR[4];
   for (int v = 0; v < 4; ++v) {
@@ -143,14 +163,20 @@ __device__ __forceinline__ void interior_flux(const Phys &P, int2 c, const doubl
   wsa = ws * A;
 }
 
-__device__ __forceinline__ void boundary_flux(const Phys &P, double t, int c, int ty, const double g[8],
-                                              const double *__restrict__ W, const double *__restrict__ grad,
-                                              const double *__restrict__ phi, double fa[4], double &wsa,
-                                              const double *__restrict__ pinf) {
-  double w[4], gr[8];
-  ld4(W, c, w);
-  ld8(grad, c, gr);
-  const double ph = phi[c];
+__device__ __forceinline__ void interior_flux(const Phys &P, int2 c, const double g[8], const double *__restrict__ W,
+                                              const double *__restrict__ grad, const double *__restrict__ phi,
+                                              double fa[4], double &wsa) {
+  double wL[4], wR[4], gL[8], gR[8];
+  ld4(W, c.x, wL);
+  ld4(W, c.y, wR);
+  ld8(grad, c.x, gL);
+  ld8(grad, c.y, gR);
+  interior_flux_v(P, g, wL, wR, gL, gR, phi[c.x], phi[c.y], fa, wsa);
+}
+
+__device__ __forceinline__ void boundary_flux_v(const Phys &P, double t, int ty, const double g[8], const double w[4],
+                                                const double gr[8], double ph, double fa[4], double &wsa,
+                                                const double *__restrict__ pinf) {
   double pL[4], pR[4];
   for (int v = 0; v < 4; ++v) pL[v] = w[v] + ph * (g[3] * gr[v] + g[4] * gr[4 + v]);
   const double nx = g[0], ny = g[1];
@@ -174,52 +200,46 @@ __device__ __forceinline__ void boundary_flux(const Phys &P, double t, int c, in
   wsa = ws * A;
 }
 
-// fused face fluxes (LDS) + ordered gather + stage update; reads the stage's
-// primitive state from Win and writes the next one to Wout (separate buffers:
-// neighbouring blocks still read Win)
-__global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, int maxf, Upd A, Phys P, double t,
-                                                        const int *__restrict__ boff, const int *__restrict__ bfaces,
-                                                        const int *__restrict__ lgat, int Fi,
-                                                        const int2 *__restrict__ fc, const double *__restrict__ fgeo,
-                                                        const int *__restrict__ bcell, const int *__restrict__ btype,
-                                                        const double *__restrict__ bgeo,
-                                                        const double *__restrict__ Win,
-                                                        const double *__restrict__ grad,
-                                                        const double *__restrict__ phi,
-                                                        const double *__restrict__ vol, double *__restrict__ Ua,
-                                                        double *__restrict__ Ub, double *__restrict__ Facc,
-                                                        double *__restrict__ Wout, double *__restrict__ dtl,
-                                                        double *__restrict__ part, const int *__restrict__ blist,
-                                                        const int *__restrict__ binst,
-                                                        const double *__restrict__ pinf_inst,
-                                                        double *__restrict__ d2out) {
-  extern __shared__ double lds[];
-  double *lF = lds;             // [maxf][4] flux*area
-  double *lS = lds + 4 * maxf;  // [maxf]    wave speed*area
-  const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
-  const int nbl = nb < 0 ? -nb : nb;
-  if (lb >= nbl) return;  // the whole workgroup leaves before any barrier
-  const int f0 = boff[lb], nf = boff[lb + 1] - f0;
-  for (int f = threadIdx.x; f < nf; f += kBlock) {
-    const int slot = bfaces[f0 + f];
-    double fa[4], wsa, g[8];
-    if (slot < Fi) {
-      ld8(fgeo, slot, g);
-      interior_flux(P, fc[slot], g, Win, grad, phi, fa, wsa);
-    } else {
-      const int b = slot - Fi;
-      ld8(bgeo, b, g);
-      boundary_flux(P, t, bcell[b], btype[b], g, Win, grad, phi, fa, wsa,
-                    binst ? pinf_inst + 4 * binst[b] : P.pinf);
-    }
-    for (int v = 0; v < 4; ++v) lF[4 * f + v] = fa[v];
-    lS[f] = wsa;
-  }
+__device__ __forceinline__ void boundary_flux(const Phys &P, double t, int c, int ty, const double g[8],
+                                              const double *__restrict__ W, const double *__restrict__ grad,
+                                              const double *__restrict__ phi, double fa[4], double &wsa,
+                                              const double *__restrict__ pinf) {
+  double w[4], gr[8];
+  ld4(W, c, w);
+  ld8(grad, c, gr);
+  boundary_flux_v(P, t, ty, g, w, gr, phi[c], fa, wsa, pinf);
+}
+
+// gather of the block's face fluxes from LDS in the reference's per-cell
+// order (src/residual.f90:97-101,153-155), -R/vol (164-166), local time step
+// (src/runge_kutta.f90:433), stage update, next primitive state, and the
+// last stage's squared increments (per-block sums for the residual norm)
+// fixed-shape tree over NT threads (deterministic); with NT = 2 kBlock the
+// upper half contributes exact zeros, so the sums equal the kBlock tree's
+template <int NT>
+__device__ __forceinline__ void block_sum4_t(double v[4], double *red) {
+  const int t = threadIdx.x;
+  for (int k = 0; k < 4; ++k) red[k * NT + t] = v[k];
   __syncthreads();
-  __shared__ double red[4 * kBlock];
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s)
+      for (int k = 0; k < 4; ++k) red[k * NT + t] = red[k * NT + t] + red[k * NT + t + s];
+    __syncthreads();
+  }
+  for (int k = 0; k < 4; ++k) v[k] = red[k * NT];
+}
+
+template <int NT>
+__device__ __forceinline__ void block_update(int N, int lb, int gw, const Upd &A, const double *__restrict__ lF,
+                                             const double *__restrict__ lS, const int *__restrict__ lgat,
+                                             const double *__restrict__ vol, double *__restrict__ Ua,
+                                             double *__restrict__ Ub, double *__restrict__ Facc,
+                                             double *__restrict__ Wout, double *__restrict__ dtl,
+                                             double *__restrict__ part, double *__restrict__ d2out,
+                                             double *__restrict__ red) {
   const int i = lb * kBlock + threadIdx.x;
   double d2[4] = {0, 0, 0, 0};
-  if (i < N) {
+  if (threadIdx.x < kBlock && i < N) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0}, wacc = 0.0;
     for (int k = 0; k < gw; ++k) {
       const int s = lgat[long(k) * N + i];
@@ -284,10 +304,220 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
     st4(Wout, i, w);
   }
   if (A.last) {
-    block_sum4(d2, red);
+    __syncthreads();  // red may alias the face fluxes just gathered
+    block_sum4_t<NT>(d2, red);
     if (threadIdx.x == 0)
       for (int v = 0; v < 4; ++v) part[4 * lb + v] = d2[v];
   }
+}
+
+// fused face fluxes (LDS) + ordered gather + stage update; reads the stage's
+// primitive state from Win and writes the next one to Wout (separate buffers:
+// neighbouring blocks still read Win)
+__global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, int maxf, Upd A, Phys P, double t,
+                                                        const int *__restrict__ boff, const int *__restrict__ bfaces,
+                                                        const int *__restrict__ lgat, int Fi,
+                                                        const int2 *__restrict__ fc, const double *__restrict__ fgeo,
+                                                        const int *__restrict__ bcell, const int *__restrict__ btype,
+                                                        const double *__restrict__ bgeo,
+                                                        const double *__restrict__ Win,
+                                                        const double *__restrict__ grad,
+                                                        const double *__restrict__ phi,
+                                                        const double *__restrict__ vol, double *__restrict__ Ua,
+                                                        double *__restrict__ Ub, double *__restrict__ Facc,
+                                                        double *__restrict__ Wout, double *__restrict__ dtl,
+                                                        double *__restrict__ part, const int *__restrict__ blist,
+                                                        const int *__restrict__ binst,
+                                                        const double *__restrict__ pinf_inst,
+                                                        double *__restrict__ d2out) {
+  extern __shared__ double lds[];
+  double *lF = lds;             // [maxf][4] flux*area (later the block reduction)
+  double *lS = lds + 4 * maxf;  // [maxf]    wave speed*area
+  __shared__ double sG[8 * kBlock];  // gradients of the block's own cells
+  const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
+  const int nbl = nb < 0 ? -nb : nb;
+  if (lb >= nbl) return;  // the whole workgroup leaves before any barrier
+  const int i0 = lb * kBlock;
+  if (i0 + int(threadIdx.x) < N) {
+    double gg[8];
+    ld8(grad, i0 + threadIdx.x, gg);
+    st8(sG, threadIdx.x, gg);
+  }
+  __syncthreads();
+  const int f0 = boff[lb], nf = boff[lb + 1] - f0;
+  for (int f = threadIdx.x; f < nf; f += kBlock) {
+    const int slot = bfaces[f0 + f];
+    double fa[4], wsa, g[8];
+    if (slot < Fi) {
+      ld8(fgeo, slot, g);
+      const int2 c = fc[slot];
+      double wL[4], wR[4], gL[8], gR[8];
+      ld4(Win, c.x, wL);
+      ld4(Win, c.y, wR);
+      const unsigned la = unsigned(c.x - i0), lc = unsigned(c.y - i0);
+      if (la < unsigned(kBlock)) ld8(sG, la, gL);
+      else ld8(grad, c.x, gL);
+      if (lc < unsigned(kBlock)) ld8(sG, lc, gR);
+      else ld8(grad, c.y, gR);
+      interior_flux_v(P, g, wL, wR, gL, gR, phi[c.x], phi[c.y], fa, wsa);
+    } else {
+      const int b = slot - Fi, c = bcell[b];
+      ld8(bgeo, b, g);
+      double w[4], gr[8];
+      ld4(Win, c, w);
+      ld8(sG, c - i0, gr);  // a boundary face's cell is one of the block's own
+      boundary_flux_v(P, t, btype[b], g, w, gr, phi[c], fa, wsa, binst ? pinf_inst + 4 * binst[b] : P.pinf);
+    }
+    for (int v = 0; v < 4; ++v) lF[4 * f + v] = fa[v];
+    lS[f] = wsa;
+  }
+  __syncthreads();
+  block_update<kBlock>(N, lb, gw, A, lF, lS, lgat, vol, Ua, Ub, Facc, Wout, dtl, part, d2out, lF);
+}
+
+// Whole RK stage in one kernel per 256-cell block: the block stages, in LDS,
+// the state and centre of every cell its gradients read (C2: the block's
+// cells, then the other cells of its faces -- together C1 -- then the rest
+// of their stencils), computes the gradient (and limiter) of the C1 cells
+// from LDS, the fluxes of its faces from LDS, and the gather + update of its
+// own cells, with 512 threads per 256-cell block.  Halo gradients are
+// recomputed by every block that needs them
+// (about 19% more gradients on triangle meshes), so gradients never travel
+// through HBM.  Arithmetic and summation order are those of k_grad_lsqc and
+// k_flux_update, so the results are bit-identical to them.
+//
+// Per block b: C2 cells s2list[s2off[b] .. s2off[b+1]), the first s2n1[b]
+// of them C1 (its own cells first, in order); C1 stencils in slices of 64
+// cells (slice sl0[b] + q/64, width slw, entries slid[sleo + k*64 + q%64] =
+// local C2 index << 4 | periodic-shift code); listed face f's cells as local
+// indices blf[f] = c1 | c2 << 16 (c2 = 0xffff on a boundary face).
+// phisrc: 0 -> limiter values 1 (no limiter), 1 -> read phi (frozen); with
+// LIM the limiter is evaluated and written for the block's own cells.
+constexpr int kStageThreads = 2 * kBlock;  // two threads per cell: each phase fits one round
+
+template <int LIM, int PMODE>
+__global__ __launch_bounds__(kStageThreads) void k_stage(
+    int N, int nb, int gw, int maxf, int max1, int max2, Upd A, Phys P, double t, const int *__restrict__ boff,
+    const int *__restrict__ bfaces, const unsigned *__restrict__ blf, const int *__restrict__ lgat, int Fi,
+    const double *__restrict__ fgeo, const int *__restrict__ btype, const double *__restrict__ bgeo,
+    const int *__restrict__ s2off, const int *__restrict__ s2n1, const int *__restrict__ s2list,
+    const int *__restrict__ sl0, const int *__restrict__ slw, const long *__restrict__ sleo,
+    const unsigned short *__restrict__ slid, const double2 *__restrict__ cxy, const double *__restrict__ ginv,
+    ShiftTable st, double p, int phisrc, double *__restrict__ phi, int lt, const int *__restrict__ nvrt,
+    const double2 *__restrict__ loff, const double *__restrict__ vol, const int *__restrict__ ioff,
+    const int *__restrict__ frz, int ninst, const double *__restrict__ Win, double *__restrict__ Ua,
+    double *__restrict__ Ub, double *__restrict__ Facc, double *__restrict__ Wout, double *__restrict__ dtl,
+    double *__restrict__ part, const int *__restrict__ blist, const int *__restrict__ binst,
+    const double *__restrict__ pinf_inst, double *__restrict__ d2out) {
+  extern __shared__ double lds[];
+  double *lW = lds;                                   // [max2][4]
+  double2 *lX = reinterpret_cast<double2 *>(lds + 4 * max2);  // [max2]
+  double *lG = lds + 6 * max2;                        // [max1][8]
+  double *lP = lG + 8 * max1;                         // [max1]
+  double *lF = lP + max1;                             // [maxf][4]
+  double *lS = lF + 4 * maxf;                         // [maxf]
+  const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
+  const int nbl = nb < 0 ? -nb : nb;
+  if (lb >= nbl) return;  // the whole workgroup leaves before any barrier
+  const int o2 = s2off[lb], n2 = s2off[lb + 1] - o2, n1 = s2n1[lb];
+  const int i0 = lb * kBlock, nown = (N - i0 < kBlock) ? N - i0 : kBlock;
+  // stage the state and centres of C2
+  for (int q = threadIdx.x; q < n2; q += kStageThreads) {
+    const int id = s2list[o2 + q];
+    double w[4];
+    ld4(Win, id, w);
+    st4(lW, q, w);
+    lX[q] = cxy[id];
+  }
+  __syncthreads();
+  // gradients (and limiter values) of C1 from LDS
+  for (int q = threadIdx.x; q < n1; q += kStageThreads) {
+    const int id = s2list[o2 + q];
+    const int si = sl0[lb] + q / kSlice, l = q % kSlice, wid = slw[si];
+    const long base = sleo[si];
+    const double2 ci = lX[q];
+    double w[4], sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, mn[4], mx[4];
+    ld4(lW, q, w);
+    for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
+    for (int k = 0; k < wid; ++k) {
+      const unsigned code = slid[base + long(k) * kSlice + l];
+      const int j = int(code >> 4), sh = int(code & 15u);
+      const double2 cj = lX[j];
+      double q4[4];
+      ld4(lW, j, q4);
+      double xj = cj.x, yj = cj.y;
+      if (sh) {
+        xj = xj + st.sx[sh - 1];
+        yj = yj + st.sy[sh - 1];
+      }
+      const double dx = xj - ci.x, dy = yj - ci.y;
+      double w2 = 1.0;
+      if (PMODE != 0) {
+        const double r2 = dx * dx + dy * dy;
+        if (PMODE == 1) w2 = (r2 > 0.0) ? 1.0 / r2 : 0.0;
+        else if (PMODE == 2) w2 = (r2 > 0.0) ? 1.0 / (r2 * r2) : 0.0;
+        else w2 = (r2 > 0.0) ? pow(r2, -p) : 0.0;
+      }
+      for (int v = 0; v < 4; ++v) {
+        const double tt = w2 * (q4[v] - w[v]);
+        sx[v] = sx[v] + tt * dx;
+        sy[v] = sy[v] + tt * dy;
+        if (LIM) {
+          mn[v] = (q4[v] < mn[v]) ? q4[v] : mn[v];
+          mx[v] = (q4[v] > mx[v]) ? q4[v] : mx[v];
+        }
+      }
+    }
+    const double2 g01 = reinterpret_cast<const double2 *>(ginv)[2 * id];
+    const double2 g2 = reinterpret_cast<const double2 *>(ginv)[2 * id + 1];
+    double gx[4], gy[4];
+    for (int v = 0; v < 4; ++v) {
+      gx[v] = g01.x * sx[v] + g01.y * sy[v];
+      gy[v] = g01.y * sx[v] + g2.x * sy[v];
+    }
+    double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
+    st8(lG, q, g8);
+    double ph = 1.0;
+    bool keep = !LIM;
+    if (LIM && ioff) {
+      int lo = 0, hi = ninst;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (ioff[mid] <= id) lo = mid;
+        else hi = mid;
+      }
+      keep = frz[lo] != 0;
+    }
+    if (!keep) {
+      ph = cell_limiter(lt, nvrt[id], N, id, loff, vol[id], w, mn, mx, gx, gy);
+      if (q < nown) phi[id] = ph;
+    } else if (phisrc == 1) {
+      ph = phi[id];
+    }
+    lP[q] = ph;
+  }
+  __syncthreads();
+  // face fluxes from LDS
+  const int f0 = boff[lb], nf = boff[lb + 1] - f0;
+  for (int f = threadIdx.x; f < nf; f += kStageThreads) {
+    const int slot = bfaces[f0 + f];
+    const unsigned pr = blf[f0 + f];
+    const int la = int(pr & 0xffffu), lc = int(pr >> 16);
+    double fa[4], wsa, g[8];
+    if (slot < Fi) {
+      ld8(fgeo, slot, g);
+      interior_flux(P, make_int2(la, lc), g, lW, lG, lP, fa, wsa);
+    } else {
+      const int b = slot - Fi;
+      ld8(bgeo, b, g);
+      boundary_flux(P, t, la, btype[b], g, lW, lG, lP, fa, wsa, binst ? pinf_inst + 4 * binst[b] : P.pinf);
+    }
+    for (int v = 0; v < 4; ++v) lF[4 * f + v] = fa[v];
+    lS[f] = wsa;
+  }
+  __syncthreads();
+  // the staged states are no longer read: their LDS holds the reduction
+  block_update<kStageThreads>(N, lb, gw, A, lF, lS, lgat, vol, Ua, Ub, Facc, Wout, dtl, part, d2out, lds);
 }
 
 // per-instance residual norms of one step from the per-cell squared

@@ -103,13 +103,18 @@ __global__ void k_prim(int N, double g, const double *__restrict__ U, double *__
 __device__ __forceinline__ double cell_limiter(int lt, int nf, int N, int i, const double2 *__restrict__ off,
                                                double vol, const double w[4], const double mn[4],
                                                const double mx[4], const double gx[4], const double gy[4]) {
+  // face offsets in registers: every index below is a compile-time constant
   double2 o[4];
+#pragma unroll
   for (int k = 0; k < 4; ++k) o[k] = (k < nf) ? off[long(k) * N + i] : make_double2(0.0, 0.0);
   double phi = 0.0;
+#pragma unroll
   for (int v = 0; v < 4; ++v) {
     const double q = w[v], dmax = mx[v] - q, dmin = mn[v] - q;
     double pe = 100.0;
-    for (int k = 0; k < nf; ++k) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k >= nf) break;
       const double pf = q + o[k].x * gx[v] + o[k].y * gy[v];
       const double diff = pf - q;
       double e = 1.0;
@@ -662,6 +667,14 @@ struct fvs_ctx {
   bool xcd_grad = false, xcd_flux = false, xcd_upd = false, xcd_fused = true;
   // limiter values frozen at their last evaluation (steady convergence aid)
   bool limiter_frozen = false;
+  // whole-stage kernel (k_stage): per-block C2 lists, local stencils and face cells
+  bool stage = false;
+  int max1 = 0, max2 = 0;
+  int *s2off = nullptr, *s2n1 = nullptr, *s2list = nullptr, *sl0 = nullptr, *slw = nullptr;
+  long *sleo = nullptr;
+  unsigned short *slid = nullptr;
+  unsigned *blf = nullptr;
+  long nslid = 0, nlist2 = 0;
   // batch mode: per-instance frozen flags (dfrz) while only some are frozen
   int *dfrz = nullptr;
   bool frz_some = false;
@@ -775,6 +788,132 @@ void pack_sell(int N, const std::vector<int> &perm, const std::vector<int> &iper
     }
 }
 
+// Per-block data of the whole-stage kernel (k_stage).  For block b: C1 =
+// its own cells (in order) then the other cells of its listed faces
+// (ascending); C2 = C1 then the remaining cells of the C1 stencils
+// (ascending); the C1 stencils re-expressed as local C2 indices (<< 4, with
+// the periodic-shift code in the low bits), in slices of 64 C1 cells padded
+// with the cell itself; each listed face's cells as local indices.  The
+// stencil entries keep the order of the global stencil, so the gradients are
+// bit-identical to k_grad_lsqc's.  Leaves C.stage false when a block needs
+// more than 4095 local cells or the LDS layout would exceed 64 KiB.
+int setup_stage(fvs_ctx &C, const std::vector<int> &boff, const std::vector<int> &bfaces,
+                const std::vector<int2> &fic, const std::vector<int> &fbc, const std::vector<int> &off,
+                const std::vector<int> &wid, const std::vector<int> &nb, const std::vector<unsigned> &code) {
+  const int N = C.N, Fi = C.Fi, nbk = nblk(N);
+  std::vector<int> s2off(nbk + 1, 0), s2n1(nbk, 0), s2list, sl0(nbk + 1, 0), slw;
+  std::vector<long> sleo;
+  std::vector<unsigned short> slid;
+  std::vector<unsigned> blf(bfaces.size(), 0u);
+  std::vector<int> lmap(N, -1), c1, extra;
+  int max1 = 0, max2 = 0;
+  long eo = 0;
+  for (int b = 0; b < nbk; ++b) {
+    const int i0 = b * kBlock, i1 = std::min(N, i0 + kBlock);
+    c1.clear();
+    extra.clear();
+    for (int i = i0; i < i1; ++i) c1.push_back(i);
+    for (int f = boff[b]; f < boff[b + 1]; ++f) {
+      const int sl = bfaces[f];
+      if (sl >= Fi) continue;
+      const int a = fic[sl].x, c = fic[sl].y;
+      if (a < i0 || a >= i1) extra.push_back(a);
+      if (c < i0 || c >= i1) extra.push_back(c);
+    }
+    std::sort(extra.begin(), extra.end());
+    extra.erase(std::unique(extra.begin(), extra.end()), extra.end());
+    c1.insert(c1.end(), extra.begin(), extra.end());
+    const int n1 = int(c1.size());
+    for (int q = 0; q < n1; ++q) lmap[c1[q]] = q;
+    extra.clear();
+    for (int q = 0; q < n1; ++q) {
+      const int i = c1[q], sg = i / kSlice, l = i % kSlice;
+      for (int k = 0; k < wid[sg]; ++k) {
+        const int j = nb[off[sg] + long(k) * kSlice + l];
+        if (lmap[j] == -1) {
+          lmap[j] = -2;
+          extra.push_back(j);
+        }
+      }
+    }
+    std::sort(extra.begin(), extra.end());
+    for (size_t k = 0; k < extra.size(); ++k) lmap[extra[k]] = n1 + int(k);
+    const int n2 = n1 + int(extra.size());
+    bool fits = n2 <= 4095;
+    const int ns = (n1 + kSlice - 1) / kSlice;
+    for (int tq = 0; tq < ns && fits; ++tq) {
+      int w = 0;
+      for (int q = tq * kSlice; q < std::min(n1, (tq + 1) * kSlice); ++q) w = std::max(w, wid[c1[q] / kSlice]);
+      slw.push_back(w);
+      sleo.push_back(eo);
+      slid.resize(size_t(eo + long(w) * kSlice), 0);
+      for (int l = 0; l < kSlice; ++l) {
+        const int q = tq * kSlice + l;
+        if (q >= n1) continue;
+        const int i = c1[q], sg = i / kSlice, li = i % kSlice;
+        for (int k = 0; k < w; ++k) {
+          unsigned v = unsigned(q) << 4;  // padding: the cell itself, no shift
+          if (k < wid[sg]) {
+            const unsigned cd = code[off[sg] + long(k) * kSlice + li];
+            v = (unsigned(lmap[int(cd & 0x07ffffffu)]) << 4) | (cd >> 27);
+          }
+          slid[size_t(eo + long(k) * kSlice + l)] = static_cast<unsigned short>(v);
+        }
+      }
+      eo += long(w) * kSlice;
+    }
+    if (!fits) return 0;
+    sl0[b + 1] = sl0[b] + ns;
+    for (int f = boff[b]; f < boff[b + 1]; ++f) {
+      const int sl = bfaces[f];
+      if (sl < Fi) blf[f] = unsigned(lmap[fic[sl].x]) | (unsigned(lmap[fic[sl].y]) << 16);
+      else blf[f] = unsigned(lmap[fbc[sl - Fi]]) | (0xffffu << 16);
+    }
+    // every local index the kernel reads stays inside the block's arrays
+    for (int tq = 0; tq < ns; ++tq) {
+      const int si = sl0[b] + tq;
+      for (int l = 0; l < kSlice && tq * kSlice + l < n1; ++l)
+        for (int k = 0; k < slw[si]; ++k)
+          if (int(slid[size_t(sleo[si] + long(k) * kSlice + l)] >> 4) >= n2)
+            return fail(FVS_EINVAL, "stage set-up: stencil entry outside the block's cell list");
+    }
+    for (int f = boff[b]; f < boff[b + 1]; ++f) {
+      const int a = int(blf[f] & 0xffffu), c = int(blf[f] >> 16);
+      if (a >= n1 || (bfaces[f] < Fi ? c >= n1 : c != 0xffff))
+        return fail(FVS_EINVAL, "stage set-up: face cell outside the block's gradient cells");
+    }
+    for (int q = 0; q < i1 - i0; ++q)
+      if (c1[q] != i0 + q) return fail(FVS_EINVAL, "stage set-up: own cells not first");
+    s2list.insert(s2list.end(), c1.begin(), c1.end());
+    s2list.insert(s2list.end(), extra.begin(), extra.end());
+    s2off[b + 1] = int(s2list.size());
+    s2n1[b] = n1;
+    max1 = std::max(max1, n1);
+    max2 = std::max(max2, n2);
+    for (int c : c1) lmap[c] = -1;
+    for (int c : extra) lmap[c] = -1;
+  }
+  max1 += max1 & 1;  // keep the LDS arrays after lP 16-byte aligned
+  // the block reduction reuses the staged-state area: 4 x 512 doubles
+  max2 = std::max(max2, kStageThreads);
+  const size_t lds = (size_t(6) * max2 + size_t(9) * max1 + size_t(5) * C.maxf) * sizeof(double);
+  if (lds > 64 * 1024) return 0;
+  if (int rc = C.upload(C.s2off, s2off)) return rc;
+  if (int rc = C.upload(C.s2n1, s2n1)) return rc;
+  if (int rc = C.upload(C.s2list, s2list)) return rc;
+  if (int rc = C.upload(C.sl0, sl0)) return rc;
+  if (int rc = C.upload(C.slw, slw)) return rc;
+  if (int rc = C.upload(C.sleo, sleo)) return rc;
+  if (int rc = C.upload(C.slid, slid)) return rc;
+  if (int rc = C.upload(C.blf, blf)) return rc;
+  C.max1 = max1;
+  C.max2 = max2;
+  C.nslid = long(slid.size());
+  C.nlist2 = long(s2list.size());
+  C.stage = true;
+  return 0;
+}
+
 int setup_device(fvs_ctx &C) {
   MeshPrep &m = C.mp;
   const int N = C.N = m.ncells;
@@ -854,7 +993,8 @@ int setup_device(fvs_ctx &C) {
   // per-block face lists for the fused flux + update kernel: every face slot
   // touching block b's cells, ascending (interior slots, then boundary slots),
   // and each cell's faces as signed local indices in the gather order
-  if (C.opt.kernel_path == 0) {
+  std::vector<int> h_boff, h_bfaces;
+  if (C.opt.kernel_path == 0 || C.opt.kernel_path == 2) {
     const int nbk = nblk(N);
     std::vector<int> boff(nbk + 1, 0), bfaces, lgat(size_t(gw) * N, 0), loc;
     int maxf = 0;
@@ -880,13 +1020,17 @@ int setup_device(fvs_ctx &C) {
       boff[b + 1] = int(bfaces.size());
       maxf = std::max(maxf, int(loc.size()));
     }
-    if (size_t(maxf) * 5 * sizeof(double) + 4 * kBlock * sizeof(double) <= 64 * 1024) {
+    // LDS of k_flux_update: face fluxes (at least the 4 x 256 reduction) +
+    // the block's gradients
+    if (size_t(std::max(5 * maxf, 4 * kBlock)) * sizeof(double) + 8 * kBlock * sizeof(double) <= 64 * 1024) {
       C.maxf = maxf;
       C.nlisted = long(bfaces.size());
       if (int rc = C.upload(C.boff, boff)) return rc;
       if (int rc = C.upload(C.bfaces, bfaces)) return rc;
       if (int rc = C.upload(C.lgat, lgat)) return rc;
       C.fused = true;
+      h_boff.swap(boff);
+      h_bfaces.swap(bfaces);
     }
   }
   // limiter face offsets (xf - xc) per local edge, in the cell's own frame
@@ -918,7 +1062,7 @@ int setup_device(fvs_ctx &C) {
     if (int rc = C.upload(C.snb, nb)) return rc;
     if (int rc = C.upload(C.scw, c2)) return rc;
     // centroid form: neighbour id | shift code << 27, per-cell G^-1
-    if (C.opt.grad_method == 3 && C.opt.kernel_path == 0 && N < (1 << 27)) {
+    if (C.opt.grad_method == 3 && (C.opt.kernel_path == 0 || C.opt.kernel_path == 2) && N < (1 << 27)) {
       std::vector<double> dsx, dsy;  // distinct periodic shifts
       std::vector<unsigned> code(nb.size(), 0u);
       bool ok = true;
@@ -988,6 +1132,8 @@ int setup_device(fvs_ctx &C) {
         if (int rc = C.upload(C.ginv, gi)) return rc;
         if (int rc = C.upload(C.sid, code)) return rc;
         C.lsqc = true;
+        if (C.opt.kernel_path == 2 && C.fused && C.opt.recon != 1)
+          if (int rc = setup_stage(C, h_boff, h_bfaces, fic, fbc, off, wid, nb, code)) return rc;
       }
     }
   }
@@ -1198,13 +1344,45 @@ int launch_flux_update(fvs_ctx &C, const Upd &A, double t) {
   {
     ProfScope ps(C, KC_FUSED);
     const int nb = nblk(C.N);
-    const size_t lds = size_t(C.maxf) * 5 * sizeof(double);
+    const size_t lds = size_t(std::max(5 * C.maxf, 4 * kBlock)) * sizeof(double);
     hipLaunchKernelGGL(k_flux_update, dim3(C.nact ? C.nact : xcd_grid(nb)), dim3(kBlock), lds, C.stream, C.N,
                        xcd_arg(nb, C.xcd_fused), C.GW, C.maxf, A, C.P, t, C.boff, C.bfaces, C.lgat, C.Fi, C.fi_cells,
                        C.fi_geo, C.fb_cell, C.fb_type, C.fb_geo, C.W, C.grad, C.phi, C.vol, C.U0, C.Ub, C.Facc,
                        C.Walt, C.dtl, C.part, C.nact ? C.blist : nullptr, C.dbinst, C.dpinf_inst,
                        A.last ? C.d2buf : nullptr);
   }
+  HIPC(hipGetLastError());
+  std::swap(C.W, C.Walt);
+  return 0;
+}
+
+// whole stage in one kernel (k_stage); swaps the primitive buffers
+int launch_stage(fvs_ctx &C, const Upd &A, double t) {
+  ProfScope ps(C, KC_FUSED);
+  const int nb = nblk(C.N);
+  const size_t lds = (size_t(6) * C.max2 + size_t(9) * C.max1 + size_t(5) * C.maxf) * sizeof(double);
+  const bool lim = C.opt.limiter > 0 && !C.limiter_frozen;
+  const int phisrc = C.opt.limiter > 0 ? 1 : 0;
+  const dim3 g(C.nact ? C.nact : xcd_grid(nb)), b(kStageThreads);
+#define FVS_STAGE(L, PM)                                                                                           \
+  hipLaunchKernelGGL((k_stage<L, PM>), g, b, lds, C.stream, C.N, xcd_arg(nb, C.xcd_fused), C.GW, C.maxf, C.max1,  \
+                     C.max2, A, C.P, t, C.boff, C.bfaces, C.blf, C.lgat, C.Fi, C.fi_geo, C.fb_type, C.fb_geo,      \
+                     C.s2off, C.s2n1, C.s2list, C.sl0, C.slw, C.sleo, C.slid, C.cxy, C.ginv, C.shifts,           \
+                     C.opt.lsq_p, phisrc, C.phi, C.opt.limiter, C.nvrt, C.loff, C.vol,                           \
+                     C.frz_some ? C.dinst_off : nullptr, C.dfrz, C.ninst, C.W, C.U0, C.Ub, C.Facc, C.Walt,       \
+                     C.dtl, C.part, C.nact ? C.blist : nullptr, C.dbinst, C.dpinf_inst, A.last ? C.d2buf : nullptr)
+  if (lim) {
+    if (C.pmode == 0) FVS_STAGE(1, 0);
+    else if (C.pmode == 1) FVS_STAGE(1, 1);
+    else if (C.pmode == 2) FVS_STAGE(1, 2);
+    else FVS_STAGE(1, 3);
+  } else {
+    if (C.pmode == 0) FVS_STAGE(0, 0);
+    else if (C.pmode == 1) FVS_STAGE(0, 1);
+    else if (C.pmode == 2) FVS_STAGE(0, 2);
+    else FVS_STAGE(0, 3);
+  }
+#undef FVS_STAGE
   HIPC(hipGetLastError());
   std::swap(C.W, C.Walt);
   return 0;
@@ -1680,7 +1858,9 @@ int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {
       ts = tst[k];
     }
     int rc;
-    if (C.fused) {
+    if (C.stage) {
+      rc = launch_stage(C, A, ts);
+    } else if (C.fused) {
       rc = launch_flux_update(C, A, ts);
     } else {
       rc = launch_fluxes(C, ts);
@@ -1975,7 +2155,13 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   // the cell-side traffic of the update kernel without the face-flux buffer
   const double bfu = (Fi + Fb) * (8 + 56) + 4.0 * double(C->nlisted) + N * (32 + 64 + 8) +
                      N * (C->GW * 4.0 + 8 + 32 + state);
-  const double b5[KC_N] = {bg, bfi, bfb, bup, bno, bfu};
+  // whole stage: own states, centres and G^-1 once, the per-block cell lists
+  // and local stencils, face records and block lists once, limiter inputs,
+  // the update's cell-side traffic
+  const double bst = N * (32 + 16 + 32) + 2.0 * double(C->nslid) + 4.0 * double(C->nlist2) + (Fi + Fb) * 56 +
+                     8.0 * double(C->nlisted) + (lim ? N * (8 + 4) + 16.0 * (3 * m.ntri + 4 * m.nquad) : 0.0) +
+                     N * (C->GW * 4.0 + 8 + 32 + state);
+  const double b5[KC_N] = {bg, bfi, bfb, bup, bno, C->stage ? bst : bfu};
   for (int k = 0; k < KC_N; ++k) {
     ms[k] = cnt[k] ? tot[k] / cnt[k] : 0.0;
     bytes[k] = b5[k];

@@ -56,7 +56,10 @@ typedef struct {
   double dt, cfl;
   int reorder;          /* device cell order: 0 reference order, 1 Morton blocks */
   int kernel_path;      /* 0: fastest available (fused LDS flux+update, centroid LSQ);
-                           1: face-flux buffer kernels only */
+                           1: face-flux buffer kernels only;
+                           2: whole stage in one kernel (gradients recomputed per
+                              block from LDS; LSQ, centroid form, triangle-sized
+                              blocks; falls back to 0 where it does not apply) */
 } fvs_options;
 
 const char *fvs_last_error(void);

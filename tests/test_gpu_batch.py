@@ -36,9 +36,9 @@ def single(k, steps, freeze_after=None, inst=INST):
     return s
 
 
-def batch(inst=INST):
+def batch(inst=INST, kernel_path=0):
     meshes = [cases.naca(n, l, cfl=1.0)[0] for (n, l), _ in inst]
-    _, sch = cases.naca(16, 8, cfl=1.0)
+    _, sch = cases.naca(16, 8, cfl=1.0, kernel_path=kernel_path)
     b = BatchSolver(meshes, [ma for _, ma in inst], sch)
     b.initialize_freestream()
     return b
@@ -109,13 +109,14 @@ def test_steady_solve_batch_converges():
         assert r["iterations"] == res["iterations"][k], (k, r["iterations"], res["iterations"])
 
 
-def test_per_instance_limiter_freeze():
+@pytest.mark.parametrize("kernel_path", [0, 2])
+def test_per_instance_limiter_freeze(kernel_path):
     """Freezing the limiter of one instance (fvs_batch_freeze_limiter) leaves
     the others re-evaluating theirs: instance 1 frozen after 30 steps, 0 and 2
     never, each bit-identical to its own single solve.  Instance 1's cells
     share launch blocks with instances 0 and 2 (the block boundaries do not
     fall on instance boundaries), so the per-cell instance lookup is used."""
-    b = batch(INST_RAGGED)
+    b = batch(INST_RAGGED, kernel_path)
     o = b.cell_offsets
     assert o[1] % 256 != 0 and o[2] % 256 != 0
     b.batch_time_integration(0.0, 30)

@@ -92,7 +92,8 @@ def test_residual_reconstruction(recon, kappa):
     assert rel_err(ws, orc.ws) <= 1e-13
 
 
-PATHS = [dict(kernel_path=0, reorder=0), dict(kernel_path=0, reorder=1), dict(kernel_path=1, reorder=0)]
+PATHS = [dict(kernel_path=0, reorder=0), dict(kernel_path=0, reorder=1), dict(kernel_path=1, reorder=0),
+         dict(kernel_path=2, reorder=1)]
 
 
 @pytest.mark.parametrize("kind", ["dirichlet", "periodic"])
@@ -299,3 +300,51 @@ def test_batch_wall_cp_matches_single():
             assert np.allclose(wb[key][k0:k0 + nw], ws[key], rtol=1e-12, atol=1e-14), (k, key)
         k0 += nw
     assert k0 == wb["x"].size
+
+
+STAGE_CASES = {
+    "vortex_periodic_lsqnn": dict(kind="periodic", sch=dict()),
+    "vortex_dirichlet_venk": dict(kind="dirichlet", sch=dict(limiter=1)),
+    "vortex_periodic_barth_p1": dict(kind="periodic", sch=dict(limiter=2, lsq_p=1.0)),
+    "vortex_dirichlet_fn_p2": dict(kind="dirichlet", sch=dict(lsq_nn=0, lsq_p=2.0)),
+    "vortex_periodic_rk4_umuscl": dict(kind="periodic", sch=dict(rk_scheme=0, rk_order=4, recon=3, kappa=1.0 / 3.0,
+                                                                  limiter=1)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(STAGE_CASES))
+def test_stage_kernel_bitwise(case):
+    """The whole-stage kernel (kernel_path 2: gradients recomputed per block
+    from LDS) gives bit-identical states and norms to the fused flux kernel
+    with its separate gradient kernel (kernel_path 0), which the trajectory
+    tests hold to the oracle."""
+    c = STAGE_CASES[case]
+    m = mesh(c["kind"], 48)
+    sch = vortex_scheme(dt=0.004, **c["sch"])
+    out = []
+    for path in (0, 2):
+        s = Solver(m, sch, kernel_path=path, reorder=1)
+        s.initialize_solution(1, 0.0)
+        n = s.time_integration(0.0, 30)
+        out.append((s.cvar, n))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_stage_kernel_steady_naca_freeze():
+    """Steady SSP-RK(4,2), Venkatakrishnan, local time stepping on a NACA
+    O-grid (slip wall + free stream), limiter frozen half-way: kernel_path 2
+    bit-identical to kernel_path 0, and the limiter values it writes match."""
+    from fvs2d_amd import cases
+    m, sch = cases.naca(64, 32, cfl=1.0, limiter=1)
+    out = []
+    for path in (0, 2):
+        s = Solver(m, sch, kernel_path=path)
+        s.initialize_solution(1, 0.0)
+        n1 = s.time_integration(0.0, 40)
+        phi = s.compute_gradient_limiter()
+        s.freeze_limiter(True)
+        n2 = s.time_integration(0.0, 40)
+        out.append((s.cvar, n1, n2, phi))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
