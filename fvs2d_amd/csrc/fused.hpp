@@ -393,6 +393,122 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
 // indices blf[f] = c1 | c2 << 16 (c2 = 0xffff on a boundary face).
 // phisrc: 0 -> limiter values 1 (no limiter), 1 -> read phi (frozen); with
 // LIM the limiter is evaluated and written for the block's own cells.
+// Gradient of local cell q (global id) from states and centres staged in
+// LDS, with its stencil as local codes (index << 4 | periodic shift) in a
+// 64-cell slice starting at sl (width wid).  Same arithmetic and order as
+// k_grad_lsqc.  Returns the limiter value: evaluated with LIM (unless the
+// cell's instance is frozen; written to phi when own), else phi[id] when
+// phisrc is 1, else 1.
+template <int LIM, int PMODE>
+__device__ __forceinline__ double lds_gradient(int q, int id, int N, const double *__restrict__ lW,
+                                               const double2 *__restrict__ lX, const unsigned short *__restrict__ sl,
+                                               int wid, const ShiftTable &st, double p,
+                                               const double *__restrict__ ginv, double g8[8], int phisrc,
+                                               double *__restrict__ phi, int lt, const int *__restrict__ nvrt,
+                                               const double2 *__restrict__ loff, const double *__restrict__ vol,
+                                               const int *__restrict__ ioff, const int *__restrict__ frz, int ninst,
+                                               bool own) {
+  const int l = q % kSlice;
+  const double2 ci = lX[q];
+  double w[4], sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, mn[4], mx[4];
+  ld4(lW, q, w);
+  for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
+  for (int k = 0; k < wid; ++k) {
+    const unsigned code = sl[long(k) * kSlice + l];
+    const int j = int(code >> 4), sh = int(code & 15u);
+    const double2 cj = lX[j];
+    double q4[4];
+    ld4(lW, j, q4);
+    double xj = cj.x, yj = cj.y;
+    if (sh) {
+      xj = xj + st.sx[sh - 1];
+      yj = yj + st.sy[sh - 1];
+    }
+    const double dx = xj - ci.x, dy = yj - ci.y;
+    double w2 = 1.0;
+    if (PMODE != 0) {
+      const double r2 = dx * dx + dy * dy;
+      if (PMODE == 1) w2 = (r2 > 0.0) ? 1.0 / r2 : 0.0;
+      else if (PMODE == 2) w2 = (r2 > 0.0) ? 1.0 / (r2 * r2) : 0.0;
+      else w2 = (r2 > 0.0) ? pow(r2, -p) : 0.0;
+    }
+    for (int v = 0; v < 4; ++v) {
+      const double tt = w2 * (q4[v] - w[v]);
+      sx[v] = sx[v] + tt * dx;
+      sy[v] = sy[v] + tt * dy;
+      if (LIM) {
+        mn[v] = (q4[v] < mn[v]) ? q4[v] : mn[v];
+        mx[v] = (q4[v] > mx[v]) ? q4[v] : mx[v];
+      }
+    }
+  }
+  const double2 g01 = reinterpret_cast<const double2 *>(ginv)[2 * id];
+  const double2 g2 = reinterpret_cast<const double2 *>(ginv)[2 * id + 1];
+  double gx[4], gy[4];
+  for (int v = 0; v < 4; ++v) {
+    gx[v] = g01.x * sx[v] + g01.y * sy[v];
+    gy[v] = g01.y * sx[v] + g2.x * sy[v];
+    g8[v] = gx[v];
+    g8[4 + v] = gy[v];
+  }
+  bool keep = !LIM;
+  if (LIM && ioff) {
+    int lo = 0, hi = ninst;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (ioff[mid] <= id) lo = mid;
+      else hi = mid;
+    }
+    keep = frz[lo] != 0;
+  }
+  if (!keep) {
+    const double ph = cell_limiter(lt, nvrt[id], N, id, loff, vol[id], w, mn, mx, gx, gy);
+    if (own) phi[id] = ph;
+    return ph;
+  }
+  return phisrc == 1 ? phi[id] : 1.0;
+}
+
+// LSQ gradient (and limiter) of the block's own cells with the block's whole
+// stencil halo staged in LDS first (the per-block lists of k_stage): one bulk
+// gather per block instead of one gather per stencil entry.  Bit-identical to
+// k_grad_lsqc.
+template <int LIM, int PMODE>
+__global__ __launch_bounds__(kBlock) void k_grad_lsqh(int N, int nb, int max2, const double *__restrict__ W,
+                                                      const int *__restrict__ s2off, const int *__restrict__ s2list,
+                                                      const int *__restrict__ sl0, const int *__restrict__ slw,
+                                                      const long *__restrict__ sleo,
+                                                      const unsigned short *__restrict__ slid,
+                                                      const double2 *__restrict__ cxy,
+                                                      const double *__restrict__ ginv, ShiftTable st, double p,
+                                                      double *__restrict__ grad, double *__restrict__ phi, int lt,
+                                                      const int *__restrict__ nvrt, const double2 *__restrict__ loff,
+                                                      const double *__restrict__ vol, const int *__restrict__ blist,
+                                                      const int *__restrict__ ioff, const int *__restrict__ frz,
+                                                      int ninst) {
+  extern __shared__ double lds[];
+  double *lW = lds;                                            // [max2][4]
+  double2 *lX = reinterpret_cast<double2 *>(lds + 4 * max2);   // [max2]
+  const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
+  if (lb >= (nb < 0 ? -nb : nb)) return;  // the whole workgroup leaves before the barrier
+  const int o2 = s2off[lb], n2 = s2off[lb + 1] - o2;
+  for (int q = threadIdx.x; q < n2; q += kBlock) {
+    const int id = s2list[o2 + q];
+    double w[4];
+    ld4(W, id, w);
+    st4(lW, q, w);
+    lX[q] = cxy[id];
+  }
+  __syncthreads();
+  const int q = threadIdx.x, id = lb * kBlock + q;
+  if (id >= N) return;
+  const int si = sl0[lb] + q / kSlice;
+  double g8[8];
+  lds_gradient<LIM, PMODE>(q, id, N, lW, lX, slid + sleo[si], slw[si], st, p, ginv, g8, 0, phi, lt, nvrt, loff, vol,
+                           ioff, frz, ninst, true);
+  st8(grad, id, g8);
+}
+
 constexpr int kStageThreads = 2 * kBlock;  // two threads per cell: each phase fits one round
 
 template <int LIM, int PMODE>
@@ -433,67 +549,11 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(
   // gradients (and limiter values) of C1 from LDS
   for (int q = threadIdx.x; q < n1; q += kStageThreads) {
     const int id = s2list[o2 + q];
-    const int si = sl0[lb] + q / kSlice, l = q % kSlice, wid = slw[si];
-    const long base = sleo[si];
-    const double2 ci = lX[q];
-    double w[4], sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, mn[4], mx[4];
-    ld4(lW, q, w);
-    for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
-    for (int k = 0; k < wid; ++k) {
-      const unsigned code = slid[base + long(k) * kSlice + l];
-      const int j = int(code >> 4), sh = int(code & 15u);
-      const double2 cj = lX[j];
-      double q4[4];
-      ld4(lW, j, q4);
-      double xj = cj.x, yj = cj.y;
-      if (sh) {
-        xj = xj + st.sx[sh - 1];
-        yj = yj + st.sy[sh - 1];
-      }
-      const double dx = xj - ci.x, dy = yj - ci.y;
-      double w2 = 1.0;
-      if (PMODE != 0) {
-        const double r2 = dx * dx + dy * dy;
-        if (PMODE == 1) w2 = (r2 > 0.0) ? 1.0 / r2 : 0.0;
-        else if (PMODE == 2) w2 = (r2 > 0.0) ? 1.0 / (r2 * r2) : 0.0;
-        else w2 = (r2 > 0.0) ? pow(r2, -p) : 0.0;
-      }
-      for (int v = 0; v < 4; ++v) {
-        const double tt = w2 * (q4[v] - w[v]);
-        sx[v] = sx[v] + tt * dx;
-        sy[v] = sy[v] + tt * dy;
-        if (LIM) {
-          mn[v] = (q4[v] < mn[v]) ? q4[v] : mn[v];
-          mx[v] = (q4[v] > mx[v]) ? q4[v] : mx[v];
-        }
-      }
-    }
-    const double2 g01 = reinterpret_cast<const double2 *>(ginv)[2 * id];
-    const double2 g2 = reinterpret_cast<const double2 *>(ginv)[2 * id + 1];
-    double gx[4], gy[4];
-    for (int v = 0; v < 4; ++v) {
-      gx[v] = g01.x * sx[v] + g01.y * sy[v];
-      gy[v] = g01.y * sx[v] + g2.x * sy[v];
-    }
-    double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
+    const int si = sl0[lb] + q / kSlice;
+    double g8[8];
+    const double ph = lds_gradient<LIM, PMODE>(q, id, N, lW, lX, slid + sleo[si], slw[si], st, p, ginv, g8, phisrc,
+                                               phi, lt, nvrt, loff, vol, ioff, frz, ninst, q < nown);
     st8(lG, q, g8);
-    double ph = 1.0;
-    bool keep = !LIM;
-    if (LIM && ioff) {
-      int lo = 0, hi = ninst;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (ioff[mid] <= id) lo = mid;
-        else hi = mid;
-      }
-      keep = frz[lo] != 0;
-    }
-    if (!keep) {
-      ph = cell_limiter(lt, nvrt[id], N, id, loff, vol[id], w, mn, mx, gx, gy);
-      if (q < nown) phi[id] = ph;
-    } else if (phisrc == 1) {
-      ph = phi[id];
-    }
     lP[q] = ph;
   }
   __syncthreads();

@@ -668,13 +668,14 @@ struct fvs_ctx {
   // limiter values frozen at their last evaluation (steady convergence aid)
   bool limiter_frozen = false;
   // whole-stage kernel (k_stage): per-block C2 lists, local stencils and face cells
-  bool stage = false;
-  int max1 = 0, max2 = 0;
+  bool stage = false, halo = false;  // k_stage in use; per-block data built (k_grad_lsqh)
+  bool use_halo = true;              // gradient from the staged halo (autotuned)
+  int max1 = 0, max2 = 0, max2n = 0;
   int *s2off = nullptr, *s2n1 = nullptr, *s2list = nullptr, *sl0 = nullptr, *slw = nullptr;
   long *sleo = nullptr;
   unsigned short *slid = nullptr;
   unsigned *blf = nullptr;
-  long nslid = 0, nlist2 = 0;
+  long nslid = 0, nlist2 = 0, nslid_own = 0;
   // batch mode: per-instance frozen flags (dfrz) while only some are frozen
   int *dfrz = nullptr;
   bool frz_some = false;
@@ -795,8 +796,10 @@ void pack_sell(int N, const std::vector<int> &perm, const std::vector<int> &iper
 // the periodic-shift code in the low bits), in slices of 64 C1 cells padded
 // with the cell itself; each listed face's cells as local indices.  The
 // stencil entries keep the order of the global stencil, so the gradients are
-// bit-identical to k_grad_lsqc's.  Leaves C.stage false when a block needs
-// more than 4095 local cells or the LDS layout would exceed 64 KiB.
+// bit-identical to k_grad_lsqc's.  C.halo (k_grad_lsqh, the default
+// gradient) is set when every block fits 4095 local cells and its staged
+// halo 64 KiB; C.stage (k_stage) with kernel_path 2 when its whole LDS
+// layout fits 64 KiB too.
 int setup_stage(fvs_ctx &C, const std::vector<int> &boff, const std::vector<int> &bfaces,
                 const std::vector<int2> &fic, const std::vector<int> &fbc, const std::vector<int> &off,
                 const std::vector<int> &wid, const std::vector<int> &nb, const std::vector<unsigned> &code) {
@@ -807,7 +810,7 @@ int setup_stage(fvs_ctx &C, const std::vector<int> &boff, const std::vector<int>
   std::vector<unsigned> blf(bfaces.size(), 0u);
   std::vector<int> lmap(N, -1), c1, extra;
   int max1 = 0, max2 = 0;
-  long eo = 0;
+  long eo = 0, eo_own = 0;
   for (int b = 0; b < nbk; ++b) {
     const int i0 = b * kBlock, i1 = std::min(N, i0 + kBlock);
     c1.clear();
@@ -846,6 +849,7 @@ int setup_stage(fvs_ctx &C, const std::vector<int> &boff, const std::vector<int>
       for (int q = tq * kSlice; q < std::min(n1, (tq + 1) * kSlice); ++q) w = std::max(w, wid[c1[q] / kSlice]);
       slw.push_back(w);
       sleo.push_back(eo);
+      if (tq * kSlice < i1 - i0) eo_own += long(w) * kSlice;
       slid.resize(size_t(eo + long(w) * kSlice), 0);
       for (int l = 0; l < kSlice; ++l) {
         const int q = tq * kSlice + l;
@@ -894,10 +898,11 @@ int setup_stage(fvs_ctx &C, const std::vector<int> &boff, const std::vector<int>
     for (int c : extra) lmap[c] = -1;
   }
   max1 += max1 & 1;  // keep the LDS arrays after lP 16-byte aligned
-  // the block reduction reuses the staged-state area: 4 x 512 doubles
+  const int max2n = max2;
+  if (size_t(6) * max2n * sizeof(double) > 64 * 1024) return 0;
+  // k_stage's block reduction reuses the staged-state area: 4 x 512 doubles
   max2 = std::max(max2, kStageThreads);
   const size_t lds = (size_t(6) * max2 + size_t(9) * max1 + size_t(5) * C.maxf) * sizeof(double);
-  if (lds > 64 * 1024) return 0;
   if (int rc = C.upload(C.s2off, s2off)) return rc;
   if (int rc = C.upload(C.s2n1, s2n1)) return rc;
   if (int rc = C.upload(C.s2list, s2list)) return rc;
@@ -908,9 +913,12 @@ int setup_stage(fvs_ctx &C, const std::vector<int> &boff, const std::vector<int>
   if (int rc = C.upload(C.blf, blf)) return rc;
   C.max1 = max1;
   C.max2 = max2;
+  C.max2n = max2n;
   C.nslid = long(slid.size());
   C.nlist2 = long(s2list.size());
-  C.stage = true;
+  C.nslid_own = eo_own;
+  C.halo = true;
+  C.stage = C.opt.kernel_path == 2 && lds <= 64 * 1024;
   return 0;
 }
 
@@ -994,7 +1002,7 @@ int setup_device(fvs_ctx &C) {
   // touching block b's cells, ascending (interior slots, then boundary slots),
   // and each cell's faces as signed local indices in the gather order
   std::vector<int> h_boff, h_bfaces;
-  if (C.opt.kernel_path == 0 || C.opt.kernel_path == 2) {
+  if (C.opt.kernel_path == 0 || C.opt.kernel_path == 2 || C.opt.kernel_path == 3) {
     const int nbk = nblk(N);
     std::vector<int> boff(nbk + 1, 0), bfaces, lgat(size_t(gw) * N, 0), loc;
     int maxf = 0;
@@ -1062,7 +1070,7 @@ int setup_device(fvs_ctx &C) {
     if (int rc = C.upload(C.snb, nb)) return rc;
     if (int rc = C.upload(C.scw, c2)) return rc;
     // centroid form: neighbour id | shift code << 27, per-cell G^-1
-    if (C.opt.grad_method == 3 && (C.opt.kernel_path == 0 || C.opt.kernel_path == 2) && N < (1 << 27)) {
+    if (C.opt.grad_method == 3 && C.opt.kernel_path != 1 && N < (1 << 27)) {
       std::vector<double> dsx, dsy;  // distinct periodic shifts
       std::vector<unsigned> code(nb.size(), 0u);
       bool ok = true;
@@ -1132,7 +1140,7 @@ int setup_device(fvs_ctx &C) {
         if (int rc = C.upload(C.ginv, gi)) return rc;
         if (int rc = C.upload(C.sid, code)) return rc;
         C.lsqc = true;
-        if (C.opt.kernel_path == 2 && C.fused && C.opt.recon != 1)
+        if (C.fused && C.opt.recon != 1 && C.opt.kernel_path != 3)
           if (int rc = setup_stage(C, h_boff, h_bfaces, fic, fbc, off, wid, nb, code)) return rc;
       }
     }
@@ -1264,7 +1272,28 @@ int launch_gradient(fvs_ctx &C, bool with_limiter) {
   const int lt = C.opt.limiter;
   dim3 g(xcd_grid(nblk(N))), b(kBlock);
   ProfScope ps(C, KC_GRAD);
-  if (C.lsqc) {
+  if (C.halo && C.use_halo) {
+    if (C.nact) g = dim3(C.nact);
+    const int nb = nblk(N);
+    const size_t lds = size_t(6) * C.max2n * sizeof(double);
+#define FVS_LSQH(L, PM)                                                                                          \
+  hipLaunchKernelGGL((k_grad_lsqh<L, PM>), g, b, lds, C.stream, N, xcd_arg(nb, C.xcd_grad), C.max2n, C.W, C.s2off, \
+                     C.s2list, C.sl0, C.slw, C.sleo, C.slid, C.cxy, C.ginv, C.shifts, C.opt.lsq_p, C.grad, C.phi, lt,  \
+                     C.nvrt, C.loff, C.vol, C.nact ? C.blist : nullptr, C.frz_some ? C.dinst_off : nullptr, C.dfrz,    \
+                     C.ninst)
+    if (lim) {
+      if (C.pmode == 0) FVS_LSQH(1, 0);
+      else if (C.pmode == 1) FVS_LSQH(1, 1);
+      else if (C.pmode == 2) FVS_LSQH(1, 2);
+      else FVS_LSQH(1, 3);
+    } else {
+      if (C.pmode == 0) FVS_LSQH(0, 0);
+      else if (C.pmode == 1) FVS_LSQH(0, 1);
+      else if (C.pmode == 2) FVS_LSQH(0, 2);
+      else FVS_LSQH(0, 3);
+    }
+#undef FVS_LSQH
+  } else if (C.lsqc) {
     if (C.nact) g = dim3(C.nact);
     const int nb = nblk(N);
 #define FVS_LSQC(L, PM)                                                                                          \
@@ -2131,7 +2160,8 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   const double N = C->N, Fi = C->Fi, Fb = C->Fb;
   const bool lim = C->opt.limiter > 0 && C->opt.recon != 1;
   double bg = 0.0;
-  if (C->lsqc) bg = N * (32 + 16 + 32 + 64) + 4.0 * m.lsq_ptr.back();
+  if (C->halo && C->use_halo) bg = N * (32 + 16 + 32 + 64) + 4.0 * double(C->nlist2) + 2.0 * double(C->nslid_own);
+  else if (C->lsqc) bg = N * (32 + 16 + 32 + 64) + 4.0 * m.lsq_ptr.back();
   else if (C->opt.grad_method == 3) bg = N * (32 + 64) + 20.0 * m.lsq_ptr.back();
   else if (C->opt.grad_method == 1) bg = N * (32 + 16 + 8 + 64) + N * C->NFW * 20.0;
   else bg = N * (32 + 16 + 8 + 64) + 20.0 * m.ggnb_ptr.back();
@@ -2214,12 +2244,13 @@ int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) {
     ms = C->last_ms / nsteps;
     return 0;
   };
-  bool *knobs[4] = {&C->xcd_grad, &C->xcd_flux, &C->xcd_upd, &C->xcd_fused};
-  const bool used[4] = {C->opt.recon != 1, !C->fused, !C->fused, C->fused};
+  bool *knobs[5] = {&C->xcd_grad, &C->xcd_flux, &C->xcd_upd, &C->xcd_fused, &C->use_halo};
+  const bool used[5] = {C->opt.recon != 1, !C->fused, !C->fused, C->fused && !C->stage,
+                        C->halo && !C->stage && C->opt.recon != 1};
   double best = 0.0;
   if (!rc) rc = trial(best);  // warm-up
   if (!rc) rc = trial(best);
-  for (int k = 0; k < 4 && !rc; ++k) {
+  for (int k = 0; k < 5 && !rc; ++k) {
     if (!used[k]) continue;
     *knobs[k] = !*knobs[k];
     double ms = 0.0;
@@ -2233,8 +2264,10 @@ int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) {
   if (hipStreamSynchronize(C->stream) != hipSuccess && !rc) rc = fail(FVS_EHIP, "autotune: sync failed");
   (void)hipFree(save);
   if (rc) return rc;
-  if (choice4)
+  if (choice4) {
     for (int k = 0; k < 4; ++k) choice4[k] = *knobs[k] ? 1 : 0;
+    if (C->halo && C->use_halo) choice4[0] |= 2;
+  }
   if (best_ms) *best_ms = best;
   return 0;
 }

@@ -60,7 +60,7 @@ class Scheme:
     dt: float = 0.01
     cfl: float = 1.0
     reorder: int = 0          # device cell order: 0 reference, 1 Morton
-    kernel_path: int = 0      # 0 fastest available, 1 face-flux buffer kernels
+    kernel_path: int = 0      # 0 fastest available, 1 face-flux buffer, 2 whole stage, 3 no block halo
 
     def to_c(self) -> _lib.Options:
         o = _lib.Options()
@@ -408,8 +408,10 @@ class Solver:
         return {k: (float(ms[i]), float(by[i]), int(n[i])) for i, k in enumerate(self.KERNEL_CLASSES)}
 
     def autotune(self, nsteps: int = 10):
-        """Pick plain or XCD-aware workgroup numbering per kernel class; the
-        state is left as it was.  Returns (choices, best ms per step)."""
+        """Pick plain or XCD-aware workgroup numbering per kernel class (bit
+        0) and, for the LSQ gradient, whether it reads its stencil from a
+        per-block halo staged in LDS (bit 1); the state is left as it was.
+        Returns (choices, best ms per step)."""
         ch = np.zeros(4, np.int32)
         best = np.zeros(1)
         check(self._L.fvs_autotune(self._h, nsteps, ptr(ch), ptr(best)))

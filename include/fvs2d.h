@@ -59,7 +59,9 @@ typedef struct {
                            1: face-flux buffer kernels only;
                            2: whole stage in one kernel (gradients recomputed per
                               block from LDS; LSQ, centroid form, triangle-sized
-                              blocks; falls back to 0 where it does not apply) */
+                              blocks; falls back to 0 where it does not apply);
+                           3: as 0 without the per-block halo data (the LSQ
+                              gradient gathers its stencil cell by cell) */
 } fvs_options;
 
 const char *fvs_last_error(void);
@@ -200,7 +202,9 @@ int fvs_batch_freeze_limiter(fvs_ctx *ctx, const int *frozen);
  * and keeps, per kernel class [gradient, face flux, gather+update, fused
  * flux+update], the faster of plain and XCD-aware workgroup numbering.  The
  * state is restored afterwards.  choice4 (may be NULL) receives the choices
- * (1 = XCD-aware), best_ms (may be NULL) the best time per step. */
+ * (1 = XCD-aware; for the gradient, bit 1 set = the gradient reads its stencil
+ * from a per-block halo staged in LDS, the other candidate it tries), best_ms
+ * (may be NULL) the best time per step. */
 int fvs_autotune(fvs_ctx *ctx, int nsteps, int *choice4, double *best_ms);
 /* wall-clock of the last fvs_time_integration call's device work (ms) */
 double fvs_last_elapsed_ms(const fvs_ctx *ctx);

@@ -93,7 +93,7 @@ def test_residual_reconstruction(recon, kappa):
 
 
 PATHS = [dict(kernel_path=0, reorder=0), dict(kernel_path=0, reorder=1), dict(kernel_path=1, reorder=0),
-         dict(kernel_path=2, reorder=1)]
+         dict(kernel_path=2, reorder=1), dict(kernel_path=3, reorder=1)]
 
 
 @pytest.mark.parametrize("kind", ["dirichlet", "periodic"])
@@ -314,21 +314,23 @@ STAGE_CASES = {
 
 @pytest.mark.parametrize("case", sorted(STAGE_CASES))
 def test_stage_kernel_bitwise(case):
-    """The whole-stage kernel (kernel_path 2: gradients recomputed per block
-    from LDS) gives bit-identical states and norms to the fused flux kernel
-    with its separate gradient kernel (kernel_path 0), which the trajectory
-    tests hold to the oracle."""
+    """The three LSQ kernel arrangements give bit-identical states and norms:
+    the fused flux kernel with the gradient from a per-block halo staged in
+    LDS (kernel_path 0), the whole stage in one kernel (2), and the fused
+    flux kernel with the cell-by-cell stencil gather (3).  The trajectory
+    tests hold all of them to the oracle."""
     c = STAGE_CASES[case]
     m = mesh(c["kind"], 48)
     sch = vortex_scheme(dt=0.004, **c["sch"])
     out = []
-    for path in (0, 2):
+    for path in (0, 2, 3):
         s = Solver(m, sch, kernel_path=path, reorder=1)
         s.initialize_solution(1, 0.0)
         n = s.time_integration(0.0, 30)
         out.append((s.cvar, n))
-    assert np.array_equal(out[0][0], out[1][0])
-    assert np.array_equal(out[0][1], out[1][1])
+    for k in (1, 2):
+        assert np.array_equal(out[0][0], out[k][0]), k
+        assert np.array_equal(out[0][1], out[k][1]), k
 
 
 def test_stage_kernel_steady_naca_freeze():
@@ -338,7 +340,7 @@ def test_stage_kernel_steady_naca_freeze():
     from fvs2d_amd import cases
     m, sch = cases.naca(64, 32, cfl=1.0, limiter=1)
     out = []
-    for path in (0, 2):
+    for path in (0, 2, 3):
         s = Solver(m, sch, kernel_path=path)
         s.initialize_solution(1, 0.0)
         n1 = s.time_integration(0.0, 40)
@@ -346,5 +348,6 @@ def test_stage_kernel_steady_naca_freeze():
         s.freeze_limiter(True)
         n2 = s.time_integration(0.0, 40)
         out.append((s.cvar, n1, n2, phi))
-    for a, b in zip(out[0], out[1]):
-        assert np.array_equal(a, b)
+    for k in (1, 2):
+        for a, b in zip(out[0], out[k]):
+            assert np.array_equal(a, b)
