@@ -256,6 +256,76 @@ __global__ __launch_bounds__(256) void k_gemm_nt(int K, const u16 *__restrict__ 
   }
 }
 
+// Large-tile form of k_gemm_nt for M a multiple of 256: 16 wavefronts own
+// 256 rows x 256 columns (wave w: rows 64(w/4).., columns 64(w%4)..), so a
+// workgroup reads each weight slab once per 256 rows instead of per 64
+// (weights are a quarter of the L2 traffic), and the slabs are double
+// buffered in LDS: one barrier per slab.  Same arithmetic as k_gemm_nt.
+constexpr int kBigRows = 256, kBigThreads = 1024;
+
+template <int EPI>
+__global__ __launch_bounds__(kBigThreads) void k_gemm_nt_big(int K, const u16 *__restrict__ A,
+                                                             const u16 *__restrict__ B, const float *__restrict__ bias,
+                                                             const u16 *__restrict__ Zp, u16 *__restrict__ O1,
+                                                             u16 *__restrict__ O2) {
+  __shared__ __attribute__((aligned(16))) u16 As[2][kBigRows * kLds];
+  __shared__ __attribute__((aligned(16))) u16 Bs[2][kW * kLds];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, wm = wv >> 2, wn = wv & 3;
+  const long r0 = long(blockIdx.x) * kBigRows;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // one 16-byte chunk of A and one of B per thread per slab (256 rows x 32)
+  const int row = tid >> 2, c8 = (tid & 3) * 8;
+  const u16 *Ap = A + (r0 + row) * K + c8;
+  const u16 *Bp = B + long(row) * K + c8;
+  uint4 ra = *reinterpret_cast<const uint4 *>(Ap);
+  uint4 rb = *reinterpret_cast<const uint4 *>(Bp);
+  *reinterpret_cast<uint4 *>(&As[0][row * kLds + c8]) = ra;
+  *reinterpret_cast<uint4 *>(&Bs[0][row * kLds + c8]) = rb;
+  __syncthreads();
+  const int ns = K / kBK;
+  for (int st = 0; st < ns; ++st) {
+    const int cur = st & 1;
+    const bool more = st + 1 < ns;
+    if (more) {
+      ra = *reinterpret_cast<const uint4 *>(Ap + (st + 1) * kBK);
+      rb = *reinterpret_cast<const uint4 *>(Bp + (st + 1) * kBK);
+    }
+    mma_slab(&As[cur][(64 * wm) * kLds], &Bs[cur][0], lane, wn, acc);
+    if (more) {
+      *reinterpret_cast<uint4 *>(&As[cur ^ 1][row * kLds + c8]) = ra;
+      *reinterpret_cast<uint4 *>(&Bs[cur ^ 1][row * kLds + c8]) = rb;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const long rr = r0 + 64 * wm + 16 * i + (lane & 15);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n0 = 64 * wn + 16 * j + 4 * (lane >> 4);
+      const long idx = rr * kW + n0;
+      if (EPI == 0) {
+        const float4 bv = *reinterpret_cast<const float4 *>(bias + n0);
+        const float z0 = acc[i][j][0] + bv.x, z1 = acc[i][j][1] + bv.y, z2 = acc[i][j][2] + bv.z,
+                    z3 = acc[i][j][3] + bv.w;
+        *reinterpret_cast<uint2 *>(O1 + idx) = make_uint2(pack2(z0, z1), pack2(z2, z3));
+        *reinterpret_cast<uint2 *>(O2 + idx) = make_uint2(pack2(gelu(z0), gelu(z1)), pack2(gelu(z2), gelu(z3)));
+      } else {
+        const uint2 zz = *reinterpret_cast<const uint2 *>(Zp + idx);
+        const float z0 = bf2f(u16(zz.x & 0xffffu)), z1 = bf2f(u16(zz.x >> 16)), z2 = bf2f(u16(zz.y & 0xffffu)),
+                    z3 = bf2f(u16(zz.y >> 16));
+        *reinterpret_cast<uint2 *>(O1 + idx) =
+            make_uint2(pack2(acc[i][j][0] * gelu_grad(z0), acc[i][j][1] * gelu_grad(z1)),
+                       pack2(acc[i][j][2] * gelu_grad(z2), acc[i][j][3] * gelu_grad(z3)));
+      }
+    }
+  }
+}
+
 // move rows (2mp, 2mp+1) x 8 columns, held as v0/v1, into a transposed LDS
 // tile T[col][m] as 32-bit pairs
 __device__ __forceinline__ void put_pairs(u16 *__restrict__ T, int col0, int mp, uint4 v0, uint4 v1) {
@@ -595,8 +665,12 @@ int run_pass(fvs_mlp &m, int M, long nvalid, bool grad, float *Y) {
   for (int l = 0; l < L; ++l) {
     Prof pf(m, MC_FWD);
     const u16 *A = (l == 0) ? m.X0 : m.H[l - 1];
-    hipLaunchKernelGGL(k_gemm_nt<0>, dim3(M / kRows), dim3(256), 0, m.stream, m.tab.inpad[l], A, m.tab.Wt[l],
-                       m.w + m.tab.offb[l], nullptr, m.Z[l], m.H[l]);
+    if (M % kBigRows == 0)
+      hipLaunchKernelGGL(k_gemm_nt_big<0>, dim3(M / kBigRows), dim3(kBigThreads), 0, m.stream, m.tab.inpad[l], A,
+                         m.tab.Wt[l], m.w + m.tab.offb[l], nullptr, m.Z[l], m.H[l]);
+    else
+      hipLaunchKernelGGL(k_gemm_nt<0>, dim3(M / kRows), dim3(256), 0, m.stream, m.tab.inpad[l], A, m.tab.Wt[l],
+                         m.w + m.tab.offb[l], nullptr, m.Z[l], m.H[l]);
   }
   {
     Prof pf(m, MC_HEAD);
@@ -617,8 +691,12 @@ int run_pass(fvs_mlp &m, int M, long nvalid, bool grad, float *Y) {
       }
       if (l > 0) {
         Prof pf(m, MC_BWD);
-        hipLaunchKernelGGL(k_gemm_nt<1>, dim3(M / kRows), dim3(256), 0, m.stream, kW, cur, m.tab.Wk[l], nullptr,
-                           m.Z[l - 1], oth, nullptr);
+        if (M % kBigRows == 0)
+          hipLaunchKernelGGL(k_gemm_nt_big<1>, dim3(M / kBigRows), dim3(kBigThreads), 0, m.stream, kW, cur,
+                             m.tab.Wk[l], nullptr, m.Z[l - 1], oth, nullptr);
+        else
+          hipLaunchKernelGGL(k_gemm_nt<1>, dim3(M / kRows), dim3(256), 0, m.stream, kW, cur, m.tab.Wk[l], nullptr,
+                             m.Z[l - 1], oth, nullptr);
         std::swap(cur, oth);
       }
     }
