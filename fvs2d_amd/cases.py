@@ -79,7 +79,7 @@ def draw_instances(n: int, seed: int = 0, resolutions=RESOLUTIONS):
 
 def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, freeze_after: int,
               resolutions=RESOLUTIONS, sequential: int = 0, chunk: int = 250, graph: bool = True,
-              freeze_drop=None):
+              freeze_drop=None, freeze_stall=None):
     """Batched steady solves of drawn instances; optionally the first
     ``sequential`` of them again one at a time for comparison."""
     from .solver import BatchSolver
@@ -92,8 +92,17 @@ def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, fr
     b.initialize_freestream()
     setup = time.perf_counter() - t0
     t0 = time.perf_counter()
-    res = b.steady_solve_batch(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after, freeze_drop=freeze_drop)
+    last = [time.perf_counter()]
+
+    def progress(done, active):
+        if time.perf_counter() - last[0] > 20.0:
+            last[0] = time.perf_counter()
+            print(f"batch: {done} steps, {active} instances active", file=sys.stderr, flush=True)
+
+    res = b.steady_solve_batch(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after, freeze_drop=freeze_drop,
+                               freeze_stall=freeze_stall, progress=progress)
     sec = time.perf_counter() - t0
+    print(f"batch: {int((res['iterations'] > 0).sum())}/{ninst} converged in {sec:.1f} s", file=sys.stderr, flush=True)
     forces = b.wall_forces_batch()
     it = res["iterations"]
     ok = it > 0
@@ -113,7 +122,8 @@ def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, fr
             s = Solver(m, sc)
             s.set_graph(graph)
             s.initialize_solution(1, 0.0)
-            r = s.steady_solve(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after, freeze_at=freeze_drop)
+            r = s.steady_solve(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after, freeze_at=freeze_drop,
+                               freeze_stall=freeze_stall)
             n_ok += r["iterations"] is not None
             s.close()
         sq = time.perf_counter() - t0
@@ -143,6 +153,7 @@ def main(argv=None):
     bt.add_argument("--cfl", type=float, default=1.25)
     bt.add_argument("--freeze-after", type=int, default=1000)
     bt.add_argument("--freeze-drop", type=float, default=None)
+    bt.add_argument("--freeze-stall", type=str, default="none", help="window,factor (e.g. 2000,0.1) or none")
     bt.add_argument("--resolutions", type=str, default="128x64,256x128,384x192,512x256")
     bt.add_argument("--sequential", type=int, default=0)
     bt.add_argument("--graph", type=int, default=1)
@@ -152,8 +163,10 @@ def main(argv=None):
                          freeze_at=a.freeze_at)
     else:
         res = tuple(tuple(int(v) for v in r.split("x")) for r in a.resolutions.split(","))
+        stall = None if a.freeze_stall == "none" else tuple(t(v) for t, v in zip((int, float),
+                                                                                 a.freeze_stall.split(",")))
         out = run_batch(a.instances, a.seed, a.max_steps, a.drop, a.cfl, a.freeze_after, res, a.sequential,
-                        graph=bool(a.graph), freeze_drop=a.freeze_drop)
+                        graph=bool(a.graph), freeze_drop=a.freeze_drop, freeze_stall=stall)
     print(json.dumps(out))
     return 0
 

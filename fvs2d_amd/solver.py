@@ -504,7 +504,7 @@ class BatchSolver(Solver):
     def steady_solve_batch(self, max_steps: int, drop: float = 1e-8, chunk: int = 200,
                            freeze_after: Optional[int] = None, freeze_drop: Optional[float] = None,
                            r_ref=None, diverge: Optional[float] = None, on_diverge=None,
-                           freeze_stall: Optional[Tuple[int, float]] = None):
+                           freeze_stall: Optional[Tuple[int, float]] = None, progress=None):
         """Pseudo-time stepping of all instances, each until its density
         residual norm is at most ``drop * r_ref[k]`` (``r_ref`` defaults to
         each instance's first-step norm); converged instances are masked out
@@ -517,7 +517,8 @@ class BatchSolver(Solver):
         reset its state and return True to restart it, its step count running
         on, otherwise it is masked out.  Returns per-instance iterations (-1
         when not converged), steps run, divergence and restart flags, and the
-        step at which each limiter was frozen (-1: never)."""
+        step at which each limiter was frozen (-1: never).  ``progress(steps,
+        active)`` (optional) is called after every chunk."""
         n = self.ninst
         self.set_active(np.ones(n, np.int32))
         iters = np.full(n, -1, np.int64)
@@ -574,6 +575,8 @@ class BatchSolver(Solver):
             self.freeze_limiter_batch(frozen)
             if np.any(self.active):
                 self.set_active(self.active)
+            if progress is not None:
+                progress(done, int(np.sum(self.active)))
         self.freeze_limiter(False)
         self.set_active(np.ones(n, np.int32))
         return dict(iterations=iters, steps_run=done, diverged=diverged, restarted=restarted, frozen_at=frozen_at,
