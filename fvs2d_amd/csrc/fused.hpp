@@ -249,7 +249,7 @@ __device__ __forceinline__ void block_update(int N, int lb, int gw, const Upd &A
         for (int v = 0; v < 4; ++v) acc[v] = acc[v] + lF[4 * f + v];
       else
         for (int v = 0; v < 4; ++v) acc[v] = acc[v] - lF[4 * f + v];
-      wacc = wacc + lS[f];
+      if (A.steady && A.first) wacc = wacc + lS[f];
     }
     const double V = vol[i];
     double R[4];
@@ -313,7 +313,8 @@ __device__ __forceinline__ void block_update(int N, int lb, int gw, const Upd &A
 
 // fused face fluxes (LDS) + ordered gather + stage update; reads the stage's
 // primitive state from Win and writes the next one to Wout (separate buffers:
-// neighbouring blocks still read Win)
+// neighbouring blocks still read Win).  phi_one: no limiter, every limiter
+// value is 1 and is not loaded.
 __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, int maxf, Upd A, Phys P, double t,
                                                         const int *__restrict__ boff, const int *__restrict__ bfaces,
                                                         const int *__restrict__ lgat, int Fi,
@@ -329,19 +330,23 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
                                                         double *__restrict__ part, const int *__restrict__ blist,
                                                         const int *__restrict__ binst,
                                                         const double *__restrict__ pinf_inst,
-                                                        double *__restrict__ d2out) {
+                                                        double *__restrict__ d2out, int phi_one) {
   extern __shared__ double lds[];
   double *lF = lds;             // [maxf][4] flux*area (later the block reduction)
-  double *lS = lds + 4 * maxf;  // [maxf]    wave speed*area
+  double *lS = lds + 4 * maxf;  // [maxf]    wave speed*area, only when the stage sets local time steps
+  const bool ws = A.steady && A.first;
   __shared__ double sG[8 * kBlock];  // gradients of the block's own cells
+  __shared__ double sW[4 * kBlock];  // and their primitive states
   const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
   const int nbl = nb < 0 ? -nb : nb;
   if (lb >= nbl) return;  // the whole workgroup leaves before any barrier
   const int i0 = lb * kBlock;
   if (i0 + int(threadIdx.x) < N) {
-    double gg[8];
+    double gg[8], ww[4];
     ld8(grad, i0 + threadIdx.x, gg);
     st8(sG, threadIdx.x, gg);
+    ld4(Win, i0 + threadIdx.x, ww);
+    st4(sW, threadIdx.x, ww);
   }
   __syncthreads();
   const int f0 = boff[lb], nf = boff[lb + 1] - f0;
@@ -352,24 +357,33 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
       ld8(fgeo, slot, g);
       const int2 c = fc[slot];
       double wL[4], wR[4], gL[8], gR[8];
-      ld4(Win, c.x, wL);
-      ld4(Win, c.y, wR);
       const unsigned la = unsigned(c.x - i0), lc = unsigned(c.y - i0);
-      if (la < unsigned(kBlock)) ld8(sG, la, gL);
-      else ld8(grad, c.x, gL);
-      if (lc < unsigned(kBlock)) ld8(sG, lc, gR);
-      else ld8(grad, c.y, gR);
-      interior_flux_v(P, g, wL, wR, gL, gR, phi[c.x], phi[c.y], fa, wsa);
+      if (la < unsigned(kBlock)) {
+        ld4(sW, la, wL);
+        ld8(sG, la, gL);
+      } else {
+        ld4(Win, c.x, wL);
+        ld8(grad, c.x, gL);
+      }
+      if (lc < unsigned(kBlock)) {
+        ld4(sW, lc, wR);
+        ld8(sG, lc, gR);
+      } else {
+        ld4(Win, c.y, wR);
+        ld8(grad, c.y, gR);
+      }
+      interior_flux_v(P, g, wL, wR, gL, gR, phi_one ? 1.0 : phi[c.x], phi_one ? 1.0 : phi[c.y], fa, wsa);
     } else {
       const int b = slot - Fi, c = bcell[b];
       ld8(bgeo, b, g);
       double w[4], gr[8];
-      ld4(Win, c, w);
-      ld8(sG, c - i0, gr);  // a boundary face's cell is one of the block's own
-      boundary_flux_v(P, t, btype[b], g, w, gr, phi[c], fa, wsa, binst ? pinf_inst + 4 * binst[b] : P.pinf);
+      ld4(sW, c - i0, w);  // a boundary face's cell is one of the block's own
+      ld8(sG, c - i0, gr);
+      boundary_flux_v(P, t, btype[b], g, w, gr, phi_one ? 1.0 : phi[c], fa, wsa,
+                      binst ? pinf_inst + 4 * binst[b] : P.pinf);
     }
     for (int v = 0; v < 4; ++v) lF[4 * f + v] = fa[v];
-    lS[f] = wsa;
+    if (ws) lS[f] = wsa;
   }
   __syncthreads();
   block_update<kBlock>(N, lb, gw, A, lF, lS, lgat, vol, Ua, Ub, Facc, Wout, dtl, part, d2out, lF);

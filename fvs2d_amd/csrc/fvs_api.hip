@@ -1029,8 +1029,8 @@ int setup_device(fvs_ctx &C) {
       maxf = std::max(maxf, int(loc.size()));
     }
     // LDS of k_flux_update: face fluxes (at least the 4 x 256 reduction) +
-    // the block's gradients
-    if (size_t(std::max(5 * maxf, 4 * kBlock)) * sizeof(double) + 8 * kBlock * sizeof(double) <= 64 * 1024) {
+    // the block's gradients and states
+    if (size_t(std::max(5 * maxf, 4 * kBlock)) * sizeof(double) + 12 * kBlock * sizeof(double) <= 64 * 1024) {
       C.maxf = maxf;
       C.nlisted = long(bfaces.size());
       if (int rc = C.upload(C.boff, boff)) return rc;
@@ -1373,12 +1373,14 @@ int launch_flux_update(fvs_ctx &C, const Upd &A, double t) {
   {
     ProfScope ps(C, KC_FUSED);
     const int nb = nblk(C.N);
-    const size_t lds = size_t(std::max(5 * C.maxf, 4 * kBlock)) * sizeof(double);
+    // the wave-speed array only where the stage sets the local time step
+    const int per = (A.steady && A.first) ? 5 : 4;
+    const size_t lds = size_t(std::max(per * C.maxf, 4 * kBlock)) * sizeof(double);
     hipLaunchKernelGGL(k_flux_update, dim3(C.nact ? C.nact : xcd_grid(nb)), dim3(kBlock), lds, C.stream, C.N,
                        xcd_arg(nb, C.xcd_fused), C.GW, C.maxf, A, C.P, t, C.boff, C.bfaces, C.lgat, C.Fi, C.fi_cells,
                        C.fi_geo, C.fb_cell, C.fb_type, C.fb_geo, C.W, C.grad, C.phi, C.vol, C.U0, C.Ub, C.Facc,
                        C.Walt, C.dtl, C.part, C.nact ? C.blist : nullptr, C.dbinst, C.dpinf_inst,
-                       A.last ? C.d2buf : nullptr);
+                       A.last ? C.d2buf : nullptr, (C.opt.limiter == 0 && C.opt.recon != 1) ? 1 : 0);
   }
   HIPC(hipGetLastError());
   std::swap(C.W, C.Walt);

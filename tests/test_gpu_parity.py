@@ -351,3 +351,18 @@ def test_stage_kernel_steady_naca_freeze():
     for k in (1, 2):
         for a, b in zip(out[0], out[k]):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kernel_path", [0, 1, 3])
+def test_trajectory_first_order(kernel_path):
+    """First-order reconstruction (limiter values 0, no gradient) through the
+    time-stepping kernels, 20 SSP-RK3 steps, against the oracle."""
+    m = mesh("dirichlet", 32)
+    sch = vortex_scheme(recon=1, dt=0.004)
+    orc = Oracle(m, **sch.oracle_kwargs())
+    orc.initialize(1, 0.0)
+    orc.time_integration(0.0, 20)
+    gpu = Solver(m, sch, kernel_path=kernel_path, reorder=1)
+    gpu.initialize_solution(1, 0.0)
+    gpu.time_integration(0.0, 20)
+    assert rel_err(gpu.cvar, orc.cvar) <= 1e-10
