@@ -17,11 +17,17 @@ Synthesisers:
   boundary edge (the reference pairs boundary cells and boundary edges one to
   one, src/residual.f90:115-125).
 * :func:`naca_ogrid` -- O-grid around a symmetric NACA 4-digit airfoil.
+
+Import: :func:`read_gambit` reads a Gambit neutral file (``.neu``) the way
+utils/tools/read_gambit.f90 does; :func:`import_gambit` also writes the three
+files that tool writes (``.grid``, ``.bc`` and a Tecplot ``.plt`` of the mesh).
 """
 from __future__ import annotations
 
 import dataclasses
 import math
+import os
+import re
 from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -265,6 +271,128 @@ def read_grid(base: str) -> Mesh:
     return Mesh(xy, nt, nq, cell_nodes, bnds)
 
 
+def _ints(line: str) -> List[int]:
+    return [int(v) for v in line.split()]
+
+
+def _float(tok: str) -> float:
+    return float(tok.replace("d", "e").replace("D", "e"))
+
+
+def read_gambit(path: str) -> Mesh:
+    """Read a 2-D Gambit neutral file into a :class:`Mesh`.
+
+    Follows utils/tools/read_gambit.f90:29-111 and its renumbering at :139-152:
+    the counts line is the 7th line (nodes, cells, element groups, boundary
+    sets); node coordinates are taken in file order; a cell is a triangle or a
+    quadrilateral by its node count (third field of its record, :42-50); cells
+    are renumbered triangles first, then quadrilaterals, each group in file
+    order; every boundary set is a name followed by one record per boundary
+    edge whose first field is the (file-numbered) cell that carries it, and
+    those cell ids are mapped through the renumbering (:161-166).  Node lists
+    are kept as written (Gambit writes them counter-clockwise).
+
+    Where the tool would silently produce a broken mesh this raises
+    ValueError: a cell with other than 3 or 4 nodes (the tool's cell-count
+    check at :54-57), a node-based boundary set (type 0: its entries are node
+    ids, which the tool would read as cell ids), or a boundary entry naming no
+    cell.  The tool reads a boundary header at fixed columns (``a39,i1,4x,i4``,
+    :101), which caps a set at 9999 entries; here the header is split on
+    blanks, which reads the same fields on every file the tool reads.
+    """
+    with open(path) as f:
+        lines = f.read().splitlines()
+    if len(lines) < 7:
+        raise ValueError(f"{path}: too short for a Gambit neutral file")
+    counts = _ints(lines[6])
+    if len(counts) < 4:
+        raise ValueError(f"{path}: line 7 must hold NUMNP NELEM NGRPS NBSETS")
+    nnodes, ncells, _, nbsets = counts[:4]
+
+    def section(start: int, key: str) -> int:
+        for i in range(start, len(lines)):
+            if key in lines[i].upper():
+                return i + 1
+        raise ValueError(f"{path}: no '{key}' section")
+
+    pos = section(7, "NODAL COORDINATES")
+    xy = np.empty((nnodes, 2), np.float64)
+    for i in range(nnodes):
+        t = lines[pos + i].split()
+        xy[i] = _float(t[1]), _float(t[2])
+    pos = section(pos + nnodes, "ELEMENTS/CELLS")
+    nv = np.empty(ncells, np.int64)
+    conn = []
+    for i in range(ncells):
+        t = _ints(lines[pos + i])
+        nv[i] = t[2]
+        if t[2] not in (3, 4):
+            raise ValueError(f"{path}: cell {t[0]} has {t[2]} nodes; only triangles and quadrilaterals")
+        conn.append([v - 1 for v in t[3:3 + t[2]]])
+        if len(conn[-1]) != t[2]:
+            raise ValueError(f"{path}: cell {t[0]} lists {len(conn[-1])} of its {t[2]} nodes")
+    tri = np.flatnonzero(nv == 3)
+    quad = np.flatnonzero(nv == 4)
+    new_id = np.empty(ncells, np.int64)
+    new_id[tri] = np.arange(tri.size)
+    new_id[quad] = tri.size + np.arange(quad.size)
+    cell_nodes = np.array([v for i in tri for v in conn[i]] + [v for i in quad for v in conn[i]], np.int32)
+    if cell_nodes.size and (cell_nodes.min() < 0 or cell_nodes.max() >= nnodes):
+        raise ValueError(f"{path}: a cell names a node outside 1..{nnodes}")
+
+    bnds = []
+    pos += ncells
+    for _ in range(nbsets):
+        pos = section(pos, "BOUNDARY CONDITIONS")
+        head = lines[pos].split()
+        k = len(head)
+        while k > 0 and re.fullmatch(r"[+-]?\d+", head[k - 1]):
+            k -= 1
+        if k == 0 or len(head) - k < 2:
+            raise ValueError(f"{path}: malformed boundary header {lines[pos]!r}")
+        name = " ".join(head[:k])
+        itype, n = int(head[k]), int(head[k + 1])
+        if itype != 1:
+            raise ValueError(f"{path}: boundary '{name}' is node-based (type {itype}); only cell-based sets")
+        ids = np.array([_ints(lines[pos + 1 + j])[0] for j in range(n)], np.int64)
+        if ids.size and (ids.min() < 1 or ids.max() > ncells):
+            raise ValueError(f"{path}: boundary '{name}' names a cell outside 1..{ncells}")
+        bnds.append((name, new_id[ids - 1].astype(np.int32)))
+        pos += 1 + n
+    return Mesh(xy, int(tri.size), int(quad.size), cell_nodes, bnds)
+
+
+def write_plt(mesh: Mesh, path: str) -> None:
+    """Tecplot finite-element file of the mesh, as utils/tools/read_gambit.f90
+    writes it (:170-196): a triangle zone, or, when the mesh has quadrilaterals,
+    a quadrilateral zone in which triangles repeat their third node."""
+    cells = mesh.cells_as_array()
+    quad = mesh.nquad > 0
+    with open(path, "w") as f:
+        f.write(' TITLE = "Example: 2D Finite Element Data"\n')
+        f.write(' VARIABLES = "X", "Y"\n')
+        f.write(f"ZONE NODES={mesh.nnodes} ELEMENTS={mesh.ncells} DATAPACKING=POINT, "
+                f"ZONETYPE={'FEQUADRILATERAL' if quad else 'FETRIANGLE'}\n")
+        for x, y in mesh.xy:
+            f.write(f"{float(x)!r} {float(y)!r}\n")
+        for c in cells:
+            if quad:
+                d = c[3] if c[3] >= 0 else c[2]
+                f.write(f"{c[0] + 1} {c[1] + 1} {c[2] + 1} {d + 1}\n")
+            else:
+                f.write(f"{c[0] + 1} {c[1] + 1} {c[2] + 1}\n")
+
+
+def import_gambit(path: str) -> Mesh:
+    """Convert ``<base>.neu`` to ``<base>.grid``, ``<base>.bc`` and ``<base>.plt``
+    beside it, as utils/tools/read_gambit.f90 does; returns the mesh."""
+    mesh = read_gambit(path)
+    base = path[: path.rfind(".")] if "." in os.path.basename(path) else path
+    write_grid(mesh, base)
+    write_plt(mesh, base + ".plt")
+    return mesh
+
+
 def concat(meshes: Sequence[Mesh]) -> Tuple[Mesh, np.ndarray, np.ndarray]:
     """Concatenate independent meshes into one ragged batch.
 
@@ -291,3 +419,14 @@ def concat(meshes: Sequence[Mesh]) -> Tuple[Mesh, np.ndarray, np.ndarray]:
         cells = [np.asarray(c, np.int64) + coff[i] for i, m in enumerate(meshes) for nm, c in m.boundaries if nm == n]
         bnds.append((n, np.concatenate(cells).astype(np.int32)))
     return Mesh(xy, int(coff[-1]), 0, cn, bnds), coff, noff
+
+
+if __name__ == "__main__":
+    import sys
+    if len(sys.argv) != 2 or not sys.argv[1].endswith(".neu"):
+        sys.exit("usage: python -m fvs2d_amd.meshgen <mesh>.neu  (writes <mesh>.grid, .bc, .plt)")
+    m = import_gambit(sys.argv[1])
+    print(f"nnodes: {m.nnodes}\nncells triangles: {m.ntri}\nncells quad: {m.nquad}\n"
+          f"# boundaries: {len(m.boundaries)}")
+    for i, (name, c) in enumerate(m.boundaries, 1):
+        print(f"boundary({i}) {name}: {len(c)} cells")

@@ -31,13 +31,15 @@ class Opts(C.Structure):
 
 REF_SRC = "/root/reference/src/ios_unstrc.f90"
 IOS_REF = os.path.join(HERE, "_ref", "ios_ref")
+GAMBIT_REF = os.path.join(HERE, "_ref", "read_gambit")
 FLANG = "/opt/rocm/bin/amdflang"
 
 
 def build() -> str:
     """Compile the oracle library in place (gcc, via oracle/Makefile), and,
     where the reference sources and amdflang are present, the reference's
-    restart-file module under its test driver (oracle/_ref/ios_ref)."""
+    restart-file module under its test driver (oracle/_ref/ios_ref) and its
+    Gambit neutral-file converter (oracle/_ref/read_gambit)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     if os.path.exists(REF_SRC) and os.path.exists(FLANG):
         subprocess.run(["make", "-s", "-B", "-C", HERE, "ref"], check=True)
