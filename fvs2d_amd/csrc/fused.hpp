@@ -241,8 +241,14 @@ __device__ __forceinline__ void block_update(int N, int lb, int gw, const Upd &A
   double d2[4] = {0, 0, 0, 0};
   if (threadIdx.x < kBlock && i < N) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0}, wacc = 0.0;
-    for (int k = 0; k < gw; ++k) {
-      const int s = lgat[long(k) * N + i];
+    // all of the cell's gather entries (at most 4 faces) are loaded at once
+    int sv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sv[k] = (k < gw) ? lgat[long(k) * N + i] : 0;
+    const double V = vol[i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int s = sv[k];
       if (s == 0) break;
       const int f = (s > 0 ? s : -s) - 1;
       if (s > 0)
@@ -251,7 +257,6 @@ __device__ __forceinline__ void block_update(int N, int lb, int gw, const Upd &A
         for (int v = 0; v < 4; ++v) acc[v] = acc[v] - lF[4 * f + v];
       if (A.steady && A.first) wacc = wacc + lS[f];
     }
-    const double V = vol[i];
     double R[4];
     for (int v = 0; v < 4; ++v) R[v] = -acc[v] / V;
     double hs = A.h;
@@ -316,11 +321,9 @@ __device__ __forceinline__ void block_update(int N, int lb, int gw, const Upd &A
 // neighbouring blocks still read Win).  phi_one: no limiter, every limiter
 // value is 1 and is not loaded.
 __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, int maxf, Upd A, Phys P, double t,
-                                                        const int *__restrict__ boff, const int *__restrict__ bfaces,
-                                                        const int *__restrict__ lgat, int Fi,
-                                                        const int2 *__restrict__ fc, const double *__restrict__ fgeo,
-                                                        const int *__restrict__ bcell, const int *__restrict__ btype,
-                                                        const double *__restrict__ bgeo,
+                                                        const int *__restrict__ boff, const int2 *__restrict__ lrec,
+                                                        const double *__restrict__ lgeo, long L,
+                                                        const int *__restrict__ lgat,
                                                         const double *__restrict__ Win,
                                                         const double *__restrict__ grad,
                                                         const double *__restrict__ phi,
@@ -328,7 +331,6 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
                                                         double *__restrict__ Ub, double *__restrict__ Facc,
                                                         double *__restrict__ Wout, double *__restrict__ dtl,
                                                         double *__restrict__ part, const int *__restrict__ blist,
-                                                        const int *__restrict__ binst,
                                                         const double *__restrict__ pinf_inst,
                                                         double *__restrict__ d2out, int phi_one) {
   extern __shared__ double lds[];
@@ -341,6 +343,16 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
   const int nbl = nb < 0 ? -nb : nb;
   if (lb >= nbl) return;  // the whole workgroup leaves before any barrier
   const int i0 = lb * kBlock;
+  const int f0 = boff[lb], nf = boff[lb + 1] - f0;
+  // the first face record is in flight while the block's own cells are staged
+  int f = threadIdx.x;
+  int2 c = make_int2(0, 0);
+  double g[8];
+  g[7] = 0.0;
+  if (f < nf) {
+    c = lrec[f0 + f];
+    for (int k = 0; k < 7; ++k) g[k] = lgeo[k * L + f0 + f];
+  }
   if (i0 + int(threadIdx.x) < N) {
     double gg[8], ww[4];
     ld8(grad, i0 + threadIdx.x, gg);
@@ -349,13 +361,13 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
     st4(sW, threadIdx.x, ww);
   }
   __syncthreads();
-  const int f0 = boff[lb], nf = boff[lb + 1] - f0;
-  for (int f = threadIdx.x; f < nf; f += kBlock) {
-    const int slot = bfaces[f0 + f];
-    double fa[4], wsa, g[8];
-    if (slot < Fi) {
-      ld8(fgeo, slot, g);
-      const int2 c = fc[slot];
+  for (; f < nf; f += kBlock) {
+    if (f != int(threadIdx.x)) {
+      c = lrec[f0 + f];
+      for (int k = 0; k < 7; ++k) g[k] = lgeo[k * L + f0 + f];
+    }
+    double fa[4], wsa;
+    if (c.y >= 0) {
       double wL[4], wR[4], gL[8], gR[8];
       const unsigned la = unsigned(c.x - i0), lc = unsigned(c.y - i0);
       if (la < unsigned(kBlock)) {
@@ -374,13 +386,12 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
       }
       interior_flux_v(P, g, wL, wR, gL, gR, phi_one ? 1.0 : phi[c.x], phi_one ? 1.0 : phi[c.y], fa, wsa);
     } else {
-      const int b = slot - Fi, c = bcell[b];
-      ld8(bgeo, b, g);
+      const int code = -1 - c.y;
       double w[4], gr[8];
-      ld4(sW, c - i0, w);  // a boundary face's cell is one of the block's own
-      ld8(sG, c - i0, gr);
-      boundary_flux_v(P, t, btype[b], g, w, gr, phi_one ? 1.0 : phi[c], fa, wsa,
-                      binst ? pinf_inst + 4 * binst[b] : P.pinf);
+      ld4(sW, c.x - i0, w);  // a boundary face's cell is one of the block's own
+      ld8(sG, c.x - i0, gr);
+      boundary_flux_v(P, t, code & 3, g, w, gr, phi_one ? 1.0 : phi[c.x], fa, wsa,
+                      pinf_inst ? pinf_inst + 4 * (code >> 2) : P.pinf);
     }
     for (int v = 0; v < 4; ++v) lF[4 * f + v] = fa[v];
     if (ws) lS[f] = wsa;

@@ -662,6 +662,10 @@ struct fvs_ctx {
   int maxf = 0;
   int *boff = nullptr, *bfaces = nullptr, *lgat = nullptr;
   long nlisted = 0;
+  // the listed faces' records in list order: cells (interior: c1, c2;
+  // boundary: cell, -1 - (type | instance << 2)) and geometry, 7 planes
+  int2 *lrec = nullptr;
+  double *lgeo = nullptr;
   double *Walt = nullptr;  // second primitive-state buffer
   // XCD-aware block numbering per kernel class (fvs_autotune picks)
   bool xcd_grad = false, xcd_flux = false, xcd_upd = false, xcd_fused = true;
@@ -1002,7 +1006,8 @@ int setup_device(fvs_ctx &C) {
   // touching block b's cells, ascending (interior slots, then boundary slots),
   // and each cell's faces as signed local indices in the gather order
   std::vector<int> h_boff, h_bfaces;
-  if (C.opt.kernel_path == 0 || C.opt.kernel_path == 2 || C.opt.kernel_path == 3) {
+  // (the fused update gathers at most 4 faces per cell: triangles and quads)
+  if ((C.opt.kernel_path == 0 || C.opt.kernel_path == 2 || C.opt.kernel_path == 3) && gw <= 4) {
     const int nbk = nblk(N);
     std::vector<int> boff(nbk + 1, 0), bfaces, lgat(size_t(gw) * N, 0), loc;
     int maxf = 0;
@@ -1036,6 +1041,25 @@ int setup_device(fvs_ctx &C) {
       if (int rc = C.upload(C.boff, boff)) return rc;
       if (int rc = C.upload(C.bfaces, bfaces)) return rc;
       if (int rc = C.upload(C.lgat, lgat)) return rc;
+      const size_t L = bfaces.size();
+      std::vector<int2> lrec(L);
+      std::vector<double> lgeo(7 * L);
+      for (size_t f = 0; f < L; ++f) {
+        const int sl = bfaces[f];
+        const double *g;
+        if (sl < Fi) {
+          lrec[f] = fic[sl];
+          g = &fig[8 * size_t(sl)];
+        } else {
+          const int bq = sl - Fi;
+          const int inst = C.ninst > 0 ? C.inst_of_real[P[fbc[bq]]] : 0;
+          lrec[f] = make_int2(fbc[bq], -1 - (fbt[bq] | (inst << 2)));
+          g = &fbg[8 * size_t(bq)];
+        }
+        for (int k = 0; k < 7; ++k) lgeo[size_t(k) * L + f] = g[k];
+      }
+      if (int rc = C.upload(C.lrec, lrec)) return rc;
+      if (int rc = C.upload(C.lgeo, lgeo)) return rc;
       C.fused = true;
       h_boff.swap(boff);
       h_bfaces.swap(bfaces);
@@ -1377,9 +1401,9 @@ int launch_flux_update(fvs_ctx &C, const Upd &A, double t) {
     const int per = (A.steady && A.first) ? 5 : 4;
     const size_t lds = size_t(std::max(per * C.maxf, 4 * kBlock)) * sizeof(double);
     hipLaunchKernelGGL(k_flux_update, dim3(C.nact ? C.nact : xcd_grid(nb)), dim3(kBlock), lds, C.stream, C.N,
-                       xcd_arg(nb, C.xcd_fused), C.GW, C.maxf, A, C.P, t, C.boff, C.bfaces, C.lgat, C.Fi, C.fi_cells,
-                       C.fi_geo, C.fb_cell, C.fb_type, C.fb_geo, C.W, C.grad, C.phi, C.vol, C.U0, C.Ub, C.Facc,
-                       C.Walt, C.dtl, C.part, C.nact ? C.blist : nullptr, C.dbinst, C.dpinf_inst,
+                       xcd_arg(nb, C.xcd_fused), C.GW, C.maxf, A, C.P, t, C.boff, C.lrec, C.lgeo, C.nlisted,
+                       C.lgat, C.W, C.grad, C.phi, C.vol, C.U0, C.Ub, C.Facc, C.Walt, C.dtl, C.part,
+                       C.nact ? C.blist : nullptr, C.ninst > 0 ? C.dpinf_inst : nullptr,
                        A.last ? C.d2buf : nullptr, (C.opt.limiter == 0 && C.opt.recon != 1) ? 1 : 0);
   }
   HIPC(hipGetLastError());
@@ -2183,9 +2207,12 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   if (C->opt.steady) state += (8.0 + 8.0 * (nst - 1)) / nst;
   const double bup = N * (C->GW * 4.0 + 8 + 32 + state) + (Fi + Fb) * 40.0;
   const double bno = 4.0 * 8 * nblk(C->N) + 32;
-  // fused: face records and block lists once, the stage's W/grad/phi once,
-  // the cell-side traffic of the update kernel without the face-flux buffer
-  const double bfu = (Fi + Fb) * (8 + 56) + 4.0 * double(C->nlisted) + N * (32 + 64 + 8) +
+  // fused: the listed face records (cells + 7 geometry values; a face on
+  // a block border is listed, and stored, once per block), the stage's W and
+  // grad once (phi unless there is no limiter), the cell-side traffic of the
+  // update kernel without the face-flux buffer
+  const bool phi1 = C->opt.limiter == 0 && C->opt.recon != 1;
+  const double bfu = double(C->nlisted) * (8 + 56) + N * (32 + 64 + (phi1 ? 0 : 8)) +
                      N * (C->GW * 4.0 + 8 + 32 + state);
   // whole stage: own states, centres and G^-1 once, the per-block cell lists
   // and local stencils, face records and block lists once, limiter inputs,
