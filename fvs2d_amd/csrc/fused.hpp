@@ -117,12 +117,11 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const doubl
     }
     accumulate(code, cj, q);
   }
-  const double2 g01 = reinterpret_cast<const double2 *>(ginv)[2 * i];
-  const double2 g2 = reinterpret_cast<const double2 *>(ginv)[2 * i + 1];
+  const double G11 = ginv[i], G12 = ginv[N + i], G22 = ginv[2 * long(N) + i];
   double gx[4], gy[4];
   for (int v = 0; v < 4; ++v) {
-    gx[v] = g01.x * sx[v] + g01.y * sy[v];
-    gy[v] = g01.y * sx[v] + g2.x * sy[v];
+    gx[v] = G11 * sx[v] + G12 * sy[v];
+    gy[v] = G12 * sx[v] + G22 * sy[v];
   }
   double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
   st8(grad, i, g8);
@@ -231,7 +230,7 @@ __device__ __forceinline__ void block_sum4_t(double v[4], double *red) {
 
 template <int NT>
 __device__ __forceinline__ void block_update(int N, int lb, int gw, const Upd &A, const double *__restrict__ lF,
-                                             const double *__restrict__ lS, const int *__restrict__ lgat,
+                                             const double *__restrict__ lS, const short *__restrict__ lgat,
                                              const double *__restrict__ vol, double *__restrict__ Ua,
                                              double *__restrict__ Ub, double *__restrict__ Facc,
                                              double *__restrict__ Wout, double *__restrict__ dtl,
@@ -323,7 +322,7 @@ __device__ __forceinline__ void block_update(int N, int lb, int gw, const Upd &A
 __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, int maxf, Upd A, Phys P, double t,
                                                         const int *__restrict__ boff, const int2 *__restrict__ lrec,
                                                         const double *__restrict__ lgeo, long L,
-                                                        const int *__restrict__ lgat,
+                                                        const short *__restrict__ lgat,
                                                         const double *__restrict__ Win,
                                                         const double *__restrict__ grad,
                                                         const double *__restrict__ phi,
@@ -467,12 +466,11 @@ __device__ __forceinline__ double lds_gradient(int q, int id, int N, const doubl
       }
     }
   }
-  const double2 g01 = reinterpret_cast<const double2 *>(ginv)[2 * id];
-  const double2 g2 = reinterpret_cast<const double2 *>(ginv)[2 * id + 1];
+  const double G11 = ginv[id], G12 = ginv[N + id], G22 = ginv[2 * long(N) + id];
   double gx[4], gy[4];
   for (int v = 0; v < 4; ++v) {
-    gx[v] = g01.x * sx[v] + g01.y * sy[v];
-    gy[v] = g01.y * sx[v] + g2.x * sy[v];
+    gx[v] = G11 * sx[v] + G12 * sy[v];
+    gy[v] = G12 * sx[v] + G22 * sy[v];
     g8[v] = gx[v];
     g8[4 + v] = gy[v];
   }
@@ -494,16 +492,16 @@ __device__ __forceinline__ double lds_gradient(int q, int id, int N, const doubl
   return phisrc == 1 ? phi[id] : 1.0;
 }
 
-// LSQ gradient (and limiter) of the block's own cells with the block's whole
-// stencil halo staged in LDS first (the per-block lists of k_stage): one bulk
-// gather per block instead of one gather per stencil entry.  Bit-identical to
+// LSQ gradient (and limiter) of the block's own cells with the cells its
+// stencils read staged in LDS first: the block's own cells (coalesced) at
+// local 0..255, the halo (the outside cells its stencils read, one gather
+// each) from 256 on; the stencils as 16-bit local entries.  Bit-identical to
 // k_grad_lsqc.
 template <int LIM, int PMODE>
-__global__ __launch_bounds__(kBlock) void k_grad_lsqh(int N, int nb, int max2, const double *__restrict__ W,
-                                                      const int *__restrict__ s2off, const int *__restrict__ s2list,
-                                                      const int *__restrict__ sl0, const int *__restrict__ slw,
-                                                      const long *__restrict__ sleo,
-                                                      const unsigned short *__restrict__ slid,
+__global__ __launch_bounds__(kBlock) void k_grad_lsqh(int N, int nb, int nl, const double *__restrict__ W,
+                                                      const int *__restrict__ ghoff, const int *__restrict__ ghlist,
+                                                      const int *__restrict__ soff, const int *__restrict__ swid,
+                                                      const unsigned short *__restrict__ lcode,
                                                       const double2 *__restrict__ cxy,
                                                       const double *__restrict__ ginv, ShiftTable st, double p,
                                                       double *__restrict__ grad, double *__restrict__ phi, int lt,
@@ -512,25 +510,31 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqh(int N, int nb, int max2, c
                                                       const int *__restrict__ ioff, const int *__restrict__ frz,
                                                       int ninst) {
   extern __shared__ double lds[];
-  double *lW = lds;                                            // [max2][4]
-  double2 *lX = reinterpret_cast<double2 *>(lds + 4 * max2);   // [max2]
+  double *lW = lds;                                          // [nl][4]
+  double2 *lX = reinterpret_cast<double2 *>(lds + 4 * nl);   // [nl]
   const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
   if (lb >= (nb < 0 ? -nb : nb)) return;  // the whole workgroup leaves before the barrier
-  const int o2 = s2off[lb], n2 = s2off[lb + 1] - o2;
-  for (int q = threadIdx.x; q < n2; q += kBlock) {
-    const int id = s2list[o2 + q];
+  const int i0 = lb * kBlock, id = i0 + threadIdx.x;
+  const int o2 = ghoff[lb], nh = ghoff[lb + 1] - o2;
+  if (id < N) {
     double w[4];
     ld4(W, id, w);
-    st4(lW, q, w);
-    lX[q] = cxy[id];
+    st4(lW, threadIdx.x, w);
+    lX[threadIdx.x] = cxy[id];
+  }
+  for (int q = threadIdx.x; q < nh; q += kBlock) {
+    const int j = ghlist[o2 + q];
+    double w[4];
+    ld4(W, j, w);
+    st4(lW, kBlock + q, w);
+    lX[kBlock + q] = cxy[j];
   }
   __syncthreads();
-  const int q = threadIdx.x, id = lb * kBlock + q;
   if (id >= N) return;
-  const int si = sl0[lb] + q / kSlice;
+  const int s = id / kSlice;
   double g8[8];
-  lds_gradient<LIM, PMODE>(q, id, N, lW, lX, slid + sleo[si], slw[si], st, p, ginv, g8, 0, phi, lt, nvrt, loff, vol,
-                           ioff, frz, ninst, true);
+  lds_gradient<LIM, PMODE>(threadIdx.x, id, N, lW, lX, lcode + soff[s], swid[s], st, p, ginv, g8, 0, phi, lt, nvrt,
+                           loff, vol, ioff, frz, ninst, true);
   st8(grad, id, g8);
 }
 
@@ -539,7 +543,7 @@ constexpr int kStageThreads = 2 * kBlock;  // two threads per cell: each phase f
 template <int LIM, int PMODE>
 __global__ __launch_bounds__(kStageThreads) void k_stage(
     int N, int nb, int gw, int maxf, int max1, int max2, Upd A, Phys P, double t, const int *__restrict__ boff,
-    const int *__restrict__ bfaces, const unsigned *__restrict__ blf, const int *__restrict__ lgat, int Fi,
+    const int *__restrict__ bfaces, const unsigned *__restrict__ blf, const short *__restrict__ lgat, int Fi,
     const double *__restrict__ fgeo, const int *__restrict__ btype, const double *__restrict__ bgeo,
     const int *__restrict__ s2off, const int *__restrict__ s2n1, const int *__restrict__ s2list,
     const int *__restrict__ sl0, const int *__restrict__ slw, const long *__restrict__ sleo,

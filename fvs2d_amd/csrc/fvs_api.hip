@@ -660,7 +660,8 @@ struct fvs_ctx {
   // fused flux + update (k_flux_update): per-block face lists, local gather
   bool fused = false;
   int maxf = 0;
-  int *boff = nullptr, *bfaces = nullptr, *lgat = nullptr;
+  int *boff = nullptr, *bfaces = nullptr;
+  short *lgat = nullptr;  // signed 1-based index into the block's face list
   long nlisted = 0;
   // the listed faces' records in list order: cells (interior: c1, c2;
   // boundary: cell, -1 - (type | instance << 2)) and geometry, 7 planes
@@ -680,6 +681,14 @@ struct fvs_ctx {
   unsigned short *slid = nullptr;
   unsigned *blf = nullptr;
   long nslid = 0, nlist2 = 0, nslid_own = 0;
+  // staged-halo gradient (k_grad_lsqh): per block, the cells outside it that
+  // its stencils read (ascending), and every stencil entry as a local index
+  // (own cells 0..255, halo 256..) << 4 | periodic-shift code, laid out like
+  // the global stencil (sliced ELL, same offsets and widths)
+  int *ghoff = nullptr, *ghlist = nullptr;
+  unsigned short *lcode = nullptr;
+  int maxh = 0;  // largest halo of a block
+  long nghl = 0, nlcode = 0;
   // batch mode: per-instance frozen flags (dfrz) while only some are frozen
   int *dfrz = nullptr;
   bool frz_some = false;
@@ -921,8 +930,59 @@ int setup_stage(fvs_ctx &C, const std::vector<int> &boff, const std::vector<int>
   C.nslid = long(slid.size());
   C.nlist2 = long(s2list.size());
   C.nslid_own = eo_own;
+  C.stage = lds <= 64 * 1024;
+  return 0;
+}
+
+// Lists of the staged-halo gradient k_grad_lsqh (see the fvs_ctx fields).
+// Local entries keep the order of the global stencil, so the gradients are
+// bit-identical to k_grad_lsqc's.  Enabled (C.halo) when every block's local
+// cells number at most 4095 and their states and centres fit 64 KiB of LDS.
+int setup_grad_halo(fvs_ctx &C, const std::vector<int> &off, const std::vector<int> &wid,
+                    const std::vector<unsigned> &code) {
+  const int N = C.N, nbk = nblk(N);
+  std::vector<int> ghoff(nbk + 1, 0), ghlist, halo;
+  std::vector<unsigned short> lcode(code.size(), 0);
+  int maxh = 0;
+  for (int b = 0; b < nbk; ++b) {
+    const int i0 = b * kBlock, i1 = std::min(N, i0 + kBlock);
+    halo.clear();
+    for (int i = i0; i < i1; ++i) {
+      const int sg = i / kSlice, l = i % kSlice;
+      for (int k = 0; k < wid[sg]; ++k) {
+        const int j = int(code[off[sg] + long(k) * kSlice + l] & 0x07ffffffu);
+        if (j < i0 || j >= i1) halo.push_back(j);
+      }
+    }
+    std::sort(halo.begin(), halo.end());
+    halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+    if (kBlock + halo.size() > 4095) return 0;
+    for (int i = i0; i < i1; ++i) {
+      const int sg = i / kSlice, l = i % kSlice;
+      for (int k = 0; k < wid[sg]; ++k) {
+        const long e = off[sg] + long(k) * kSlice + l;
+        const int j = int(code[e] & 0x07ffffffu);
+        int loc = j - i0;
+        if (j < i0 || j >= i1) loc = kBlock + int(std::lower_bound(halo.begin(), halo.end(), j) - halo.begin());
+        if (loc < 0 || loc >= kBlock + int(halo.size()) || (loc < kBlock && i0 + loc >= i1))
+          return fail(FVS_EINVAL, "halo set-up: stencil entry outside the block's local cells");
+        lcode[e] = static_cast<unsigned short>((unsigned(loc) << 4) | (code[e] >> 27));
+      }
+    }
+    ghlist.insert(ghlist.end(), halo.begin(), halo.end());
+    ghoff[b + 1] = int(ghlist.size());
+    maxh = std::max(maxh, int(halo.size()));
+  }
+  if (size_t(6) * (kBlock + maxh) * sizeof(double) > 64 * 1024) return 0;
+  if (int rc = C.upload(C.ghoff, ghoff)) return rc;
+  if (ghlist.empty()) ghlist.push_back(0);
+  if (int rc = C.upload(C.ghlist, ghlist)) return rc;
+  if (int rc = C.upload(C.lcode, lcode)) return rc;
+  C.maxh = maxh;
+  C.nghl = ghoff[nbk];
+  C.nlcode = 0;
+  for (int w : wid) C.nlcode += long(w) * kSlice;
   C.halo = true;
-  C.stage = C.opt.kernel_path == 2 && lds <= 64 * 1024;
   return 0;
 }
 
@@ -1009,7 +1069,8 @@ int setup_device(fvs_ctx &C) {
   // (the fused update gathers at most 4 faces per cell: triangles and quads)
   if ((C.opt.kernel_path == 0 || C.opt.kernel_path == 2 || C.opt.kernel_path == 3) && gw <= 4) {
     const int nbk = nblk(N);
-    std::vector<int> boff(nbk + 1, 0), bfaces, lgat(size_t(gw) * N, 0), loc;
+    std::vector<int> boff(nbk + 1, 0), bfaces, loc;
+    std::vector<short> lgat(size_t(gw) * N, 0);
     int maxf = 0;
     for (int b = 0; b < nbk; ++b) {
       loc.clear();
@@ -1027,7 +1088,7 @@ int setup_device(fvs_ctx &C) {
           if (!sg) continue;
           const int slot = (sg > 0 ? sg : -sg) - 1;
           const int li = int(std::lower_bound(loc.begin(), loc.end(), slot) - loc.begin()) + 1;
-          lgat[size_t(k) * N + i] = sg > 0 ? li : -li;
+          lgat[size_t(k) * N + i] = short(sg > 0 ? li : -li);
         }
       bfaces.insert(bfaces.end(), loc.begin(), loc.end());
       boff[b + 1] = int(bfaces.size());
@@ -1130,7 +1191,7 @@ int setup_device(fvs_ctx &C) {
       if (ok) {
         for (size_t q = 0; q < dsx.size(); ++q) C.shifts.sx[q] = dsx[q], C.shifts.sy[q] = dsy[q];
         std::vector<double2> cxy(N);
-        std::vector<double> gi(4 * size_t(N), 0.0);
+        std::vector<double> gi(3 * size_t(N), 0.0);  // planes: G^-1 11, 12, 22
         const double p = C.opt.lsq_p;
         for (int i = 0; i < N; ++i) {
           const int c = P[i];
@@ -1155,16 +1216,18 @@ int setup_device(fvs_ctx &C) {
             g22 = g22 + d2 * d2;
           }
           const double det = g11 * g22 - g12 * g12;
-          gi[4 * size_t(i)] = 1.0 / det * g22;
-          gi[4 * size_t(i) + 1] = -(1.0 / det * g12);
-          gi[4 * size_t(i) + 2] = 1.0 / det * g11;
+          gi[size_t(i)] = 1.0 / det * g22;
+          gi[size_t(N) + i] = -(1.0 / det * g12);
+          gi[2 * size_t(N) + i] = 1.0 / det * g11;
         }
         C.pmode = (p == 0.0) ? 0 : (p == 1.0 ? 1 : (p == 2.0 ? 2 : 3));
         if (int rc = C.upload(C.cxy, cxy)) return rc;
         if (int rc = C.upload(C.ginv, gi)) return rc;
         if (int rc = C.upload(C.sid, code)) return rc;
         C.lsqc = true;
-        if (C.fused && C.opt.recon != 1 && C.opt.kernel_path != 3)
+        if (C.opt.recon != 1 && C.opt.kernel_path != 3)
+          if (int rc = setup_grad_halo(C, off, wid, code)) return rc;
+        if (C.fused && C.opt.recon != 1 && C.opt.kernel_path == 2)
           if (int rc = setup_stage(C, h_boff, h_bfaces, fic, fbc, off, wid, nb, code)) return rc;
       }
     }
@@ -1299,10 +1362,11 @@ int launch_gradient(fvs_ctx &C, bool with_limiter) {
   if (C.halo && C.use_halo) {
     if (C.nact) g = dim3(C.nact);
     const int nb = nblk(N);
-    const size_t lds = size_t(6) * C.max2n * sizeof(double);
+    const int nl = kBlock + C.maxh;
+    const size_t lds = size_t(6) * nl * sizeof(double);
 #define FVS_LSQH(L, PM)                                                                                          \
-  hipLaunchKernelGGL((k_grad_lsqh<L, PM>), g, b, lds, C.stream, N, xcd_arg(nb, C.xcd_grad), C.max2n, C.W, C.s2off, \
-                     C.s2list, C.sl0, C.slw, C.sleo, C.slid, C.cxy, C.ginv, C.shifts, C.opt.lsq_p, C.grad, C.phi, lt,  \
+  hipLaunchKernelGGL((k_grad_lsqh<L, PM>), g, b, lds, C.stream, N, xcd_arg(nb, C.xcd_grad), nl, C.W, C.ghoff,      \
+                     C.ghlist, C.soff, C.swid, C.lcode, C.cxy, C.ginv, C.shifts, C.opt.lsq_p, C.grad, C.phi, lt,     \
                      C.nvrt, C.loff, C.vol, C.nact ? C.blist : nullptr, C.frz_some ? C.dinst_off : nullptr, C.dfrz,    \
                      C.ninst)
     if (lim) {
@@ -2186,8 +2250,8 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   const double N = C->N, Fi = C->Fi, Fb = C->Fb;
   const bool lim = C->opt.limiter > 0 && C->opt.recon != 1;
   double bg = 0.0;
-  if (C->halo && C->use_halo) bg = N * (32 + 16 + 32 + 64) + 4.0 * double(C->nlist2) + 2.0 * double(C->nslid_own);
-  else if (C->lsqc) bg = N * (32 + 16 + 32 + 64) + 4.0 * m.lsq_ptr.back();
+  if (C->halo && C->use_halo) bg = N * (32 + 16 + 24 + 64) + 4.0 * double(C->nghl) + 2.0 * double(C->nlcode);
+  else if (C->lsqc) bg = N * (32 + 16 + 24 + 64) + 4.0 * m.lsq_ptr.back();
   else if (C->opt.grad_method == 3) bg = N * (32 + 64) + 20.0 * m.lsq_ptr.back();
   else if (C->opt.grad_method == 1) bg = N * (32 + 16 + 8 + 64) + N * C->NFW * 20.0;
   else bg = N * (32 + 16 + 8 + 64) + 20.0 * m.ggnb_ptr.back();
@@ -2213,13 +2277,13 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   // update kernel without the face-flux buffer
   const bool phi1 = C->opt.limiter == 0 && C->opt.recon != 1;
   const double bfu = double(C->nlisted) * (8 + 56) + N * (32 + 64 + (phi1 ? 0 : 8)) +
-                     N * (C->GW * 4.0 + 8 + 32 + state);
+                     N * (C->GW * 2.0 + 8 + 32 + state);
   // whole stage: own states, centres and G^-1 once, the per-block cell lists
   // and local stencils, face records and block lists once, limiter inputs,
   // the update's cell-side traffic
-  const double bst = N * (32 + 16 + 32) + 2.0 * double(C->nslid) + 4.0 * double(C->nlist2) + (Fi + Fb) * 56 +
+  const double bst = N * (32 + 16 + 24) + 2.0 * double(C->nslid) + 4.0 * double(C->nlist2) + (Fi + Fb) * 56 +
                      8.0 * double(C->nlisted) + (lim ? N * (8 + 4) + 16.0 * (3 * m.ntri + 4 * m.nquad) : 0.0) +
-                     N * (C->GW * 4.0 + 8 + 32 + state);
+                     N * (C->GW * 2.0 + 8 + 32 + state);
   const double b5[KC_N] = {bg, bfi, bfb, bup, bno, C->stage ? bst : bfu};
   for (int k = 0; k < KC_N; ++k) {
     ms[k] = cnt[k] ? tot[k] / cnt[k] : 0.0;
