@@ -417,55 +417,53 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
 // indices blf[f] = c1 | c2 << 16 (c2 = 0xffff on a boundary face).
 // phisrc: 0 -> limiter values 1 (no limiter), 1 -> read phi (frozen); with
 // LIM the limiter is evaluated and written for the block's own cells.
-// Gradient of local cell q (global id) from states and centres staged in
-// LDS, with its stencil as local codes (index << 4 | periodic shift) in a
-// 64-cell slice starting at sl (width wid).  Same arithmetic and order as
-// k_grad_lsqc.  Returns the limiter value: evaluated with LIM (unless the
-// cell's instance is frozen; written to phi when own), else phi[id] when
-// phisrc is 1, else 1.
+// One stencil entry of an LDS-staged LSQ gradient: local code (index << 4 |
+// periodic shift) of neighbour j of the cell at centre ci with state w.
+// Same arithmetic as k_grad_lsqc's accumulate.
 template <int LIM, int PMODE>
-__device__ __forceinline__ double lds_gradient(int q, int id, int N, const double *__restrict__ lW,
-                                               const double2 *__restrict__ lX, const unsigned short *__restrict__ sl,
-                                               int wid, const ShiftTable &st, double p,
-                                               const double *__restrict__ ginv, double g8[8], int phisrc,
-                                               double *__restrict__ phi, int lt, const int *__restrict__ nvrt,
-                                               const double2 *__restrict__ loff, const double *__restrict__ vol,
-                                               const int *__restrict__ ioff, const int *__restrict__ frz, int ninst,
-                                               bool own) {
-  const int l = q % kSlice;
-  const double2 ci = lX[q];
-  double w[4], sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, mn[4], mx[4];
-  ld4(lW, q, w);
-  for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
-  for (int k = 0; k < wid; ++k) {
-    const unsigned code = sl[long(k) * kSlice + l];
-    const int j = int(code >> 4), sh = int(code & 15u);
-    const double2 cj = lX[j];
-    double q4[4];
-    ld4(lW, j, q4);
-    double xj = cj.x, yj = cj.y;
-    if (sh) {
-      xj = xj + st.sx[sh - 1];
-      yj = yj + st.sy[sh - 1];
-    }
-    const double dx = xj - ci.x, dy = yj - ci.y;
-    double w2 = 1.0;
-    if (PMODE != 0) {
-      const double r2 = dx * dx + dy * dy;
-      if (PMODE == 1) w2 = (r2 > 0.0) ? 1.0 / r2 : 0.0;
-      else if (PMODE == 2) w2 = (r2 > 0.0) ? 1.0 / (r2 * r2) : 0.0;
-      else w2 = (r2 > 0.0) ? pow(r2, -p) : 0.0;
-    }
-    for (int v = 0; v < 4; ++v) {
-      const double tt = w2 * (q4[v] - w[v]);
-      sx[v] = sx[v] + tt * dx;
-      sy[v] = sy[v] + tt * dy;
-      if (LIM) {
-        mn[v] = (q4[v] < mn[v]) ? q4[v] : mn[v];
-        mx[v] = (q4[v] > mx[v]) ? q4[v] : mx[v];
-      }
+__device__ __forceinline__ void lds_accum(unsigned code, const double2 ci, const double w[4],
+                                          const double *__restrict__ lW, const double2 *__restrict__ lX,
+                                          const ShiftTable &st, double p, double sx[4], double sy[4], double mn[4],
+                                          double mx[4]) {
+  const int j = int(code >> 4), sh = int(code & 15u);
+  const double2 cj = lX[j];
+  double q4[4];
+  ld4(lW, j, q4);
+  double xj = cj.x, yj = cj.y;
+  if (sh) {
+    xj = xj + st.sx[sh - 1];
+    yj = yj + st.sy[sh - 1];
+  }
+  const double dx = xj - ci.x, dy = yj - ci.y;
+  double w2 = 1.0;
+  if (PMODE != 0) {
+    const double r2 = dx * dx + dy * dy;
+    if (PMODE == 1) w2 = (r2 > 0.0) ? 1.0 / r2 : 0.0;
+    else if (PMODE == 2) w2 = (r2 > 0.0) ? 1.0 / (r2 * r2) : 0.0;
+    else w2 = (r2 > 0.0) ? pow(r2, -p) : 0.0;
+  }
+  for (int v = 0; v < 4; ++v) {
+    const double tt = w2 * (q4[v] - w[v]);
+    sx[v] = sx[v] + tt * dx;
+    sy[v] = sy[v] + tt * dy;
+    if (LIM) {
+      mn[v] = (q4[v] < mn[v]) ? q4[v] : mn[v];
+      mx[v] = (q4[v] > mx[v]) ? q4[v] : mx[v];
     }
   }
+}
+
+// G^-1 product and limiter of a gathered cell.  Returns the limiter value:
+// evaluated with LIM (unless the cell's instance is frozen; written to phi
+// when own), else phi[id] when phisrc is 1, else 1.
+template <int LIM>
+__device__ __forceinline__ double grad_finish(int id, int N, const double w[4], const double sx[4],
+                                              const double sy[4], const double mn[4], const double mx[4],
+                                              const double *__restrict__ ginv, double g8[8], int phisrc,
+                                              double *__restrict__ phi, int lt, const int *__restrict__ nvrt,
+                                              const double2 *__restrict__ loff, const double *__restrict__ vol,
+                                              const int *__restrict__ ioff, const int *__restrict__ frz, int ninst,
+                                              bool own) {
   const double G11 = ginv[id], G12 = ginv[N + id], G22 = ginv[2 * long(N) + id];
   double gx[4], gy[4];
   for (int v = 0; v < 4; ++v) {
@@ -492,16 +490,44 @@ __device__ __forceinline__ double lds_gradient(int q, int id, int N, const doubl
   return phisrc == 1 ? phi[id] : 1.0;
 }
 
+// Gradient of local cell q (global id) from states and centres staged in
+// LDS, with its stencil as local codes (index << 4 | periodic shift) in a
+// 64-cell slice starting at sl (width wid).  Same arithmetic and order as
+// k_grad_lsqc.
+template <int LIM, int PMODE>
+__device__ __forceinline__ double lds_gradient(int q, int id, int N, const double *__restrict__ lW,
+                                               const double2 *__restrict__ lX, const unsigned short *__restrict__ sl,
+                                               int wid, const ShiftTable &st, double p,
+                                               const double *__restrict__ ginv, double g8[8], int phisrc,
+                                               double *__restrict__ phi, int lt, const int *__restrict__ nvrt,
+                                               const double2 *__restrict__ loff, const double *__restrict__ vol,
+                                               const int *__restrict__ ioff, const int *__restrict__ frz, int ninst,
+                                               bool own) {
+  const int l = q % kSlice;
+  const double2 ci = lX[q];
+  double w[4], sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, mn[4], mx[4];
+  ld4(lW, q, w);
+  for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
+  for (int k = 0; k < wid; ++k)
+    lds_accum<LIM, PMODE>(sl[long(k) * kSlice + l], ci, w, lW, lX, st, p, sx, sy, mn, mx);
+  return grad_finish<LIM>(id, N, w, sx, sy, mn, mx, ginv, g8, phisrc, phi, lt, nvrt, loff, vol, ioff, frz, ninst,
+                          own);
+}
+
 // LSQ gradient (and limiter) of the block's own cells with the cells its
 // stencils read staged in LDS first: the block's own cells (coalesced) at
 // local 0..255, the halo (the outside cells its stencils read, one gather
-// each) from 256 on; the stencils as 16-bit local entries.  Bit-identical to
-// k_grad_lsqc.
+// each) from 256 on.  The stencils are 16-bit local codes, two per 32-bit
+// word (entries 2kk and 2kk+1 of lane l of slice s at lcode[poff[s] + 64 kk
+// + l]); a thread loads the first kGradRegs of its own while the block is
+// staged, so the LDS loop waits on no global load.  Same arithmetic and order
+// as k_grad_lsqc, so the gradients are bit-identical to it.
+constexpr int kGradRegs = 16;
 template <int LIM, int PMODE>
 __global__ __launch_bounds__(kBlock) void k_grad_lsqh(int N, int nb, int nl, const double *__restrict__ W,
                                                       const int *__restrict__ ghoff, const int *__restrict__ ghlist,
-                                                      const int *__restrict__ soff, const int *__restrict__ swid,
-                                                      const unsigned short *__restrict__ lcode,
+                                                      const int *__restrict__ poff, const int *__restrict__ swid,
+                                                      const unsigned *__restrict__ lcode,
                                                       const double2 *__restrict__ cxy,
                                                       const double *__restrict__ ginv, ShiftTable st, double p,
                                                       double *__restrict__ grad, double *__restrict__ phi, int lt,
@@ -515,8 +541,14 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqh(int N, int nb, int nl, con
   const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
   if (lb >= (nb < 0 ? -nb : nb)) return;  // the whole workgroup leaves before the barrier
   const int i0 = lb * kBlock, id = i0 + threadIdx.x;
+  const bool act = id < N;
+  const int s = id / kSlice, wid = act ? swid[s] : 0;
+  const unsigned *lc = lcode + (act ? poff[s] : 0) + threadIdx.x % kSlice;
+  unsigned cr[kGradRegs / 2];
+#pragma unroll
+  for (int kk = 0; kk < kGradRegs / 2; ++kk) cr[kk] = (2 * kk < wid) ? lc[kk * kSlice] : 0u;
   const int o2 = ghoff[lb], nh = ghoff[lb + 1] - o2;
-  if (id < N) {
+  if (act) {
     double w[4];
     ld4(W, id, w);
     st4(lW, threadIdx.x, w);
@@ -530,11 +562,18 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqh(int N, int nb, int nl, con
     lX[kBlock + q] = cxy[j];
   }
   __syncthreads();
-  if (id >= N) return;
-  const int s = id / kSlice;
+  if (!act) return;
+  const double2 ci = lX[threadIdx.x];
+  double w[4], sx[4] = {0, 0, 0, 0}, sy[4] = {0, 0, 0, 0}, mn[4], mx[4];
+  ld4(lW, threadIdx.x, w);
+  for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
+#pragma unroll
+  for (int k = 0; k < kGradRegs; ++k)
+    if (k < wid) lds_accum<LIM, PMODE>((cr[k >> 1] >> (16 * (k & 1))) & 0xffffu, ci, w, lW, lX, st, p, sx, sy, mn, mx);
+  for (int k = kGradRegs; k < wid; ++k)
+    lds_accum<LIM, PMODE>((lc[(k >> 1) * kSlice] >> (16 * (k & 1))) & 0xffffu, ci, w, lW, lX, st, p, sx, sy, mn, mx);
   double g8[8];
-  lds_gradient<LIM, PMODE>(threadIdx.x, id, N, lW, lX, lcode + soff[s], swid[s], st, p, ginv, g8, 0, phi, lt, nvrt,
-                           loff, vol, ioff, frz, ninst, true);
+  grad_finish<LIM>(id, N, w, sx, sy, mn, mx, ginv, g8, 0, phi, lt, nvrt, loff, vol, ioff, frz, ninst, true);
   st8(grad, id, g8);
 }
 

@@ -682,11 +682,12 @@ struct fvs_ctx {
   unsigned *blf = nullptr;
   long nslid = 0, nlist2 = 0, nslid_own = 0;
   // staged-halo gradient (k_grad_lsqh): per block, the cells outside it that
-  // its stencils read (ascending), and every stencil entry as a local index
-  // (own cells 0..255, halo 256..) << 4 | periodic-shift code, laid out like
-  // the global stencil (sliced ELL, same offsets and widths)
-  int *ghoff = nullptr, *ghlist = nullptr;
-  unsigned short *lcode = nullptr;
+  // its stencils read (ascending), and every stencil entry as a 16-bit local
+  // index (own cells 0..255, halo 256..) << 4 | periodic-shift code, two
+  // entries per word: sliced ELL with the global stencil's widths, entries
+  // 2kk, 2kk+1 of lane l of slice s at lcode[poff[s] + 64 kk + l]
+  int *ghoff = nullptr, *ghlist = nullptr, *poff = nullptr;
+  unsigned *lcode = nullptr;
   int maxh = 0;  // largest halo of a block
   long nghl = 0, nlcode = 0;
   // batch mode: per-instance frozen flags (dfrz) while only some are frozen
@@ -941,8 +942,9 @@ int setup_stage(fvs_ctx &C, const std::vector<int> &boff, const std::vector<int>
 int setup_grad_halo(fvs_ctx &C, const std::vector<int> &off, const std::vector<int> &wid,
                     const std::vector<unsigned> &code) {
   const int N = C.N, nbk = nblk(N);
-  std::vector<int> ghoff(nbk + 1, 0), ghlist, halo;
-  std::vector<unsigned short> lcode(code.size(), 0);
+  std::vector<int> ghoff(nbk + 1, 0), ghlist, halo, poff(wid.size() + 1, 0);
+  for (size_t sg = 0; sg < wid.size(); ++sg) poff[sg + 1] = poff[sg] + (wid[sg] + 1) / 2 * kSlice;
+  std::vector<unsigned> lcode(size_t(poff.back()), 0u);
   int maxh = 0;
   for (int b = 0; b < nbk; ++b) {
     const int i0 = b * kBlock, i1 = std::min(N, i0 + kBlock);
@@ -966,7 +968,8 @@ int setup_grad_halo(fvs_ctx &C, const std::vector<int> &off, const std::vector<i
         if (j < i0 || j >= i1) loc = kBlock + int(std::lower_bound(halo.begin(), halo.end(), j) - halo.begin());
         if (loc < 0 || loc >= kBlock + int(halo.size()) || (loc < kBlock && i0 + loc >= i1))
           return fail(FVS_EINVAL, "halo set-up: stencil entry outside the block's local cells");
-        lcode[e] = static_cast<unsigned short>((unsigned(loc) << 4) | (code[e] >> 27));
+        lcode[size_t(poff[sg]) + size_t(k / 2) * kSlice + l] |= ((unsigned(loc) << 4) | (code[e] >> 27))
+                                                                << (16 * (k & 1));
       }
     }
     ghlist.insert(ghlist.end(), halo.begin(), halo.end());
@@ -977,11 +980,11 @@ int setup_grad_halo(fvs_ctx &C, const std::vector<int> &off, const std::vector<i
   if (int rc = C.upload(C.ghoff, ghoff)) return rc;
   if (ghlist.empty()) ghlist.push_back(0);
   if (int rc = C.upload(C.ghlist, ghlist)) return rc;
+  if (int rc = C.upload(C.poff, poff)) return rc;
   if (int rc = C.upload(C.lcode, lcode)) return rc;
   C.maxh = maxh;
   C.nghl = ghoff[nbk];
-  C.nlcode = 0;
-  for (int w : wid) C.nlcode += long(w) * kSlice;
+  C.nlcode = poff.back();
   C.halo = true;
   return 0;
 }
@@ -1366,7 +1369,7 @@ int launch_gradient(fvs_ctx &C, bool with_limiter) {
     const size_t lds = size_t(6) * nl * sizeof(double);
 #define FVS_LSQH(L, PM)                                                                                          \
   hipLaunchKernelGGL((k_grad_lsqh<L, PM>), g, b, lds, C.stream, N, xcd_arg(nb, C.xcd_grad), nl, C.W, C.ghoff,      \
-                     C.ghlist, C.soff, C.swid, C.lcode, C.cxy, C.ginv, C.shifts, C.opt.lsq_p, C.grad, C.phi, lt,     \
+                     C.ghlist, C.poff, C.swid, C.lcode, C.cxy, C.ginv, C.shifts, C.opt.lsq_p, C.grad, C.phi, lt,     \
                      C.nvrt, C.loff, C.vol, C.nact ? C.blist : nullptr, C.frz_some ? C.dinst_off : nullptr, C.dfrz,    \
                      C.ninst)
     if (lim) {
@@ -2250,7 +2253,7 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   const double N = C->N, Fi = C->Fi, Fb = C->Fb;
   const bool lim = C->opt.limiter > 0 && C->opt.recon != 1;
   double bg = 0.0;
-  if (C->halo && C->use_halo) bg = N * (32 + 16 + 24 + 64) + 4.0 * double(C->nghl) + 2.0 * double(C->nlcode);
+  if (C->halo && C->use_halo) bg = N * (32 + 16 + 24 + 64) + 4.0 * double(C->nghl) + 4.0 * double(C->nlcode);
   else if (C->lsqc) bg = N * (32 + 16 + 24 + 64) + 4.0 * m.lsq_ptr.back();
   else if (C->opt.grad_method == 3) bg = N * (32 + 64) + 20.0 * m.lsq_ptr.back();
   else if (C->opt.grad_method == 1) bg = N * (32 + 16 + 8 + 64) + N * C->NFW * 20.0;
