@@ -228,26 +228,48 @@ __device__ __forceinline__ void block_sum4_t(double v[4], double *red) {
   for (int k = 0; k < 4; ++k) v[k] = red[k * NT];
 }
 
+// The cell-side inputs of the stage update, loaded before the face fluxes
+// are complete (their latency overlaps the barrier): the cell's gather
+// entries (at most 4 faces), its volume, the step-start state and the stage
+// state or accumulator the update reads, and its local time step.
+struct UpdIn {
+  int sv[4];
+  double V, dl;
+  double u0[4], x[4];  // x: SSP-RK3 stage state, else accumulator (0 on the first stage)
+};
+
+__device__ __forceinline__ void block_update_load(int N, int i, int gw, const Upd &A, const short *__restrict__ lgat,
+                                                  const double *__restrict__ vol, const double *__restrict__ Ua,
+                                                  const double *__restrict__ Ub, const double *__restrict__ Facc,
+                                                  const double *__restrict__ dtl, UpdIn &in) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) in.sv[k] = (k < gw) ? lgat[long(k) * N + i] : 0;
+  in.V = vol[i];
+  in.dl = (A.steady && !A.first) ? dtl[i] : 0.0;
+  ld4(Ua, i, in.u0);
+  if (A.mode == M_SSP3) {
+    if (A.first) for (int v = 0; v < 4; ++v) in.x[v] = in.u0[v];
+    else ld4(Ub, i, in.x);
+  } else {
+    if (A.first) for (int v = 0; v < 4; ++v) in.x[v] = 0.0;
+    else ld4(Facc, i, in.x);
+  }
+}
+
 template <int NT>
-__device__ __forceinline__ void block_update(int N, int lb, int gw, const Upd &A, const double *__restrict__ lF,
-                                             const double *__restrict__ lS, const short *__restrict__ lgat,
-                                             const double *__restrict__ vol, double *__restrict__ Ua,
-                                             double *__restrict__ Ub, double *__restrict__ Facc,
-                                             double *__restrict__ Wout, double *__restrict__ dtl,
-                                             double *__restrict__ part, double *__restrict__ d2out,
-                                             double *__restrict__ red) {
+__device__ __forceinline__ void block_update_finish(int N, int lb, const Upd &A, const UpdIn &in,
+                                                    const double *__restrict__ lF, const double *__restrict__ lS,
+                                                    double *__restrict__ Ua, double *__restrict__ Ub,
+                                                    double *__restrict__ Facc, double *__restrict__ Wout,
+                                                    double *__restrict__ dtl, double *__restrict__ part,
+                                                    double *__restrict__ d2out, double *__restrict__ red) {
   const int i = lb * kBlock + threadIdx.x;
   double d2[4] = {0, 0, 0, 0};
   if (threadIdx.x < kBlock && i < N) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0}, wacc = 0.0;
-    // all of the cell's gather entries (at most 4 faces) are loaded at once
-    int sv[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) sv[k] = (k < gw) ? lgat[long(k) * N + i] : 0;
-    const double V = vol[i];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int s = sv[k];
+      const int s = in.sv[k];
       if (s == 0) break;
       const int f = (s > 0 ? s : -s) - 1;
       if (s > 0)
@@ -256,6 +278,7 @@ __device__ __forceinline__ void block_update(int N, int lb, int gw, const Upd &A
         for (int v = 0; v < 4; ++v) acc[v] = acc[v] - lF[4 * f + v];
       if (A.steady && A.first) wacc = wacc + lS[f];
     }
+    const double V = in.V;
     double R[4];
     for (int v = 0; v < 4; ++v) R[v] = -acc[v] / V;
     double hs = A.h;
@@ -265,20 +288,17 @@ __device__ __forceinline__ void block_update(int N, int lb, int gw, const Upd &A
         dl = A.cfl * V / (0.5 * wacc);
         dtl[i] = dl;
       } else {
-        dl = dtl[i];
+        dl = in.dl;
       }
       hs = dl * A.h;
     }
-    double u0[4], un[4];
-    ld4(Ua, i, u0);
+    const double *u0 = in.u0;
+    double un[4];
     if (A.mode == M_SSP3) {
-      double uc[4];
-      if (A.first) for (int v = 0; v < 4; ++v) uc[v] = u0[v];
-      else ld4(Ub, i, uc);
+      const double *uc = in.x;
       for (int v = 0; v < 4; ++v) un[v] = A.a * u0[v] + A.b * (uc[v] + hs * R[v]);
     } else if (A.mode == M_SSP42) {
-      double fo[4] = {0, 0, 0, 0};
-      if (!A.first) ld4(Facc, i, fo);
+      const double *fo = in.x;
       for (int v = 0; v < 4; ++v) un[v] = u0[v] + hs * (A.coef * R[v] + fo[v]);
       if (!A.last) {
         double fn[4];
@@ -286,8 +306,8 @@ __device__ __forceinline__ void block_update(int N, int lb, int gw, const Upd &A
         st4(Facc, i, fn);
       }
     } else {
-      double fo[4] = {0, 0, 0, 0}, fn[4];
-      if (!A.first) ld4(Facc, i, fo);
+      const double *fo = in.x;
+      double fn[4];
       for (int v = 0; v < 4; ++v) fn[v] = fo[v] + A.coef * R[v];
       if (A.use_acc) for (int v = 0; v < 4; ++v) un[v] = u0[v] + hs * fn[v];
       else for (int v = 0; v < 4; ++v) un[v] = u0[v] + hs * R[v];
@@ -395,8 +415,11 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
     for (int v = 0; v < 4; ++v) lF[4 * f + v] = fa[v];
     if (ws) lS[f] = wsa;
   }
+  UpdIn in;
+  const int ic = i0 + threadIdx.x;
+  if (ic < N) block_update_load(N, ic, gw, A, lgat, vol, Ua, Ub, Facc, dtl, in);
   __syncthreads();
-  block_update<kBlock>(N, lb, gw, A, lF, lS, lgat, vol, Ua, Ub, Facc, Wout, dtl, part, d2out, lF);
+  block_update_finish<kBlock>(N, lb, A, in, lF, lS, Ua, Ub, Facc, Wout, dtl, part, d2out, lF);
 }
 
 // Whole RK stage in one kernel per 256-cell block: the block stages, in LDS,
@@ -645,7 +668,12 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(
   }
   __syncthreads();
   // the staged states are no longer read: their LDS holds the reduction
-  block_update<kStageThreads>(N, lb, gw, A, lF, lS, lgat, vol, Ua, Ub, Facc, Wout, dtl, part, d2out, lds);
+  {
+    UpdIn in;
+    const int ic = lb * kBlock + threadIdx.x;
+    if (threadIdx.x < kBlock && ic < N) block_update_load(N, ic, gw, A, lgat, vol, Ua, Ub, Facc, dtl, in);
+    block_update_finish<kStageThreads>(N, lb, A, in, lF, lS, Ua, Ub, Facc, Wout, dtl, part, d2out, lds);
+  }
 }
 
 // per-instance residual norms of one step from the per-cell squared
