@@ -163,11 +163,29 @@ def main():
     ap.add_argument("--steady-train", type=int, default=256)
     ap.add_argument("--steady-test", type=int, default=64)
     ap.add_argument("--steady-seq", type=int, default=8)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only stream this many MiB through a plain read-once/write-once kernel (x1.0 "
+                         "multiply), 3 times, and print its rate: the calibration of the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the multi-GPU "
               "decomposition is a design, not an implementation", file=sys.stderr)
         sys.exit(2)
+    if args.copy_mib:
+        import torch
+        n = args.copy_mib * (1 << 20) // 8
+        src = torch.full((n,), 1.5, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / 3
+        print(json.dumps({"copy_mib_each_way": args.copy_mib, "ms": round(sec * 1e3, 4),
+                          "TBps_read_plus_write": round(2 * n * 8 / sec / 1e12, 3)}))
+        return
 
     from fvs2d_amd.solver import Solver
 
