@@ -412,6 +412,108 @@ __global__ __launch_bounds__(256) void k_gemm_tn(int K, int Kreal, int chunk, co
   if (kt == 0) Gs[offb + tid] = db;
 }
 
+// The same weight-gradient partials with four wave groups per workgroup
+// (1024 threads): group g multiplies slabs g, g + 4, g + 8, ... of the row
+// slice into its own accumulators and LDS buffers, so a CU keeps 16
+// wavefronts in flight instead of 4; the groups' sums are then added into
+// group 0's in group order (((g0 + g1) + g2) + g3), through LDS, one row
+// block at a time.  Needs chunk to be a multiple of 4 kBK.
+constexpr int kTnGroups = 4;
+constexpr int kTnBuf = (kRows + kW) * kLds;  // one group's A and D slabs (elements)
+__global__ __launch_bounds__(256 * kTnGroups) void k_gemm_tn_g(int K, int Kreal, int chunk, const u16 *__restrict__ A,
+                                                               const u16 *__restrict__ D, float *__restrict__ G,
+                                                               long P, long offW, long offb) {
+  __shared__ __attribute__((aligned(16))) u16 smem[kTnGroups * kTnBuf];
+  const int tid = threadIdx.x & 255, grp = threadIdx.x >> 8, lane = tid & 63, w = tid >> 6;
+  u16 *As = smem + grp * kTnBuf, *Bs = As + kRows * kLds;
+  const int nkt = K / kRows;
+  const int kt = blockIdx.x % nkt, s = blockIdx.x / nkt;
+  const long m0 = long(s) * chunk + grp * kBK;
+  const int kb = kt * kRows;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float db = 0.f;
+  const int amp = tid & 15, akg = tid >> 4;
+  const u16 *Ap = A + (m0 + 2 * amp) * K + kb + 8 * akg;
+  const int bmp0 = tid & 15, bng0 = tid >> 4, bng1 = (tid + 256) >> 4;
+  const u16 *Dp0 = D + (m0 + 2 * bmp0) * kW + 8 * bng0;
+  const u16 *Dp1 = D + (m0 + 2 * bmp0) * kW + 8 * bng1;
+  uint4 ra0{}, ra1{}, rb00, rb01, rb10, rb11;
+  if (tid < 128) {
+    ra0 = *reinterpret_cast<const uint4 *>(Ap);
+    ra1 = *reinterpret_cast<const uint4 *>(Ap + K);
+  }
+  rb00 = *reinterpret_cast<const uint4 *>(Dp0);
+  rb01 = *reinterpret_cast<const uint4 *>(Dp0 + kW);
+  rb10 = *reinterpret_cast<const uint4 *>(Dp1);
+  rb11 = *reinterpret_cast<const uint4 *>(Dp1 + kW);
+  constexpr int stride = kTnGroups * kBK;
+  for (int mm = 0; mm < chunk; mm += stride) {
+    if (tid < 128) put_pairs(As, 8 * akg, amp, ra0, ra1);
+    put_pairs(Bs, 8 * bng0, bmp0, rb00, rb01);
+    put_pairs(Bs, 8 * bng1, bmp0, rb10, rb11);
+    __syncthreads();
+    if (mm + stride < chunk) {
+      const long o = long(mm + stride);
+      if (tid < 128) {
+        ra0 = *reinterpret_cast<const uint4 *>(Ap + o * K);
+        ra1 = *reinterpret_cast<const uint4 *>(Ap + (o + 1) * K);
+      }
+      rb00 = *reinterpret_cast<const uint4 *>(Dp0 + o * kW);
+      rb01 = *reinterpret_cast<const uint4 *>(Dp0 + (o + 1) * kW);
+      rb10 = *reinterpret_cast<const uint4 *>(Dp1 + o * kW);
+      rb11 = *reinterpret_cast<const uint4 *>(Dp1 + (o + 1) * kW);
+    }
+    if (kt == 0) {
+      const u16 *row = Bs + tid * kLds;
+      for (int c = 0; c < kBK; ++c) db += bf2f(row[c]);
+    }
+    mma_slab(As, Bs, lane, w, acc);
+    __syncthreads();
+  }
+  // groups 1..3 hand their sums to group 0 through LDS, one row block i at a time
+  float *red = reinterpret_cast<float *>(smem);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (grp > 0)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        *reinterpret_cast<f32x4 *>(red + (long((grp - 1) * 256 + tid) * 16 + 4 * j)) = acc[i][j];
+    __syncthreads();
+    if (grp == 0)
+      for (int g2 = 1; g2 < kTnGroups; ++g2)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const f32x4 o = *reinterpret_cast<const f32x4 *>(red + (long((g2 - 1) * 256 + tid) * 16 + 4 * j));
+          acc[i][j] = acc[i][j] + o;
+        }
+    __syncthreads();
+  }
+  if (kt == 0) {
+    if (grp > 0) red[(grp - 1) * 256 + tid] = db;
+    __syncthreads();
+    if (grp == 0)
+      for (int g2 = 1; g2 < kTnGroups; ++g2) db += red[(g2 - 1) * 256 + tid];
+  }
+  if (grp != 0) return;
+  float *Gs = G + long(s) * P;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int k = kb + 16 * i + (lane & 15);
+    if (k >= Kreal) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n0 = 64 * w + 16 * j + 4 * (lane >> 4);
+      *reinterpret_cast<float4 *>(Gs + offW + long(k) * kW + n0) =
+          make_float4(acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
+    }
+  }
+  if (kt == 0) Gs[offb + tid] = db;
+}
+
 // ------------------------------------------------------------ output layer
 constexpr int kHeadRows = 128;                 // rows per workgroup
 constexpr int kHeadPart = kW * kOut + kOut;    // output-layer parameters
@@ -686,8 +788,12 @@ int run_pass(fvs_mlp &m, int M, long nvalid, bool grad, float *Y) {
         Prof pf(m, MC_WGRAD);
         const u16 *A = (l == 0) ? m.X0 : m.H[l - 1];
         const int K = m.tab.inpad[l];
-        hipLaunchKernelGGL(k_gemm_tn, dim3((K / kRows) * S), dim3(256), 0, m.stream, K, m.tab.in[l], chunk, A, cur,
-                           m.G, m.P, m.tab.offW[l], m.tab.offb[l]);
+        if (chunk % (kTnGroups * kBK) == 0)
+          hipLaunchKernelGGL(k_gemm_tn_g, dim3((K / kRows) * S), dim3(256 * kTnGroups), 0, m.stream, K,
+                             m.tab.in[l], chunk, A, cur, m.G, m.P, m.tab.offW[l], m.tab.offb[l]);
+        else
+          hipLaunchKernelGGL(k_gemm_tn, dim3((K / kRows) * S), dim3(256), 0, m.stream, K, m.tab.in[l], chunk, A,
+                             cur, m.G, m.P, m.tab.offW[l], m.tab.offb[l]);
       }
       if (l > 0) {
         Prof pf(m, MC_BWD);
