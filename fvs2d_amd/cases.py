@@ -77,13 +77,27 @@ def draw_instances(n: int, seed: int = 0, resolutions=RESOLUTIONS):
             for r, m, a, t in zip(res, mach, aoa, thick)]
 
 
+def stall_rule_for(d, freeze_stall, stall_scale: bool):
+    """The (window, factor) stall rule of instance ``d``: the window times
+    n_around / 64 with ``stall_scale`` (64 x 32 grids keep the window)."""
+    if freeze_stall is None:
+        return None
+    w, f = freeze_stall
+    return (int(w * d["n_around"] / 64) if stall_scale else int(w), float(f))
+
+
 def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, freeze_after: int,
               resolutions=RESOLUTIONS, sequential: int = 0, chunk: int = 250, graph: bool = True,
-              freeze_drop=None, freeze_stall=None):
+              freeze_drop=None, freeze_stall=None, stall_scale: bool = False):
     """Batched steady solves of drawn instances; optionally the first
-    ``sequential`` of them again one at a time for comparison."""
+    ``sequential`` of them again one at a time for comparison.  With
+    ``stall_scale`` the stall window grows with the grid: ``window * n_around
+    / 64`` steps per instance (the residual of an explicit pseudo-time scheme
+    needs a number of steps proportional to the cells across the domain)."""
     from .solver import BatchSolver
     inst = draw_instances(ninst, seed, resolutions)
+    stall_of = lambda d: stall_rule_for(d, freeze_stall, stall_scale)  # noqa: E731
+    batch_stall = None if freeze_stall is None else ([stall_of(d)[0] for d in inst], freeze_stall[1])
     meshes = [meshgen.naca_ogrid(d["n_around"], d["n_layers"], thickness=d["thickness"]) for d in inst]
     _, sch = naca(16, 8, cfl=cfl)
     t0 = time.perf_counter()
@@ -100,7 +114,7 @@ def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, fr
             print(f"batch: {done} steps, {active} instances active", file=sys.stderr, flush=True)
 
     res = b.steady_solve_batch(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after, freeze_drop=freeze_drop,
-                               freeze_stall=freeze_stall, progress=progress)
+                               freeze_stall=batch_stall, progress=progress)
     sec = time.perf_counter() - t0
     print(f"batch: {int((res['iterations'] > 0).sum())}/{ninst} converged in {sec:.1f} s", file=sys.stderr, flush=True)
     forces = b.wall_forces_batch()
@@ -123,7 +137,7 @@ def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, fr
             s.set_graph(graph)
             s.initialize_solution(1, 0.0)
             r = s.steady_solve(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after, freeze_at=freeze_drop,
-                               freeze_stall=freeze_stall)
+                               freeze_stall=stall_of(d))
             n_ok += r["iterations"] is not None
             s.close()
         sq = time.perf_counter() - t0
@@ -154,6 +168,7 @@ def main(argv=None):
     bt.add_argument("--freeze-after", type=int, default=1000)
     bt.add_argument("--freeze-drop", type=float, default=None)
     bt.add_argument("--freeze-stall", type=str, default="none", help="window,factor (e.g. 2000,0.1) or none")
+    bt.add_argument("--stall-scale", type=int, default=1, help="1: stall window x n_around/64 per instance")
     bt.add_argument("--resolutions", type=str, default="128x64,256x128,384x192,512x256")
     bt.add_argument("--sequential", type=int, default=0)
     bt.add_argument("--graph", type=int, default=1)
@@ -166,7 +181,8 @@ def main(argv=None):
         stall = None if a.freeze_stall == "none" else tuple(t(v) for t, v in zip((int, float),
                                                                                  a.freeze_stall.split(",")))
         out = run_batch(a.instances, a.seed, a.max_steps, a.drop, a.cfl, a.freeze_after, res, a.sequential,
-                        graph=bool(a.graph), freeze_drop=a.freeze_drop, freeze_stall=stall)
+                        graph=bool(a.graph), freeze_drop=a.freeze_drop, freeze_stall=stall,
+                        stall_scale=bool(a.stall_scale))
     print(json.dumps(out))
     return 0
 

@@ -512,7 +512,8 @@ class BatchSolver(Solver):
         norm is at most ``freeze_drop * r_ref[k]`` or after ``freeze_after`` of
         its own steps since its (re)start, or, with ``freeze_stall = (window,
         factor)``, once its norm at a chunk end is above ``factor`` times its
-        norm ``window`` steps earlier (it stalls).  It has diverged when a norm is
+        norm ``window`` steps earlier (it stalls); ``window`` may be one value
+        per instance.  It has diverged when a norm is
         non-finite or above ``diverge * r_ref[k]``; ``on_diverge(k)`` may then
         reset its state and return True to restart it, its step count running
         on, otherwise it is masked out.  Returns per-instance iterations (-1
@@ -520,6 +521,11 @@ class BatchSolver(Solver):
         step at which each limiter was frozen (-1: never).  ``progress(steps,
         active)`` (optional) is called after every chunk."""
         n = self.ninst
+        if freeze_stall is not None:
+            win = np.broadcast_to(np.asarray(freeze_stall[0], np.int64), (n,))
+
+            def stall_rule(k):
+                return int(win[k]), freeze_stall[1]
         self.set_active(np.ones(n, np.int32))
         iters = np.full(n, -1, np.int64)
         r_ref = None if r_ref is None else np.asarray(r_ref, np.float64)
@@ -563,7 +569,7 @@ class BatchSolver(Solver):
                 ends[k].append((int(since[k]), float(nr[-1, k])))
                 if not frozen[k] and ((freeze_drop is not None and nr[-1, k] <= freeze_drop * r_ref[k])
                                       or (freeze_after is not None and since[k] >= freeze_after)
-                                      or (freeze_stall is not None and _stalled(ends[k], freeze_stall))):
+                                      or (freeze_stall is not None and _stalled(ends[k], stall_rule(k)))):
                     frozen[k] = 1
                     frozen_at[k] = done + m
             for k in reset:

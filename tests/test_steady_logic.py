@@ -18,3 +18,12 @@ def test_stall_rule():
     assert not _stalled(ends, rule)          # 0.06 <= 0.1 * 1.0 (step 250)
     ends = [(250, 1.0), (500, 0.5), (1500, 0.06)]
     assert _stalled(ends, rule)              # 0.06 > 0.1 * 0.5 (step 500)
+
+
+def test_stall_window_scales_with_grid():
+    from fvs2d_amd.cases import stall_rule_for
+    d64, d512 = dict(n_around=64), dict(n_around=512)
+    assert stall_rule_for(d64, (2000, 0.1), True) == (2000, 0.1)
+    assert stall_rule_for(d512, (2000, 0.1), True) == (16000, 0.1)
+    assert stall_rule_for(d512, (2000, 0.1), False) == (2000, 0.1)
+    assert stall_rule_for(d512, None, True) is None
