@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -768,13 +769,15 @@ std::vector<int> morton_order(const MeshPrep &m, const std::vector<int> &inst) {
   return perm;
 }
 
-// sliced-ELL packing of a CSR stencil (cells in device order)
-void pack_sell(int N, const std::vector<int> &perm, const std::vector<int> &iperm, const std::vector<int> &ptr,
+// sliced-ELL packing of a CSR stencil (cells in device order); false when
+// the padded entry count does not fit the 32-bit slice offsets
+bool pack_sell(int N, const std::vector<int> &perm, const std::vector<int> &iperm, const std::vector<int> &ptr,
                const std::vector<int> &col, const double *val2, std::vector<int> &off, std::vector<int> &wid,
                std::vector<int> &nb, std::vector<double2> &cw) {
   const int ns = (N + kSlice - 1) / kSlice;
   off.assign(ns + 1, 0);
   wid.assign(ns, 0);
+  long total = 0;
   for (int s = 0; s < ns; ++s) {
     int w = 0;
     for (int l = 0; l < kSlice; ++l) {
@@ -782,7 +785,9 @@ void pack_sell(int N, const std::vector<int> &perm, const std::vector<int> &iper
       if (i < N) w = std::max(w, ptr[perm[i] + 1] - ptr[perm[i]]);
     }
     wid[s] = w;
-    off[s + 1] = off[s] + w * kSlice;
+    total += long(w) * kSlice;
+    if (total > long(INT32_MAX)) return false;
+    off[s + 1] = int(total);
   }
   nb.assign(off[ns], 0);
   cw.assign(val2 ? off[ns] : 0, make_double2(0.0, 0.0));
@@ -801,6 +806,7 @@ void pack_sell(int N, const std::vector<int> &perm, const std::vector<int> &iper
         }
       }
     }
+  return true;
 }
 
 // Per-block data of the whole-stage kernel (k_stage).  For block b: C1 =
@@ -943,7 +949,7 @@ int setup_grad_halo(fvs_ctx &C, const std::vector<int> &off, const std::vector<i
                     const std::vector<unsigned> &code) {
   const int N = C.N, nbk = nblk(N);
   std::vector<int> ghoff(nbk + 1, 0), ghlist, halo, poff(wid.size() + 1, 0);
-  for (size_t sg = 0; sg < wid.size(); ++sg) poff[sg + 1] = poff[sg] + (wid[sg] + 1) / 2 * kSlice;
+  for (size_t sg = 0; sg < wid.size(); ++sg) poff[sg + 1] = poff[sg] + (wid[sg] + 1) / 2 * kSlice;  // <= off
   std::vector<unsigned> lcode(size_t(poff.back()), 0u);
   int maxh = 0;
   for (int b = 0; b < nbk; ++b) {
@@ -1152,7 +1158,8 @@ int setup_device(fvs_ctx &C) {
     }
     std::vector<int> off, wid, nb;
     std::vector<double2> c2;
-    pack_sell(N, P, IP, m.lsq_ptr, m.lsq_cell, cw.data(), off, wid, nb, c2);
+    if (!pack_sell(N, P, IP, m.lsq_ptr, m.lsq_cell, cw.data(), off, wid, nb, c2))
+      return fail(FVS_EINVAL, "mesh too large: LSQ stencil entries exceed 32-bit offsets");
     if (int rc = C.upload(C.soff, off)) return rc;
     if (int rc = C.upload(C.swid, wid)) return rc;
     if (int rc = C.upload(C.snb, nb)) return rc;
@@ -1259,7 +1266,8 @@ int setup_device(fvs_ctx &C) {
     for (int i = 0; i < N; ++i) c0[i] = make_double2(m.ggnb_coef0[2 * P[i]], m.ggnb_coef0[2 * P[i] + 1]);
     std::vector<int> off, wid, nb;
     std::vector<double2> c2;
-    pack_sell(N, P, IP, m.ggnb_ptr, m.ggnb_cell, m.ggnb_c.data(), off, wid, nb, c2);
+    if (!pack_sell(N, P, IP, m.ggnb_ptr, m.ggnb_cell, m.ggnb_c.data(), off, wid, nb, c2))
+      return fail(FVS_EINVAL, "mesh too large: GGNB stencil entries exceed 32-bit offsets");
     if (int rc = C.upload(C.ggnb0, c0)) return rc;
     if (int rc = C.upload(C.goff, off)) return rc;
     if (int rc = C.upload(C.gwid, wid)) return rc;
