@@ -43,11 +43,13 @@ def naca(n_around: int = 512, n_layers: int = 256, mach: float = 0.8, aoa: float
 
 
 def run_steady(n_around, n_layers, mach, aoa, thickness, cfl, max_steps, drop, limiter=1, chunk=500,
-               freeze_at=None):
+               freeze_at=None, graph: bool = True):
     m, sch = naca(n_around, n_layers, mach, aoa, thickness, cfl, limiter=limiter)
     t0 = time.perf_counter()
     s = Solver(m, sch)
     s.initialize_solution(1, 0.0)
+    s.autotune(5)
+    s.set_graph(graph)
     setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     res = s.steady_solve(max_steps, drop=drop, chunk=chunk, freeze_at=freeze_at)
@@ -159,6 +161,7 @@ def main(argv=None):
     st.add_argument("--drop", type=float, default=1e-8)
     st.add_argument("--limiter", type=int, default=1)
     st.add_argument("--freeze-at", type=float, default=None)
+    st.add_argument("--graph", type=int, default=0, help="1: replay each chunk's steps as a HIP graph")
     bt = sub.add_parser("batch")
     bt.add_argument("--instances", type=int, default=16)
     bt.add_argument("--seed", type=int, default=0)
@@ -175,7 +178,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "steady":
         out = run_steady(a.n, a.layers, a.mach, a.aoa, a.thickness, a.cfl, a.max_steps, a.drop, a.limiter,
-                         freeze_at=a.freeze_at)
+                         freeze_at=a.freeze_at, graph=bool(a.graph))
     else:
         res = tuple(tuple(int(v) for v in r.split("x")) for r in a.resolutions.split(","))
         stall = None if a.freeze_stall == "none" else tuple(t(v) for t, v in zip((int, float),
