@@ -19,6 +19,43 @@ struct ShiftTable {
   double sx[15], sy[15];
 };
 
+// One stencil term of the LSQ gradient, shared by every gradient kernel
+// (k_grad_lsqc, k_grad_lsqh, k_stage): neighbour centre (xj, yj) and state
+// q, cell centre ci and state w.  The multiply-adds are written as explicit
+// FMAs, so the rounding is the same in every kernel whatever the compiler's
+// contraction choices in each of them (the kernels stay bit-identical).
+template <int LIM, int PMODE>
+__device__ __forceinline__ void lsq_term(double xj, double yj, const double2 ci, const double q[4],
+                                         const double w[4], double p, double sx[4], double sy[4], double mn[4],
+                                         double mx[4]) {
+  const double dx = xj - ci.x, dy = yj - ci.y;
+  double w2 = 1.0;
+  if (PMODE != 0) {
+    const double r2 = fma(dx, dx, dy * dy);
+    if (PMODE == 1) w2 = (r2 > 0.0) ? 1.0 / r2 : 0.0;
+    else if (PMODE == 2) w2 = (r2 > 0.0) ? 1.0 / (r2 * r2) : 0.0;
+    else w2 = (r2 > 0.0) ? pow(r2, -p) : 0.0;
+  }
+  for (int v = 0; v < 4; ++v) {
+    const double t = w2 * (q[v] - w[v]);
+    sx[v] = fma(t, dx, sx[v]);
+    sy[v] = fma(t, dy, sy[v]);
+    if (LIM) {
+      mn[v] = (q[v] < mn[v]) ? q[v] : mn[v];
+      mx[v] = (q[v] > mx[v]) ? q[v] : mx[v];
+    }
+  }
+}
+
+// gradient = G^-1 (sx, sy) with G^-1's entries 11, 12, 22 (explicit FMAs as above)
+__device__ __forceinline__ void lsq_apply_ginv(double G11, double G12, double G22, const double sx[4],
+                                               const double sy[4], double gx[4], double gy[4]) {
+  for (int v = 0; v < 4; ++v) {
+    gx[v] = fma(G11, sx[v], G12 * sy[v]);
+    gy[v] = fma(G12, sx[v], G22 * sy[v]);
+  }
+}
+
 // PMODE: 0 -> p = 0, 1 -> p = 1, 2 -> p = 2, 3 -> general p (pow)
 template <int LIM, int PMODE>
 __global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const double *__restrict__ W,
@@ -62,23 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const doubl
       xj = xj + st.sx[sh - 1];
       yj = yj + st.sy[sh - 1];
     }
-    const double dx = xj - ci.x, dy = yj - ci.y;
-    double w2 = 1.0;
-    if (PMODE != 0) {
-      const double r2 = dx * dx + dy * dy;
-      if (PMODE == 1) w2 = (r2 > 0.0) ? 1.0 / r2 : 0.0;
-      else if (PMODE == 2) w2 = (r2 > 0.0) ? 1.0 / (r2 * r2) : 0.0;
-      else w2 = (r2 > 0.0) ? pow(r2, -p) : 0.0;
-    }
-    for (int v = 0; v < 4; ++v) {
-      const double t = w2 * (q[v] - w[v]);
-      sx[v] = sx[v] + t * dx;
-      sy[v] = sy[v] + t * dy;
-      if (LIM) {
-        mn[v] = (q[v] < mn[v]) ? q[v] : mn[v];
-        mx[v] = (q[v] > mx[v]) ? q[v] : mx[v];
-      }
-    }
+    lsq_term<LIM, PMODE>(xj, yj, ci, q, w, p, sx, sy, mn, mx);
   };
   int k = 0;
   for (; k + 4 <= wid; k += 4) {
@@ -117,12 +138,8 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const doubl
     }
     accumulate(code, cj, q);
   }
-  const double G11 = ginv[i], G12 = ginv[N + i], G22 = ginv[2 * long(N) + i];
   double gx[4], gy[4];
-  for (int v = 0; v < 4; ++v) {
-    gx[v] = G11 * sx[v] + G12 * sy[v];
-    gy[v] = G12 * sx[v] + G22 * sy[v];
-  }
+  lsq_apply_ginv(ginv[i], ginv[N + i], ginv[2 * long(N) + i], sx, sy, gx, gy);
   double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
   st8(grad, i, g8);
   if (LIM) {
@@ -442,7 +459,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
 // LIM the limiter is evaluated and written for the block's own cells.
 // One stencil entry of an LDS-staged LSQ gradient: local code (index << 4 |
 // periodic shift) of neighbour j of the cell at centre ci with state w.
-// Same arithmetic as k_grad_lsqc's accumulate.
+// The term itself is the shared lsq_term.
 template <int LIM, int PMODE>
 __device__ __forceinline__ void lds_accum(unsigned code, const double2 ci, const double w[4],
                                           const double *__restrict__ lW, const double2 *__restrict__ lX,
@@ -457,41 +474,24 @@ __device__ __forceinline__ void lds_accum(unsigned code, const double2 ci, const
     xj = xj + st.sx[sh - 1];
     yj = yj + st.sy[sh - 1];
   }
-  const double dx = xj - ci.x, dy = yj - ci.y;
-  double w2 = 1.0;
-  if (PMODE != 0) {
-    const double r2 = dx * dx + dy * dy;
-    if (PMODE == 1) w2 = (r2 > 0.0) ? 1.0 / r2 : 0.0;
-    else if (PMODE == 2) w2 = (r2 > 0.0) ? 1.0 / (r2 * r2) : 0.0;
-    else w2 = (r2 > 0.0) ? pow(r2, -p) : 0.0;
-  }
-  for (int v = 0; v < 4; ++v) {
-    const double tt = w2 * (q4[v] - w[v]);
-    sx[v] = sx[v] + tt * dx;
-    sy[v] = sy[v] + tt * dy;
-    if (LIM) {
-      mn[v] = (q4[v] < mn[v]) ? q4[v] : mn[v];
-      mx[v] = (q4[v] > mx[v]) ? q4[v] : mx[v];
-    }
-  }
+  lsq_term<LIM, PMODE>(xj, yj, ci, q4, w, p, sx, sy, mn, mx);
 }
 
-// G^-1 product and limiter of a gathered cell.  Returns the limiter value:
-// evaluated with LIM (unless the cell's instance is frozen; written to phi
-// when own), else phi[id] when phisrc is 1, else 1.
+// G^-1 product (G = its 11, 12, 22 entries) and limiter of a gathered
+// cell.  Returns the limiter value: evaluated with LIM (unless the cell's
+// instance is frozen; written to phi when own), else phi[id] when phisrc is
+// 1, else 1.
 template <int LIM>
 __device__ __forceinline__ double grad_finish(int id, int N, const double w[4], const double sx[4],
                                               const double sy[4], const double mn[4], const double mx[4],
-                                              const double *__restrict__ ginv, double g8[8], int phisrc,
+                                              const double G[3], double g8[8], int phisrc,
                                               double *__restrict__ phi, int lt, const int *__restrict__ nvrt,
                                               const double2 *__restrict__ loff, const double *__restrict__ vol,
                                               const int *__restrict__ ioff, const int *__restrict__ frz, int ninst,
                                               bool own) {
-  const double G11 = ginv[id], G12 = ginv[N + id], G22 = ginv[2 * long(N) + id];
   double gx[4], gy[4];
+  lsq_apply_ginv(G[0], G[1], G[2], sx, sy, gx, gy);
   for (int v = 0; v < 4; ++v) {
-    gx[v] = G11 * sx[v] + G12 * sy[v];
-    gy[v] = G12 * sx[v] + G22 * sy[v];
     g8[v] = gx[v];
     g8[4 + v] = gy[v];
   }
@@ -533,7 +533,8 @@ __device__ __forceinline__ double lds_gradient(int q, int id, int N, const doubl
   for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
   for (int k = 0; k < wid; ++k)
     lds_accum<LIM, PMODE>(sl[long(k) * kSlice + l], ci, w, lW, lX, st, p, sx, sy, mn, mx);
-  return grad_finish<LIM>(id, N, w, sx, sy, mn, mx, ginv, g8, phisrc, phi, lt, nvrt, loff, vol, ioff, frz, ninst,
+  const double G[3] = {ginv[id], ginv[N + id], ginv[2 * long(N) + id]};
+  return grad_finish<LIM>(id, N, w, sx, sy, mn, mx, G, g8, phisrc, phi, lt, nvrt, loff, vol, ioff, frz, ninst,
                           own);
 }
 
@@ -547,7 +548,8 @@ __device__ __forceinline__ double lds_gradient(int q, int id, int N, const doubl
 // as k_grad_lsqc, so the gradients are bit-identical to it.
 constexpr int kGradRegs = 16;
 template <int LIM, int PMODE>
-__global__ __launch_bounds__(kBlock) void k_grad_lsqh(int N, int nb, int nl, const double *__restrict__ W,
+// (without the limiter the kernel fits 72 registers: seven wavefronts per SIMD)
+__global__ __launch_bounds__(kBlock, LIM ? 1 : 7) void k_grad_lsqh(int N, int nb, int nl, const double *__restrict__ W,
                                                       const int *__restrict__ ghoff, const int *__restrict__ ghlist,
                                                       const int *__restrict__ poff, const int *__restrict__ swid,
                                                       const unsigned *__restrict__ lcode,
@@ -571,7 +573,11 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqh(int N, int nb, int nl, con
 #pragma unroll
   for (int kk = 0; kk < kGradRegs / 2; ++kk) cr[kk] = (2 * kk < wid) ? lc[kk * kSlice] : 0u;
   const int o2 = ghoff[lb], nh = ghoff[lb + 1] - o2;
+  double G[3] = {0.0, 0.0, 0.0};
   if (act) {
+    G[0] = ginv[id];
+    G[1] = ginv[N + id];
+    G[2] = ginv[2 * long(N) + id];
     double w[4];
     ld4(W, id, w);
     st4(lW, threadIdx.x, w);
@@ -596,7 +602,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqh(int N, int nb, int nl, con
   for (int k = kGradRegs; k < wid; ++k)
     lds_accum<LIM, PMODE>((lc[(k >> 1) * kSlice] >> (16 * (k & 1))) & 0xffffu, ci, w, lW, lX, st, p, sx, sy, mn, mx);
   double g8[8];
-  grad_finish<LIM>(id, N, w, sx, sy, mn, mx, ginv, g8, 0, phi, lt, nvrt, loff, vol, ioff, frz, ninst, true);
+  grad_finish<LIM>(id, N, w, sx, sy, mn, mx, G, g8, 0, phi, lt, nvrt, loff, vol, ioff, frz, ninst, true);
   st8(grad, id, g8);
 }
 
