@@ -663,7 +663,16 @@ __global__ __launch_bounds__(256) void k_adam(long P, int nsplit, int mode, cons
     if (p >= offHead) {
       g = gh[p - offHead];
     } else {
-      for (int k = 0; k < nsplit; ++k) g += G[long(k) * P + p];
+      // the slices' partials in flight 16 at a time, summed in slice order
+      int k = 0;
+      for (; k + 16 <= nsplit; k += 16) {
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = G[long(k + u) * P + p];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) g += v[u];
+      }
+      for (; k < nsplit; ++k) g += G[long(k) * P + p];
     }
     if (gout) gout[p] = g;
     if (mode == 2) {
