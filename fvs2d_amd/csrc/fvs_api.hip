@@ -1577,25 +1577,25 @@ static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, in
   if (opt->rk_scheme == FVS_RK_CLASSIC && (opt->rk_order < 1 || opt->rk_order > 4))
     return fail(FVS_EINVAL, "rk_order must be 1..4");
   if (!(opt->gamma > 1.0)) return fail(FVS_EINVAL, "gamma must exceed 1");
+  if (ntri < 0 || nquad < 0 || ntri + nquad < 1 || nnodes < 3) return fail(FVS_EINVAL, "fvs_create: empty mesh");
+  if (ninst > 0) {
+    const int N = ntri + nquad;
+    if (!cell_offsets || !mach_aoa || cell_offsets[0] != 0 || cell_offsets[ninst] != N)
+      return fail(FVS_EINVAL, "fvs_create_batch: cell_offsets must run from 0 to ncells");
+    for (int k = 0; k < ninst; ++k)
+      if (cell_offsets[k + 1] <= cell_offsets[k])
+        return fail(FVS_EINVAL, "fvs_create_batch: cell_offsets must increase (every instance has cells)");
+  }
   int devs = 0;
   if (hipGetDeviceCount(&devs) != hipSuccess || devs < 1) return fail(FVS_EHIP, "no HIP device available");
   auto *C = new fvs_ctx();
   C->opt = *opt;
   if (ninst > 0) {
     const int N = ntri + nquad;
-    if (!cell_offsets || !mach_aoa || cell_offsets[0] != 0 || cell_offsets[ninst] != N) {
-      delete C;
-      return fail(FVS_EINVAL, "fvs_create_batch: cell_offsets must run from 0 to ncells");
-    }
     C->ninst = ninst;
     C->inst_of_real.resize(N);
-    for (int k = 0; k < ninst; ++k) {
-      if (cell_offsets[k + 1] < cell_offsets[k]) {
-        delete C;
-        return fail(FVS_EINVAL, "fvs_create_batch: cell_offsets must not decrease");
-      }
+    for (int k = 0; k < ninst; ++k)
       for (int c = cell_offsets[k]; c < cell_offsets[k + 1]; ++c) C->inst_of_real[c] = k;
-    }
     C->mach_aoa.assign(mach_aoa, mach_aoa + 2 * size_t(ninst));
   }
   SchemeFlags fl;

@@ -83,8 +83,10 @@ int fvs_create(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad,
  * case per run): ninst independent meshes concatenated into one (instance k
  * owns cells [cell_offsets[k], cell_offsets[k+1]) of the concatenation, its
  * boundaries listed with the others), each with its own free stream
- * (mach_aoa[2k], mach_aoa[2k+1] in degrees).  Every RK stage is one launch over
- * all instances; per-instance residual norms, masks and forces below. */
+ * (mach_aoa[2k], mach_aoa[2k+1] in degrees).  cell_offsets must increase
+ * strictly from 0 to ntri + nquad (every instance has cells).  Every RK
+ * stage is one launch over all instances; per-instance residual norms, masks
+ * and forces below. */
 int fvs_create_batch(fvs_ctx **out, int ninst, const int *cell_offsets, const double *mach_aoa, int nnodes,
                      const double *xy, int ntri, int nquad, const int *cell_nodes, int nb, const int *bnd_ncells,
                      const int *bnd_types, const int *bnd_cells, const fvs_options *opt);

@@ -60,6 +60,38 @@ def test_bad_arguments_return_codes():
     assert L.fvs_set_timestep(None, 0.1, 1.0) == -1
 
 
+def test_argument_checks_before_the_device():
+    """Options, empty meshes and batch offsets are rejected with a message
+    before any device call (so these run without a GPU)."""
+    import numpy as np
+    L = _lib.load()
+    m = meshgen.jittered_square(4)
+    xy = np.ascontiguousarray(m.xy, np.float64)
+    cn = np.ascontiguousarray(m.cell_nodes, np.int32)
+    nb_, ty, cells = m.bc_arrays()
+    P = lambda a: a.ctypes.data  # noqa: E731
+    h = C.c_void_p()
+
+    def create(**kw):
+        o = Scheme(**kw).to_c()
+        return L.fvs_create(C.byref(h), m.nnodes, P(xy), m.ntri, 0, P(cn), None, len(nb_), P(nb_), P(ty),
+                            P(cells), C.byref(o))
+
+    for kw, msg in ((dict(recon=0), b"recon"), (dict(limiter=4), b"limiter"), (dict(rk_scheme=3), b"rk_scheme"),
+                    (dict(rk_scheme=0, rk_order=5), b"rk_order"), (dict(gamma=1.0), b"gamma")):
+        assert create(**kw) == -1 and msg in L.fvs_last_error(), kw
+    o = Scheme().to_c()
+    assert L.fvs_create(C.byref(h), m.nnodes, P(xy), 0, 0, P(cn), None, len(nb_), P(nb_), P(ty), P(cells),
+                        C.byref(o)) == -1 and b"empty" in L.fvs_last_error()
+    ma = np.zeros(4)
+    for off, msg in (([0, 0, m.ntri], b"increase"), ([0, 10, m.ntri - 1], b"run from 0"),
+                     ([0, m.ntri, m.ntri], b"increase")):
+        off = np.array(off, np.int32)
+        rc = L.fvs_create_batch(C.byref(h), 2, P(off), P(ma), m.nnodes, P(xy), m.ntri, 0, P(cn), len(nb_),
+                                P(nb_), P(ty), P(cells), C.byref(o))
+        assert rc == -1 and msg in L.fvs_last_error(), off
+
+
 def _gpu_present():
     return os.path.exists("/dev/kfd")
 

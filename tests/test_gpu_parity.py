@@ -126,6 +126,26 @@ def test_trajectory_config1(kind, scheme):
         assert rel_err(v_gpu, v_ref) <= 1e-9, path
 
 
+@pytest.mark.parametrize("n", [3, 5, 9])
+@pytest.mark.parametrize("kind", ["dirichlet", "periodic"])
+def test_trajectory_tiny_meshes(kind, n):
+    """Meshes far below one 256-cell block (18, 50 and 162 cells: one
+    partial block, partial 64-cell slices), every kernel arrangement, with
+    and without a limiter: 20 steps within 1e-10 of the oracle."""
+    m = mesh(kind, n)
+    for limiter in (0, 1):
+        sch = vortex_scheme(dt=0.02, limiter=limiter)
+        orc = Oracle(m, **sch.oracle_kwargs())
+        orc.initialize(1, 0.0)
+        orc.time_integration(0.0, 20)
+        for path in PATHS:
+            gpu = Solver(m, sch, **path)
+            gpu.initialize_solution(1, 0.0)
+            gpu.time_integration(0.0, 20)
+            assert rel_err(gpu.cvar, orc.cvar) <= 1e-10, (limiter, path)
+            gpu.close()
+
+
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_trajectory_classic_orders(order):
     m = mesh("dirichlet", 32)
