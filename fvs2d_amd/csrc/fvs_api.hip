@@ -171,14 +171,22 @@ __device__ __forceinline__ void stencil_minmax(int i, const int *__restrict__ so
                                                const double w[4], double mn[4], double mx[4]) {
   const int s = i / kSlice, l = i % kSlice;
   for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
-  for (int k = 0; k < swid[s]; ++k) {
-    const int j = snb[soff[s] + long(k) * kSlice + l];
-    double q[4];
-    ld4(W, j, q);
-    for (int v = 0; v < 4; ++v) {
-      mn[v] = (q[v] < mn[v]) ? q[v] : mn[v];
-      mx[v] = (q[v] > mx[v]) ? q[v] : mx[v];
-    }
+  const int wid = swid[s];
+  const long base = soff[s] + l;
+  // four entries per round: their ids, then their states, all in flight together
+  for (int k0 = 0; k0 < wid; k0 += 4) {
+    int j[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) j[u] = (k0 + u < wid) ? snb[base + long(k0 + u) * kSlice] : i;
+    double q[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ld4(W, j[u], q[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      for (int v = 0; v < 4; ++v) {
+        mn[v] = (q[u][v] < mn[v]) ? q[u][v] : mn[v];
+        mx[v] = (q[u][v] > mx[v]) ? q[u][v] : mx[v];
+      }
   }
 }
 
@@ -194,21 +202,29 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggcb(int nbx, int N, int nfw, c
                                                       const int *__restrict__ snb) {
   const int i = xcd_block(nbx) * kBlock + threadIdx.x;
   if (i >= N) return;
-  double w[4], gx[4], gy[4];
+  // every face neighbour's id and coefficient, then every state, in flight
+  // together (nfw <= 4); the sum keeps the face order
+  int j[4];
+  double2 c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    j[k] = (k < nfw) ? nb[long(k) * N + i] : i;
+    c[k] = (k < nfw) ? cn[long(k) * N + i] : make_double2(0.0, 0.0);
+  }
+  double w[4], gx[4], gy[4], q[4][4];
   ld4(W, i, w);
   const double2 a = c0[i];
-  for (int v = 0; v < 4; ++v) gx[v] = a.x * w[v], gy[v] = a.y * w[v];
-  for (int k = 0; k < nfw; ++k) {
-    const int j = nb[long(k) * N + i];
-    const double2 c = cn[long(k) * N + i];
-    double q[4];
-    ld4(W, j, q);
-    for (int v = 0; v < 4; ++v) {
-      gx[v] = gx[v] + c.x * q[v];
-      gy[v] = gy[v] + c.y * q[v];
-    }
-  }
   const double V = vol[i];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) ld4(W, j[k], q[k]);
+  for (int v = 0; v < 4; ++v) gx[v] = a.x * w[v], gy[v] = a.y * w[v];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < nfw)
+      for (int v = 0; v < 4; ++v) {
+        gx[v] = gx[v] + c[k].x * q[k][v];
+        gy[v] = gy[v] + c[k].y * q[k][v];
+      }
   for (int v = 0; v < 4; ++v) gx[v] = gx[v] / V, gy[v] = gy[v] / V;
   double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
   st8(grad, i, g8);
@@ -238,16 +254,27 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggnb(int nbx, int N, const doub
   const double2 a = c0[i];
   for (int v = 0; v < 4; ++v) gx[v] = a.x * w[v], gy[v] = a.y * w[v];
   const int base = goff[s], wid = gwid[s];
-  for (int k = 0; k < wid; ++k) {
-    const long e = base + long(k) * kSlice + l;
-    const int j = gnb[e];
-    const double2 c = gc[e];
-    double q[4];
-    ld4(W, j, q);
-    for (int v = 0; v < 4; ++v) {
-      gx[v] = gx[v] + c.x * q[v];
-      gy[v] = gy[v] + c.y * q[v];
+  // four entries per round: ids and coefficients, then states, in flight
+  // together; the sum keeps the entry order
+  for (int k0 = 0; k0 < wid; k0 += 4) {
+    int j[4];
+    double2 c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long e = base + long(k0 + u) * kSlice + l;
+      j[u] = (k0 + u < wid) ? gnb[e] : i;
+      c[u] = (k0 + u < wid) ? gc[e] : make_double2(0.0, 0.0);
     }
+    double q[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) ld4(W, j[u], q[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (k0 + u < wid)
+        for (int v = 0; v < 4; ++v) {
+          gx[v] = gx[v] + c[u].x * q[u][v];
+          gy[v] = gy[v] + c[u].y * q[u][v];
+        }
   }
   const double V = vol[i];
   for (int v = 0; v < 4; ++v) gx[v] = gx[v] / V, gy[v] = gy[v] / V;
