@@ -166,10 +166,9 @@ def naca_ogrid(n_around: int = 512, n_layers: int = 256, *, thickness: float = 0
     at both edges), ordered trailing edge -> upper surface -> leading edge ->
     lower surface.  Node k of the ray from surface node i sits at distance
     d_k = first * (r^k - 1)/(r - 1) along a direction that turns from the
-    surface normal (at the wall) to the ray from mid-chord at angle s (far
-    away); r is solved so that d reaches ``far_radius``, and each ray's
-    distances are scaled so its last node lies on the circle of that radius
-    about mid-chord, at angle s.
+    surface normal (at the wall) to the ray from mid-chord (far away); r is
+    solved so that d reaches ``far_radius``, and each ray's distances are
+    scaled so its last node lies on the circle of that radius about mid-chord.
     Each quad is split into two counter-clockwise triangles; ``slip_wall`` lists
     the triangles on the airfoil, ``freestream`` those on the outer circle, one
     boundary edge each."""
@@ -181,10 +180,7 @@ def naca_ogrid(n_around: int = 512, n_layers: int = 256, *, thickness: float = 0
     if m % 2 == 0:
         ys[m // 2] = 0.0
     xc0 = 0.5
-    # far-field nodes at uniform angles about mid-chord (the surface nodes'
-    # own angles crowd quadratically towards the trailing edge, which leaves
-    # slivers of aspect ratio ~1000 on the outer circle)
-    ang = s.copy()
+    ang = np.arctan2(ys, xs - xc0)
     # outward unit normals from central differences along the surface
     tx = np.roll(xs, -1) - np.roll(xs, 1)
     ty = np.roll(ys, -1) - np.roll(ys, 1)
