@@ -539,8 +539,18 @@ __global__ __launch_bounds__(256) void k_head(long M, long nvalid, float inv_no,
     for (int o = 0; o < kOut; ++o) gw[c][o] = 0.f;
   for (int o = 0; o < kOut; ++o) gb[o] = 0.f;
   const long mbeg = long(blockIdx.x) * kHeadRows, mend = mbeg + kHeadRows < M ? mbeg + kHeadRows : M;
+  // the next row's H and Z are in flight while this row is processed
+  uint4 hn{}, zn{};
+  if (mbeg + rs < mend) {
+    hn = *reinterpret_cast<const uint4 *>(H + (mbeg + rs) * kW + 8 * q);
+    if (dZ) zn = *reinterpret_cast<const uint4 *>(Z + (mbeg + rs) * kW + 8 * q);
+  }
   for (long m = mbeg + rs; m < mend; m += 8) {
-    const uint4 hv = *reinterpret_cast<const uint4 *>(H + m * kW + 8 * q);
+    const uint4 hv = hn, zv = zn;
+    if (m + 8 < mend) {
+      hn = *reinterpret_cast<const uint4 *>(H + (m + 8) * kW + 8 * q);
+      if (dZ) zn = *reinterpret_cast<const uint4 *>(Z + (m + 8) * kW + 8 * q);
+    }
     const unsigned hw[4] = {hv.x, hv.y, hv.z, hv.w};
     float h[8];
     for (int c = 0; c < 8; ++c) h[c] = bf2f(u16((hw[c >> 1] >> (16 * (c & 1))) & 0xffffu));
@@ -563,7 +573,6 @@ __global__ __launch_bounds__(256) void k_head(long M, long nvalid, float inv_no,
       dy[o] = 2.f * r * inv_no;
       if (q == 0) ls += r * r;
     }
-    const uint4 zv = *reinterpret_cast<const uint4 *>(Z + m * kW + 8 * q);
     const unsigned zw[4] = {zv.x, zv.y, zv.z, zv.w};
     unsigned pk[4] = {0, 0, 0, 0};
     for (int c = 0; c < 8; ++c) {
