@@ -73,10 +73,11 @@ def timed_run(solver, steps: int, warmup: int):
     return wall, solver.last_elapsed_ms / 1e3
 
 
-def cpu_baseline(mesh, dt: float, steps: int):
+def cpu_baseline(mesh, dt: float, steps: int, threads: int = 1):
     from oracle.oracle import Oracle
     sch = scheme_for(dt=dt)
     orc = Oracle(mesh, **sch.oracle_kwargs())
+    orc.set_threads(threads)
     orc.initialize(1, 0.0)
     t0 = time.perf_counter()
     orc.time_integration(0.0, steps)
@@ -265,11 +266,24 @@ def main():
 
     cpu = None
     if args.cpu_steps > 0:
-        rate, sec = cpu_baseline(mesh, dt, args.cpu_steps)
-        cpu = {"value": round(rate, 1), "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
+        # the oracle on one core, then on every core this process may use
+        # (OpenMP over cells, bit-identical to the serial run)
+        rate1, sec1 = cpu_baseline(mesh, dt, args.cpu_steps)
+        print(f"[bench] cpu oracle, 1 thread: {rate1 / 1e6:.3f} M cell-stage/s ({sec1:.1f}s)", file=sys.stderr,
+              flush=True)
+        threads = max(1, int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1))
+        cpu = {"value": round(rate1, 1), "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
                "sample": f"C oracle (gcc -O2, serial), {args.cpu_steps} SSP-RK3 steps of the same "
-                         f"{N}-cell mesh and scheme, {sec:.1f} s"}
-        print(f"[bench] cpu oracle: {rate / 1e6:.3f} M cell-stage/s ({sec:.1f}s)", file=sys.stderr, flush=True)
+                         f"{N}-cell mesh and scheme, {sec1:.1f} s"}
+        if threads > 1:
+            steps_n = args.cpu_steps * min(threads, 4)
+            rate, sec = cpu_baseline(mesh, dt, steps_n, threads)
+            print(f"[bench] cpu oracle, {threads} threads: {rate / 1e6:.3f} M cell-stage/s ({sec:.1f}s)",
+                  file=sys.stderr, flush=True)
+            cpu = {"value": round(rate, 1), "unit": "cell-stage updates/s", "cores": threads, "kind": "port",
+                   "sample": f"C oracle (gcc -O2, OpenMP over cells, {threads} threads), {steps_n} SSP-RK3 steps "
+                             f"of the same {N}-cell mesh and scheme, {sec:.1f} s",
+                   "single_core": {"value": round(rate1, 1), "sample": cpu["sample"]}}
 
     mlp = None
     if args.mlp_steps > 0:

@@ -26,6 +26,12 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+/* per-cell loops run on nthr threads when orc_set_threads asked for more
+ * than one; every cell's arithmetic is the serial one */
+#define ORC_PAR _Pragma("omp parallel for if(o->nthr > 1) num_threads(o->nthr > 1 ? o->nthr : 1) schedule(static)")
 
 static char g_err[512];
 const char *orc_last_error(void) { return g_err; }
@@ -71,6 +77,12 @@ struct orc {
   double *pvar, *cvar, *grad, *resid, *phi, *ws, *dt_local;
   double pvar_inf[4];
   double kappa;
+  /* threaded evaluation (orc_set_threads): per cell, its faces in the serial
+   * scatter order as signed (face + 1) -- interior faces by position in
+   * edge_intr, then boundary faces after them -- and the face products */
+  int nthr;
+  int *cf_ptr, *cf;
+  double *fa, *wa;
 };
 
 static void *xcalloc(size_t n, size_t s) {
@@ -731,7 +743,7 @@ void orc_destroy(orc_t *o) {
                   o->bnd_cptr, o->bnd_cells, o->bnd_nedges, o->bnd_eptr, o->bnd_edges, o->lsq_ptr, o->lsq_cell,
                   o->lsq_w, o->lsq_coef, o->ggcb_ptr, o->ggcb_coef0, o->ggcb_coefnb, o->ggnb_coef0,
                   o->ggnb_coefedg, o->ggnb_ptr, o->ggnb_cell, o->ggnb_in, o->ggnb_coefnb, o->pvar, o->cvar,
-                  o->grad, o->resid, o->phi, o->ws, o->dt_local};
+                  o->grad, o->resid, o->phi, o->ws, o->dt_local, o->cf_ptr, o->cf, o->fa, o->wa};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   free(o);
 }
@@ -787,6 +799,7 @@ void orc_get_dt_local(const orc_t *o, double *g) { memcpy(g, o->dt_local, o->nce
 /* ------------------------------------------------------------------------ */
 void orc_cvar2pvar(orc_t *o) {
   double g = o->op.gamma;
+  ORC_PAR
   for (int ic = 0; ic < o->ncells; ++ic) {
     double *pv = &o->pvar[4 * ic];
     const double *cv = &o->cvar[4 * ic];
@@ -817,6 +830,7 @@ static void grad_generic(const orc_t *o, int nvar, const double *q, int qstride,
   int N = o->ncells;
   int m = o->op.grad_method;
   if (m == 3) {
+    ORC_PAR
     for (int ic = 0; ic < N; ++ic) {
       for (int k = 0; k < nvar; ++k) gx[gstride * ic + k] = 0.0, gy[gstride * ic + k] = 0.0;
       for (int in = o->lsq_ptr[ic]; in < o->lsq_ptr[ic + 1]; ++in) {
@@ -829,7 +843,8 @@ static void grad_generic(const orc_t *o, int nvar, const double *q, int qstride,
       }
     }
   } else if (m == 1) {
-    for (int iv = 0; iv < nvar; ++iv)
+    for (int iv = 0; iv < nvar; ++iv) {
+      ORC_PAR
       for (int ic = 0; ic < N; ++ic) {
         double a = o->ggcb_coef0[2 * ic] * q[qstride * ic + iv];
         double b = o->ggcb_coef0[2 * ic + 1] * q[qstride * ic + iv];
@@ -841,8 +856,10 @@ static void grad_generic(const orc_t *o, int nvar, const double *q, int qstride,
         gx[gstride * ic + iv] = a / o->vol[ic];
         gy[gstride * ic + iv] = b / o->vol[ic];
       }
+    }
   } else if (m == 2) {
-    for (int iv = 0; iv < nvar; ++iv)
+    for (int iv = 0; iv < nvar; ++iv) {
+      ORC_PAR
       for (int ic = 0; ic < N; ++ic) {
         double a = o->ggnb_coef0[2 * ic] * q[qstride * ic + iv];
         double b = o->ggnb_coef0[2 * ic + 1] * q[qstride * ic + iv];
@@ -854,6 +871,7 @@ static void grad_generic(const orc_t *o, int nvar, const double *q, int qstride,
         gx[gstride * ic + iv] = a / o->vol[ic];
         gy[gstride * ic + iv] = b / o->vol[ic];
       }
+    }
   }
 }
 
@@ -900,7 +918,8 @@ void orc_limiter(orc_t *o) {
   if (o->op.limiter == 0) { for (int i = 0; i < N; ++i) o->phi[i] = 1.0; return; }
   double *phi4 = xcalloc(4 * N, sizeof(double));
   const double *gx = o->grad, *gy = o->grad + 4 * N;
-  for (int iv = 0; iv < 4; ++iv)
+  for (int iv = 0; iv < 4; ++iv) {
+    ORC_PAR
     for (int ic = 0; ic < N; ++ic) {
       double q = o->pvar[4 * ic + iv];
       double pmin = q, pmax = q;
@@ -924,6 +943,8 @@ void orc_limiter(orc_t *o) {
       for (int k = 1; k < 4; ++k) m = (pe[k] < m) ? pe[k] : m;
       phi4[4 * ic + iv] = (1.0 < m) ? 1.0 : m;
     }
+  }
+  ORC_PAR
   for (int ic = 0; ic < N; ++ic) {
     double m = phi4[4 * ic];
     for (int k = 1; k < 4; ++k) m = (phi4[4 * ic + k] < m) ? phi4[4 * ic + k] : m;
@@ -1080,59 +1101,144 @@ int orc_bc_state(const orc_t *o, int bc_type, double t, double x, double y, doub
 /* ------------------------------------------------------------------------ */
 /* compute_residual: src/residual.f90:23-177                                */
 /* ------------------------------------------------------------------------ */
+/* flux * area and ws * area of interior face i (position in edge_intr) */
+static void interior_face(const orc_t *o, int i, double *fa, double *wa) {
+  double g = o->op.gamma, kap = o->kappa;
+  const double *gx = o->grad, *gy = o->grad + 4 * o->ncells;
+  int ie = o->edge_intr[i];
+  double xf = o->ex[ie], yf = o->ey[ie], af = o->ea[ie], nxf = o->enx[ie], nyf = o->eny[ie];
+  int icL = o->ec1[ie], icR = o->ec2[ie];
+  double xcL = o->cx[icL], ycL = o->cy[icL];
+  double xcR = o->cx[icR] + o->esx[ie], ycR = o->cy[icR] + o->esy[ie];
+  double pfL[4], pfR[4], flux[4], wsm;
+  for (int k = 0; k < 4; ++k) {
+    double gC = o->pvar[4 * icR + k] - o->pvar[4 * icL + k];
+    double gL = (xf - xcL) * gx[4 * icL + k] + (yf - ycL) * gy[4 * icL + k];
+    double gR = (xf - xcR) * gx[4 * icR + k] + (yf - ycR) * gy[4 * icR + k];
+    pfL[k] = o->pvar[4 * icL + k] + o->phi[icL] * (kap / 2.0 * gC + (1.0 - kap) * gL);
+    pfR[k] = o->pvar[4 * icR + k] + o->phi[icR] * (-(kap / 2.0 * gC) + (1.0 - kap) * gR);
+  }
+  orc_roe(g, pfL, pfR, nxf, nyf, flux, &wsm);
+  for (int k = 0; k < 4; ++k) fa[k] = flux[k] * af;
+  *wa = wsm * af;
+}
+
+/* flux * area and ws * area of boundary edge i of boundary ib; returns its cell */
+static int boundary_face(const orc_t *o, int ib, int i, double time, double *fa, double *wa) {
+  const double *gx = o->grad, *gy = o->grad + 4 * o->ncells;
+  int ie = o->bnd_edges[o->bnd_eptr[ib] + i];
+  double xf = o->ex[ie], yf = o->ey[ie], af = o->ea[ie], nxf = o->enx[ie], nyf = o->eny[ie];
+  int icL = o->bnd_cells[o->bnd_cptr[ib] + i];
+  double xcL = o->cx[icL], ycL = o->cy[icL];
+  double pfL[4], pfR[4], flux[4], wsm;
+  for (int k = 0; k < 4; ++k)
+    pfL[k] = o->pvar[4 * icL + k] +
+             o->phi[icL] * ((xf - xcL) * gx[4 * icL + k] + (yf - ycL) * gy[4 * icL + k]);
+  orc_bc_state(o, o->bnd_type[ib], time, xf, yf, nxf, nyf, pfL, pfR);
+  orc_roe(o->op.gamma, pfL, pfR, nxf, nyf, flux, &wsm);
+  for (int k = 0; k < 4; ++k) fa[k] = flux[k] * af;
+  *wa = wsm * af;
+  return icL;
+}
+
+/* compute_residual: src/residual.f90:23-177.  Serially the face fluxes are
+ * scattered to their cells in face order; threaded, every face's products
+ * go to an array and each cell adds its faces in that same order, so both
+ * give the same bits. */
 void orc_compute_residual(orc_t *o, double time) {
   int N = o->ncells;
-  double g = o->op.gamma, kap = o->kappa;
-  memset(o->resid, 0, 4 * N * 8);
-  memset(o->ws, 0, N * 8);
   orc_cvar2pvar(o);
   orc_gradient(o);
   orc_limiter(o);
-  const double *gx = o->grad, *gy = o->grad + 4 * N;
+  if (o->nthr > 1 && o->cf_ptr) {
+    int nfi = o->nedges_intr;
+    ORC_PAR
+    for (int i = 0; i < nfi; ++i) interior_face(o, i, o->fa + 4 * (size_t)i, o->wa + i);
+    for (int ib = 0; ib < o->nb; ++ib) {
+      int b0 = nfi + (o->bnd_eptr[ib] - o->bnd_eptr[0]);
+      ORC_PAR
+      for (int i = 0; i < o->bnd_nedges[ib]; ++i)
+        (void)boundary_face(o, ib, i, time, o->fa + 4 * (size_t)(b0 + i), o->wa + b0 + i);
+    }
+    ORC_PAR
+    for (int ic = 0; ic < N; ++ic) {
+      double r[4] = {0.0, 0.0, 0.0, 0.0}, w = 0.0;
+      for (int e = o->cf_ptr[ic]; e < o->cf_ptr[ic + 1]; ++e) {
+        int sg = o->cf[e], f = (sg > 0 ? sg : -sg) - 1;
+        for (int k = 0; k < 4; ++k) r[k] = sg > 0 ? r[k] + o->fa[4 * (size_t)f + k] : r[k] - o->fa[4 * (size_t)f + k];
+        w = w + o->wa[f];
+      }
+      for (int k = 0; k < 4; ++k) o->resid[4 * ic + k] = -r[k] / o->vol[ic];
+      o->ws[ic] = w;
+    }
+    return;
+  }
+  memset(o->resid, 0, 4 * N * 8);
+  memset(o->ws, 0, N * 8);
   for (int i = 0; i < o->nedges_intr; ++i) {
     int ie = o->edge_intr[i];
-    double xf = o->ex[ie], yf = o->ey[ie], af = o->ea[ie], nxf = o->enx[ie], nyf = o->eny[ie];
     int icL = o->ec1[ie], icR = o->ec2[ie];
-    double xcL = o->cx[icL], ycL = o->cy[icL];
-    double xcR = o->cx[icR] + o->esx[ie], ycR = o->cy[icR] + o->esy[ie];
-    double pfL[4], pfR[4], flux[4], wsm;
+    double fa[4], wa;
+    interior_face(o, i, fa, &wa);
     for (int k = 0; k < 4; ++k) {
-      double gC = o->pvar[4 * icR + k] - o->pvar[4 * icL + k];
-      double gL = (xf - xcL) * gx[4 * icL + k] + (yf - ycL) * gy[4 * icL + k];
-      double gR = (xf - xcR) * gx[4 * icR + k] + (yf - ycR) * gy[4 * icR + k];
-      pfL[k] = o->pvar[4 * icL + k] + o->phi[icL] * (kap / 2.0 * gC + (1.0 - kap) * gL);
-      pfR[k] = o->pvar[4 * icR + k] + o->phi[icR] * (-(kap / 2.0 * gC) + (1.0 - kap) * gR);
+      o->resid[4 * icL + k] = o->resid[4 * icL + k] + fa[k];
+      o->resid[4 * icR + k] = o->resid[4 * icR + k] - fa[k];
     }
-    orc_roe(g, pfL, pfR, nxf, nyf, flux, &wsm);
-    for (int k = 0; k < 4; ++k) {
-      o->resid[4 * icL + k] = o->resid[4 * icL + k] + flux[k] * af;
-      o->resid[4 * icR + k] = o->resid[4 * icR + k] - flux[k] * af;
-    }
-    o->ws[icL] = o->ws[icL] + wsm * af;
-    o->ws[icR] = o->ws[icR] + wsm * af;
+    o->ws[icL] = o->ws[icL] + wa;
+    o->ws[icR] = o->ws[icR] + wa;
   }
   for (int ib = 0; ib < o->nb; ++ib) {
     for (int i = 0; i < o->bnd_nedges[ib]; ++i) {
-      int ie = o->bnd_edges[o->bnd_eptr[ib] + i];
-      double xf = o->ex[ie], yf = o->ey[ie], af = o->ea[ie], nxf = o->enx[ie], nyf = o->eny[ie];
-      int icL = o->bnd_cells[o->bnd_cptr[ib] + i];
-      double xcL = o->cx[icL], ycL = o->cy[icL];
-      double pfL[4], pfR[4], flux[4], wsm;
-      for (int k = 0; k < 4; ++k)
-        pfL[k] = o->pvar[4 * icL + k] +
-                 o->phi[icL] * ((xf - xcL) * gx[4 * icL + k] + (yf - ycL) * gy[4 * icL + k]);
-      orc_bc_state(o, o->bnd_type[ib], time, xf, yf, nxf, nyf, pfL, pfR);
-      orc_roe(g, pfL, pfR, nxf, nyf, flux, &wsm);
-      for (int k = 0; k < 4; ++k) o->resid[4 * icL + k] = o->resid[4 * icL + k] + flux[k] * af;
-      o->ws[icL] = o->ws[icL] + wsm * af;
+      double fa[4], wa;
+      int icL = boundary_face(o, ib, i, time, fa, &wa);
+      for (int k = 0; k < 4; ++k) o->resid[4 * icL + k] = o->resid[4 * icL + k] + fa[k];
+      o->ws[icL] = o->ws[icL] + wa;
     }
   }
   for (int ic = 0; ic < N; ++ic)
     for (int k = 0; k < 4; ++k) o->resid[4 * ic + k] = -o->resid[4 * ic + k] / o->vol[ic];
 }
 
+/* Threads for the per-cell work of later calls (1 = serial, the default);
+ * builds each cell's face list in the serial scatter order. */
+int orc_set_threads(orc_t *o, int n) {
+  if (!o || n < 1) return -1;
+  o->nthr = n;
+  if (n == 1 || o->cf_ptr) return 0;
+  int N = o->ncells, nfi = o->nedges_intr, nbe = 0;
+  for (int ib = 0; ib < o->nb; ++ib) nbe += o->bnd_nedges[ib];
+  int *cnt = xcalloc(N + 1, sizeof(int));
+  for (int i = 0; i < nfi; ++i) {
+    int ie = o->edge_intr[i];
+    cnt[o->ec1[ie] + 1]++;
+    cnt[o->ec2[ie] + 1]++;
+  }
+  for (int ib = 0; ib < o->nb; ++ib)
+    for (int i = 0; i < o->bnd_nedges[ib]; ++i) cnt[o->bnd_cells[o->bnd_cptr[ib] + i] + 1]++;
+  for (int c = 0; c < N; ++c) cnt[c + 1] += cnt[c];
+  o->cf_ptr = cnt;
+  o->cf = xcalloc((size_t)cnt[N] + 1, sizeof(int));
+  int *fill = xcalloc(N, sizeof(int));
+  for (int i = 0; i < nfi; ++i) {
+    int ie = o->edge_intr[i], a = o->ec1[ie], b = o->ec2[ie];
+    o->cf[cnt[a] + fill[a]++] = i + 1;
+    o->cf[cnt[b] + fill[b]++] = -(i + 1);
+  }
+  int f = nfi;
+  for (int ib = 0; ib < o->nb; ++ib)
+    for (int i = 0; i < o->bnd_nedges[ib]; ++i, ++f) {
+      int c = o->bnd_cells[o->bnd_cptr[ib] + i];
+      o->cf[cnt[c] + fill[c]++] = f + 1;
+    }
+  free(fill);
+  o->fa = xcalloc(4 * (size_t)(nfi + nbe) + 4, sizeof(double));
+  o->wa = xcalloc((size_t)(nfi + nbe) + 1, sizeof(double));
+  return 0;
+}
+
 /* compute_local_time: src/runge_kutta.f90:424-437 */
 void orc_local_time(orc_t *o) {
+  ORC_PAR
   for (int ic = 0; ic < o->ncells; ++ic) o->dt_local[ic] = o->op.cfl * o->vol[ic] / (0.5 * o->ws[ic]);
 }
 
@@ -1230,17 +1336,24 @@ void orc_time_integration(orc_t *o, double t1, int nsteps, double *norms, double
       for (int rk = 0; rk < 4; ++rk) {
         orc_compute_residual(o, tstart[rk]);
         if (o->op.steady && rk == 0) orc_local_time(o);
+        ORC_PAR
         for (int i = 0; i < 4 * N; ++i) fcvar[i] = fcvar[i] + rk_coef[rk] * o->resid[i];
         if (!o->op.steady) {
           /* time_integ_RK 156-162 */
-          if (rk < 3) for (int i = 0; i < 4 * N; ++i) o->cvar[i] = cvar0[i] + h_rk[rk] * o->resid[i];
-          else for (int i = 0; i < 4 * N; ++i) o->cvar[i] = cvar0[i] + h_rk[rk] * fcvar[i];
+          if (rk < 3) {
+            ORC_PAR
+            for (int i = 0; i < 4 * N; ++i) o->cvar[i] = cvar0[i] + h_rk[rk] * o->resid[i];
+          } else {
+            ORC_PAR
+            for (int i = 0; i < 4 * N; ++i) o->cvar[i] = cvar0[i] + h_rk[rk] * fcvar[i];
+          }
         } else {
           /* time_integ_RK_steady 299-313 (the reference aborts before its
            * first step, src/runge_kutta.f90:270,321; restated as written) */
           double cst = 1.0 / 2.0;
           if (rk == 2) cst = 1.0;
           if (rk == 3) cst = 1.0 / 6.0;
+          ORC_PAR
           for (int ic = 0; ic < N; ++ic)
             for (int k = 0; k < 4; ++k) {
               int i = 4 * ic + k;
@@ -1255,17 +1368,20 @@ void orc_time_integration(orc_t *o, double t1, int nsteps, double *norms, double
         orc_compute_residual(o, ts);
         if (!o->op.steady) {
           /* time_integ_SSPRK 225-226 */
+          ORC_PAR
           for (int i = 0; i < 4 * N; ++i) o->cvar[i] = cvar0[i] + h_rk[rk] * (rk_coef[rk] * o->resid[i] + fcvar[i]);
         } else {
           /* time_integ_SSPRK_steady 379-385 */
           if (rk == 0) orc_local_time(o);
           double cst = (rk == 3) ? 1.0 / 4.0 : 1.0 / 3.0;
+          ORC_PAR
           for (int ic = 0; ic < N; ++ic)
             for (int k = 0; k < 4; ++k) {
               int i = 4 * ic + k;
               o->cvar[i] = cvar0[i] + o->dt_local[ic] * cst * (rk_coef[rk] * o->resid[i] + fcvar[i]);
             }
         }
+        ORC_PAR
         for (int i = 0; i < 4 * N; ++i) fcvar[i] = fcvar[i] + o->resid[i];
       }
     } else {
@@ -1276,6 +1392,7 @@ void orc_time_integration(orc_t *o, double t1, int nsteps, double *norms, double
       for (int s = 0; s < 3; ++s) {
         orc_compute_residual(o, ts[s]);
         if (o->op.steady && s == 0) orc_local_time(o);
+        ORC_PAR
         for (int ic = 0; ic < N; ++ic) {
           double h = o->op.steady ? o->dt_local[ic] : dt;
           for (int k = 0; k < 4; ++k) {

@@ -97,6 +97,9 @@ int  orc_bc_state(const orc_t *o, int bc_type, double t, double x, double y, dou
 void orc_initialize(orc_t *o, int mode, double t0);
 /* integrate nsteps from t1; norms: nsteps*4 (may be NULL); verr: nsteps*13 (may be NULL) */
 void orc_time_integration(orc_t *o, double t1, int nsteps, double *norms, double *verr);
+/* per-cell work of later calls on n threads (OpenMP; 1 = serial, the
+ * default); the results are bit-identical to the serial ones */
+int  orc_set_threads(orc_t *o, int n);
 void orc_vortex_error(const orc_t *o, double t, double *out13);
 /* wall forces over slip/solid walls: cl, cd, cl1, cd1 (src/io.f90:340-449) */
 void orc_wall_forces(const orc_t *o, double *out4);

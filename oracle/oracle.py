@@ -68,6 +68,7 @@ def lib():
         L.orc_compute_residual.argtypes = [P, C.c_double]
         L.orc_initialize.argtypes = [P, C.c_int, C.c_double]
         L.orc_time_integration.argtypes = [P, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_set_threads.argtypes = [P, C.c_int]
         L.orc_roe.argtypes = [C.c_double, dp, dp, C.c_double, C.c_double, dp, dp]
         L.orc_limiter_fn.restype = C.c_double
         L.orc_limiter_fn.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
@@ -216,6 +217,12 @@ class Oracle:
         return self._get("orc_get_dt_local", (self.ncells,))
 
     # -- operators -------------------------------------------------------------
+    def set_threads(self, n: int):
+        """Per-cell work of later calls on n OpenMP threads (bit-identical to
+        the serial evaluation; 1, the default, is serial)."""
+        if self.L.orc_set_threads(C.c_void_p(self.h), int(n)) != 0:
+            raise ValueError("thread count must be positive")
+
     def initialize(self, mode=1, t0=0.0):
         self.L.orc_initialize(C.c_void_p(self.h), mode, t0)
 
