@@ -6,8 +6,10 @@
  *
  * It restates, loop for loop and in the reference's summation order, the
  * Fortran routines cited beside each function (shirzadgit/fvs2d, src/...).
- * It is deliberately plain, serial C: no reassociation, no FMA contraction
- * (build with -ffp-contract=off), 0-based indices with -1 for "no cell".
+ * It is deliberately plain C: no reassociation, no FMA contraction (build
+ * with -ffp-contract=off), 0-based indices with -1 for "no cell".  Serial by
+ * default; orc_set_threads runs the per-cell loops on OpenMP threads with the
+ * same per-cell arithmetic and order (bitwise the serial results).
  *
  * Builder-new extensions (no reference counterpart, marked as such):
  *   - periodic node identification (node_canon): topology on canonical node
