@@ -276,7 +276,7 @@ def main():
                "sample": f"C oracle (gcc -O2, serial), {args.cpu_steps} SSP-RK3 steps of the same "
                          f"{N}-cell mesh and scheme, {sec1:.1f} s"}
         if threads > 1:
-            steps_n = args.cpu_steps * min(threads, 4)
+            steps_n = max(args.cpu_steps, int(0.8 * args.cpu_steps * threads))  # a sample of several seconds
             rate, sec = cpu_baseline(mesh, dt, steps_n, threads)
             print(f"[bench] cpu oracle, {threads} threads: {rate / 1e6:.3f} M cell-stage/s ({sec:.1f}s)",
                   file=sys.stderr, flush=True)
