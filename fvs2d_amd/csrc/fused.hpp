@@ -255,9 +255,11 @@ struct UpdIn {
   double u0[4], x[4];  // x: SSP-RK3 stage state, else accumulator (0 on the first stage)
 };
 
+// Ux: where SSP-RK3 reads the stage state (the stage-state buffer the
+// update writes in place, or the separate stage input).
 __device__ __forceinline__ void block_update_load(int N, int i, int gw, const Upd &A, const short *__restrict__ lgat,
                                                   const double *__restrict__ vol, const double *__restrict__ Ua,
-                                                  const double *__restrict__ Ub, const double *__restrict__ Facc,
+                                                  const double *__restrict__ Ux, const double *__restrict__ Facc,
                                                   const double *__restrict__ dtl, UpdIn &in) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) in.sv[k] = (k < gw) ? lgat[long(k) * N + i] : 0;
@@ -266,7 +268,7 @@ __device__ __forceinline__ void block_update_load(int N, int i, int gw, const Up
   ld4(Ua, i, in.u0);
   if (A.mode == M_SSP3) {
     if (A.first) for (int v = 0; v < 4; ++v) in.x[v] = in.u0[v];
-    else ld4(Ub, i, in.x);
+    else ld4(Ux, i, in.x);
   } else {
     if (A.first) for (int v = 0; v < 4; ++v) in.x[v] = 0.0;
     else ld4(Facc, i, in.x);
@@ -340,9 +342,11 @@ __device__ __forceinline__ void block_update_finish(int N, int lb, const Upd &A,
     } else if (A.mode == M_SSP3) {
       st4(Ub, i, un);
     }
-    double w[4];
-    cons2prim(un, A.gamma, w);
-    st4(Wout, i, w);
+    if (Wout) {
+      double w[4];
+      cons2prim(un, A.gamma, w);
+      st4(Wout, i, w);
+    }
   }
   if (A.last) {
     __syncthreads();  // red may alias the face fluxes just gathered
@@ -355,7 +359,13 @@ __device__ __forceinline__ void block_update_finish(int N, int lb, const Upd &A,
 // fused face fluxes (LDS) + ordered gather + stage update; reads the stage's
 // primitive state from Win and writes the next one to Wout (separate buffers:
 // neighbouring blocks still read Win).  phi_one: no limiter, every limiter
-// value is 1 and is not loaded.
+// value is 1 and is not loaded.  With cons (SSP-RK3 only), Win is the
+// stage's conservative input state, converted as it is read, the update
+// takes its stage state from it and writes the new one to Ub (a buffer other
+// than Win), and no primitive state is written (Wout null).  The update
+// reads its own cell's stage input a second time rather than keeping it from
+// the staging: in registers or LDS it costs a wavefront per SIMD, which is
+// slower than the re-read.
 __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, int maxf, Upd A, Phys P, double t,
                                                         const int *__restrict__ boff, const int2 *__restrict__ lrec,
                                                         const double *__restrict__ lgeo, long L,
@@ -368,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
                                                         double *__restrict__ Wout, double *__restrict__ dtl,
                                                         double *__restrict__ part, const int *__restrict__ blist,
                                                         const double *__restrict__ pinf_inst,
-                                                        double *__restrict__ d2out, int phi_one) {
+                                                        double *__restrict__ d2out, int phi_one, int cons) {
   extern __shared__ double lds[];
   double *lF = lds;             // [maxf][4] flux*area (later the block reduction)
   double *lS = lds + 4 * maxf;  // [maxf]    wave speed*area, only when the stage sets local time steps
@@ -392,8 +402,9 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
   if (i0 + int(threadIdx.x) < N) {
     double gg[8], ww[4];
     ld8(grad, i0 + threadIdx.x, gg);
-    st8(sG, threadIdx.x, gg);
     ld4(Win, i0 + threadIdx.x, ww);
+    st8(sG, threadIdx.x, gg);
+    make_prim(cons, A.gamma, ww);
     st4(sW, threadIdx.x, ww);
   }
   __syncthreads();
@@ -420,6 +431,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
         ld4(Win, c.y, wR);
         ld8(grad, c.y, gR);
       }
+      if (la >= unsigned(kBlock)) make_prim(cons, A.gamma, wL);
+      if (lc >= unsigned(kBlock)) make_prim(cons, A.gamma, wR);
       interior_flux_v(P, g, wL, wR, gL, gR, phi_one ? 1.0 : phi[c.x], phi_one ? 1.0 : phi[c.y], fa, wsa);
     } else {
       const int code = -1 - c.y;
@@ -434,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
   }
   UpdIn in;
   const int ic = i0 + threadIdx.x;
-  if (ic < N) block_update_load(N, ic, gw, A, lgat, vol, Ua, Ub, Facc, dtl, in);
+  if (ic < N) block_update_load(N, ic, gw, A, lgat, vol, Ua, cons ? Win : Ub, Facc, dtl, in);
   __syncthreads();
   block_update_finish<kBlock>(N, lb, A, in, lF, lS, Ua, Ub, Facc, Wout, dtl, part, d2out, lF);
 }
@@ -545,11 +558,13 @@ __device__ __forceinline__ double lds_gradient(int q, int id, int N, const doubl
 // word (entries 2kk and 2kk+1 of lane l of slice s at lcode[poff[s] + 64 kk
 // + l]); a thread loads the first kGradRegs of its own while the block is
 // staged, so the LDS loop waits on no global load.  Same arithmetic and order
-// as k_grad_lsqc, so the gradients are bit-identical to it.
+// as k_grad_lsqc, so the gradients are bit-identical to it.  With cons, W
+// holds conservative states, converted as they are staged.
 constexpr int kGradRegs = 16;
 template <int LIM, int PMODE>
 // (without the limiter the kernel fits 72 registers: seven wavefronts per SIMD)
 __global__ __launch_bounds__(kBlock, LIM ? 1 : 7) void k_grad_lsqh(int N, int nb, int nl, const double *__restrict__ W,
+                                                      int cons, double gamma,
                                                       const int *__restrict__ ghoff, const int *__restrict__ ghlist,
                                                       const int *__restrict__ poff, const int *__restrict__ swid,
                                                       const unsigned *__restrict__ lcode,
@@ -578,17 +593,21 @@ __global__ __launch_bounds__(kBlock, LIM ? 1 : 7) void k_grad_lsqh(int N, int nb
     G[0] = ginv[id];
     G[1] = ginv[N + id];
     G[2] = ginv[2 * long(N) + id];
+    const double2 xy = cxy[id];
     double w[4];
     ld4(W, id, w);
+    lX[threadIdx.x] = xy;
+    make_prim(cons, gamma, w);
     st4(lW, threadIdx.x, w);
-    lX[threadIdx.x] = cxy[id];
   }
   for (int q = threadIdx.x; q < nh; q += kBlock) {
     const int j = ghlist[o2 + q];
+    const double2 xy = cxy[j];
     double w[4];
     ld4(W, j, w);
+    lX[kBlock + q] = xy;
+    make_prim(cons, gamma, w);
     st4(lW, kBlock + q, w);
-    lX[kBlock + q] = cxy[j];
   }
   __syncthreads();
   if (!act) return;

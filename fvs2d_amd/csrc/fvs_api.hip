@@ -78,6 +78,15 @@ __device__ __forceinline__ void st4(double *__restrict__ p, long i, const double
   reinterpret_cast<double2 *>(p)[2 * i] = make_double2(v[0], v[1]);
   reinterpret_cast<double2 *>(p)[2 * i + 1] = make_double2(v[2], v[3]);
 }
+// a loaded state made primitive: with cons it holds conservative variables
+// and is converted in place (called after the loads that travel with it are
+// issued, so the conversion waits on the state alone)
+__device__ __forceinline__ void make_prim(bool cons, double g, double w[4]) {
+  if (cons) {
+    const double u[4] = {w[0], w[1], w[2], w[3]};
+    cons2prim(u, g, w);
+  }
+}
 __device__ __forceinline__ void ld8(const double *__restrict__ p, long i, double v[8]) {
   const double2 *q = reinterpret_cast<const double2 *>(p) + 4 * i;
   double2 a = q[0], b = q[1], c = q[2], d = q[3];
@@ -696,6 +705,10 @@ struct fvs_ctx {
   int2 *lrec = nullptr;
   double *lgeo = nullptr;
   double *Walt = nullptr;  // second primitive-state buffer
+  // SSP-RK3 on the fused path with the staged-halo (or no) gradient: the
+  // stages read conservative states (U0 -> Ub -> Uc -> U0) and no primitive
+  // state is written; W is refreshed once after the steps
+  double *Uc = nullptr;
   // XCD-aware block numbering per kernel class (fvs_autotune picks)
   bool xcd_grad = false, xcd_flux = false, xcd_upd = false, xcd_fused = true;
   // limiter values frozen at their last evaluation (steady convergence aid)
@@ -1313,6 +1326,7 @@ int setup_device(fvs_ctx &C) {
   if (int rc = C.alloc(C.Facc, n4)) return rc;
   if (int rc = C.alloc(C.W, n4)) return rc;
   if (int rc = C.alloc(C.Walt, n4)) return rc;
+  if (int rc = C.alloc(C.Uc, n4)) return rc;
   if (int rc = C.alloc(C.grad, 2 * n4)) return rc;
   if (int rc = C.alloc(C.phi, N)) return rc;
   if (int rc = C.alloc(C.dtl, N)) return rc;
@@ -1391,11 +1405,14 @@ struct ProfScope {
   }
 };
 
-int launch_gradient(fvs_ctx &C, bool with_limiter) {
+// gradient (and limiter) of the primitive state W, or with cons of the
+// conservative state src (staged-halo kernel only)
+int launch_gradient(fvs_ctx &C, bool with_limiter, const double *src = nullptr) {
   const int N = C.N;
   const bool lim = with_limiter && C.opt.limiter > 0 && C.opt.recon != 1 && !C.limiter_frozen;
   const int lt = C.opt.limiter;
   dim3 g(xcd_grid(nblk(N))), b(kBlock);
+  if (src && !(C.halo && C.use_halo)) return fail(FVS_ESTATE, "conservative gradient input needs the staged halo");
   ProfScope ps(C, KC_GRAD);
   if (C.halo && C.use_halo) {
     if (C.nact) g = dim3(C.nact);
@@ -1403,7 +1420,8 @@ int launch_gradient(fvs_ctx &C, bool with_limiter) {
     const int nl = kBlock + C.maxh;
     const size_t lds = size_t(6) * nl * sizeof(double);
 #define FVS_LSQH(L, PM)                                                                                          \
-  hipLaunchKernelGGL((k_grad_lsqh<L, PM>), g, b, lds, C.stream, N, xcd_arg(nb, C.xcd_grad), nl, C.W, C.ghoff,      \
+  hipLaunchKernelGGL((k_grad_lsqh<L, PM>), g, b, lds, C.stream, N, xcd_arg(nb, C.xcd_grad), nl, src ? src : C.W, \
+                     src ? 1 : 0, C.opt.gamma, C.ghoff,                                                         \
                      C.ghlist, C.poff, C.swid, C.lcode, C.cxy, C.ginv, C.shifts, C.opt.lsq_p, C.grad, C.phi, lt,     \
                      C.nvrt, C.loff, C.vol, C.nact ? C.blist : nullptr, C.frz_some ? C.dinst_off : nullptr, C.dfrz,    \
                      C.ninst)
@@ -1493,9 +1511,13 @@ int launch_update(fvs_ctx &C, const Upd &A) {
 }
 
 // fused flux + gather + update of one stage; swaps the primitive buffers
-int launch_flux_update(fvs_ctx &C, const Upd &A, double t) {
+// uin / uout (SSP-RK3, conservative stages): the stage's input state and the
+// buffer its new stage state goes to; without them the stage reads W and
+// writes the next primitive state to Walt (then swapped)
+int launch_flux_update(fvs_ctx &C, const Upd &A, double t, const double *uin = nullptr, double *uout = nullptr) {
+  const bool cons = uin != nullptr;
   if (C.opt.recon != 1)
-    if (int rc = launch_gradient(C, true)) return rc;
+    if (int rc = launch_gradient(C, true, uin)) return rc;
   {
     ProfScope ps(C, KC_FUSED);
     const int nb = nblk(C.N);
@@ -1504,12 +1526,13 @@ int launch_flux_update(fvs_ctx &C, const Upd &A, double t) {
     const size_t lds = size_t(std::max(per * C.maxf, 4 * kBlock)) * sizeof(double);
     hipLaunchKernelGGL(k_flux_update, dim3(C.nact ? C.nact : xcd_grid(nb)), dim3(kBlock), lds, C.stream, C.N,
                        xcd_arg(nb, C.xcd_fused), C.GW, C.maxf, A, C.P, t, C.boff, C.lrec, C.lgeo, C.nlisted,
-                       C.lgat, C.W, C.grad, C.phi, C.vol, C.U0, C.Ub, C.Facc, C.Walt, C.dtl, C.part,
-                       C.nact ? C.blist : nullptr, C.ninst > 0 ? C.dpinf_inst : nullptr,
-                       A.last ? C.d2buf : nullptr, (C.opt.limiter == 0 && C.opt.recon != 1) ? 1 : 0);
+                       C.lgat, cons ? uin : C.W, C.grad, C.phi, C.vol, C.U0, cons ? uout : C.Ub, C.Facc,
+                       cons ? nullptr : C.Walt, C.dtl, C.part, C.nact ? C.blist : nullptr,
+                       C.ninst > 0 ? C.dpinf_inst : nullptr, A.last ? C.d2buf : nullptr,
+                       (C.opt.limiter == 0 && C.opt.recon != 1) ? 1 : 0, cons ? 1 : 0);
   }
   HIPC(hipGetLastError());
-  std::swap(C.W, C.Walt);
+  if (!cons) std::swap(C.W, C.Walt);
   return 0;
 }
 
@@ -1974,6 +1997,13 @@ namespace {
 // SSP-RK3 of this library).  Norm outputs go to dn / dinst at the step index
 // held in the device counter, which the step advances; so the same launch
 // sequence can be captured once in a HIP graph and replayed.
+// SSP-RK3 stages on conservative states (no primitive-state traffic): the
+// fused path with the staged-halo gradient or a first-order reconstruction
+bool cons_stages(const fvs_ctx &C) {
+  return C.fused && !C.stage && C.opt.rk_scheme == FVS_RK_SSP3 &&
+         (C.opt.recon == 1 || (C.halo && C.use_halo));
+}
+
 int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {
   const fvs_options &o = C.opt;
   const double dt = o.dt;
@@ -2017,6 +2047,9 @@ int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {
     int rc;
     if (C.stage) {
       rc = launch_stage(C, A, ts);
+    } else if (cons_stages(C)) {
+      double *const in[3] = {C.U0, C.Ub, C.Uc}, *const out[3] = {C.Ub, C.Uc, C.U0};
+      rc = launch_flux_update(C, A, ts, in[k], out[k]);
     } else if (C.fused) {
       rc = launch_flux_update(C, A, ts);
     } else {
@@ -2085,6 +2118,10 @@ int run_steps(fvs_ctx &C, double t1, int nsteps, double *dn, double *dv, double 
     if (int rc = launch_step(C, told, dn, dinst)) return rc;
     if (dv)
       if (int rc = launch_vortex_err(C, told + dt, dv + 13 * size_t(s))) return rc;
+  }
+  if (cons_stages(C)) {  // the stages wrote no primitive state
+    hipLaunchKernelGGL(k_prim, dim3(nblk(C.N)), dim3(kBlock), 0, C.stream, C.N, C.opt.gamma, C.U0, C.W);
+    HIPC(hipGetLastError());
   }
   return 0;
 }
@@ -2314,8 +2351,15 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   // grad once (phi unless there is no limiter), the cell-side traffic of the
   // update kernel without the face-flux buffer
   const bool phi1 = C->opt.limiter == 0 && C->opt.recon != 1;
-  const double bfu = double(C->nlisted) * (8 + 56) + N * (32 + 64 + (phi1 ? 0 : 8)) +
-                     N * (C->GW * 2.0 + 8 + 32 + state);
+  double bfu = double(C->nlisted) * (8 + 56) + N * (32 + 64 + (phi1 ? 0 : 8)) + N * (C->GW * 2.0 + 8 + 32 + state);
+  if (cons_stages(*C)) {
+    // conservative stages: the stage input (read once, also the update's
+    // stage state; on the first stage also its step-start state), the
+    // step-start state on the other stages, the new state; no W write
+    double st3 = (32 + 64 + 64) / 3.0;
+    if (C->opt.steady) st3 += (8.0 + 8.0 * (nst - 1)) / nst;
+    bfu = double(C->nlisted) * (8 + 56) + N * (32 + 64 + (phi1 ? 0 : 8)) + N * (C->GW * 2.0 + 8 + st3);
+  }
   // whole stage: own states, centres and G^-1 once, the per-block cell lists
   // and local stencils, face records and block lists once, limiter inputs,
   // the update's cell-side traffic

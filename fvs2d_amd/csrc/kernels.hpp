@@ -32,8 +32,11 @@ struct Phys {
 // ---------------------------------------------------------------------------
 // pointwise physics
 // ---------------------------------------------------------------------------
-// cvar -> pvar: src/data_solution.f90:72-86
+// cvar -> pvar: src/data_solution.f90:72-86.  No multiply-add contraction:
+// every kernel that converts a state (the update that writes it, or a
+// gradient / flux kernel staging a conservative state) gets the same bits.
 __device__ __forceinline__ void cons2prim(const double u[4], double g, double w[4]) {
+#pragma clang fp contract(off)
   w[0] = u[0];
   w[1] = u[1] / u[0];
   w[2] = u[2] / u[0];

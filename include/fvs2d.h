@@ -55,7 +55,9 @@ typedef struct {
   int steady;           /* local time stepping                                  */
   double dt, cfl;
   int reorder;          /* device cell order: 0 reference order, 1 Morton blocks */
-  int kernel_path;      /* 0: fastest available (fused LDS flux+update, centroid LSQ);
+  int kernel_path;      /* 0: fastest available (fused LDS flux+update, centroid LSQ;
+                              SSP-RK3 stages read conservative stage states and
+                              write no primitive state);
                            1: face-flux buffer kernels only;
                            2: whole stage in one kernel (gradients recomputed per
                               block from LDS; LSQ, centroid form, triangle-sized
