@@ -256,18 +256,23 @@ struct UpdIn {
 };
 
 // Ux: where SSP-RK3 reads the stage state (the stage-state buffer the
-// update writes in place, or the separate stage input).
+// update writes in place, or the separate stage input); with lx it is read
+// from the block's LDS copy sx (row threadIdx.x) instead, and so is the
+// step-start state on the first stage (the stage input then).
 __device__ __forceinline__ void block_update_load(int N, int i, int gw, const Upd &A, const short *__restrict__ lgat,
                                                   const double *__restrict__ vol, const double *__restrict__ Ua,
                                                   const double *__restrict__ Ux, const double *__restrict__ Facc,
-                                                  const double *__restrict__ dtl, UpdIn &in) {
+                                                  const double *__restrict__ dtl, UpdIn &in, bool lx = false,
+                                                  const double *__restrict__ sx = nullptr) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) in.sv[k] = (k < gw) ? lgat[long(k) * N + i] : 0;
   in.V = vol[i];
   in.dl = (A.steady && !A.first) ? dtl[i] : 0.0;
-  ld4(Ua, i, in.u0);
+  if (lx && A.first) ld4(sx, threadIdx.x, in.u0);
+  else ld4(Ua, i, in.u0);
   if (A.mode == M_SSP3) {
     if (A.first) for (int v = 0; v < 4; ++v) in.x[v] = in.u0[v];
+    else if (lx) ld4(sx, threadIdx.x, in.x);
     else ld4(Ux, i, in.x);
   } else {
     if (A.first) for (int v = 0; v < 4; ++v) in.x[v] = 0.0;
@@ -360,12 +365,12 @@ __device__ __forceinline__ void block_update_finish(int N, int lb, const Upd &A,
 // primitive state from Win and writes the next one to Wout (separate buffers:
 // neighbouring blocks still read Win).  phi_one: no limiter, every limiter
 // value is 1 and is not loaded.  With cons (SSP-RK3 only), Win is the
-// stage's conservative input state, converted as it is read, the update
-// takes its stage state from it and writes the new one to Ub (a buffer other
-// than Win), and no primitive state is written (Wout null).  The update
-// reads its own cell's stage input a second time rather than keeping it from
-// the staging: in registers or LDS it costs a wavefront per SIMD, which is
-// slower than the re-read.
+// stage's conservative input state and no primitive state is written (Wout
+// null): the block's own cells are staged unconverted, every face side is
+// converted as it is read (from LDS or HBM), the update takes its stage state
+// from the staged copy and writes the new one to Ub (a buffer other than
+// Win).  Converted copies would need another 8 KB of LDS or 8 registers per
+// lane, a wavefront per SIMD less.
 __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, int maxf, Upd A, Phys P, double t,
                                                         const int *__restrict__ boff, const int2 *__restrict__ lrec,
                                                         const double *__restrict__ lgeo, long L,
@@ -384,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
   double *lS = lds + 4 * maxf;  // [maxf]    wave speed*area, only when the stage sets local time steps
   const bool ws = A.steady && A.first;
   __shared__ double sG[8 * kBlock];  // gradients of the block's own cells
-  __shared__ double sW[4 * kBlock];  // and their primitive states
+  __shared__ double sW[4 * kBlock];  // and their states (primitive; with cons conservative)
   const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
   const int nbl = nb < 0 ? -nb : nb;
   if (lb >= nbl) return;  // the whole workgroup leaves before any barrier
@@ -404,7 +409,6 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
     ld8(grad, i0 + threadIdx.x, gg);
     ld4(Win, i0 + threadIdx.x, ww);
     st8(sG, threadIdx.x, gg);
-    make_prim(cons, A.gamma, ww);
     st4(sW, threadIdx.x, ww);
   }
   __syncthreads();
@@ -431,14 +435,15 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
         ld4(Win, c.y, wR);
         ld8(grad, c.y, gR);
       }
-      if (la >= unsigned(kBlock)) make_prim(cons, A.gamma, wL);
-      if (lc >= unsigned(kBlock)) make_prim(cons, A.gamma, wR);
+      make_prim(cons, A.gamma, wL);
+      make_prim(cons, A.gamma, wR);
       interior_flux_v(P, g, wL, wR, gL, gR, phi_one ? 1.0 : phi[c.x], phi_one ? 1.0 : phi[c.y], fa, wsa);
     } else {
       const int code = -1 - c.y;
       double w[4], gr[8];
       ld4(sW, c.x - i0, w);  // a boundary face's cell is one of the block's own
       ld8(sG, c.x - i0, gr);
+      make_prim(cons, A.gamma, w);
       boundary_flux_v(P, t, code & 3, g, w, gr, phi_one ? 1.0 : phi[c.x], fa, wsa,
                       pinf_inst ? pinf_inst + 4 * (code >> 2) : P.pinf);
     }
@@ -447,7 +452,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
   }
   UpdIn in;
   const int ic = i0 + threadIdx.x;
-  if (ic < N) block_update_load(N, ic, gw, A, lgat, vol, Ua, cons ? Win : Ub, Facc, dtl, in);
+  if (ic < N) block_update_load(N, ic, gw, A, lgat, vol, Ua, cons ? Win : Ub, Facc, dtl, in, cons != 0, sW);
   __syncthreads();
   block_update_finish<kBlock>(N, lb, A, in, lF, lS, Ua, Ub, Facc, Wout, dtl, part, d2out, lF);
 }
