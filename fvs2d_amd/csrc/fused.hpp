@@ -256,28 +256,35 @@ struct UpdIn {
 };
 
 // Ux: where SSP-RK3 reads the stage state (the stage-state buffer the
-// update writes in place, or the separate stage input); with lx it is read
-// from the block's LDS copy sx (row threadIdx.x) instead, and so is the
-// step-start state on the first stage (the stage input then).
+// update writes in place).  x_given: the caller fills in the SSP-RK3 stage
+// state (and on the first stage the step-start state) itself.
 __device__ __forceinline__ void block_update_load(int N, int i, int gw, const Upd &A, const short *__restrict__ lgat,
                                                   const double *__restrict__ vol, const double *__restrict__ Ua,
                                                   const double *__restrict__ Ux, const double *__restrict__ Facc,
-                                                  const double *__restrict__ dtl, UpdIn &in, bool lx = false,
-                                                  const double *__restrict__ sx = nullptr) {
+                                                  const double *__restrict__ dtl, UpdIn &in, bool x_given = false) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) in.sv[k] = (k < gw) ? lgat[long(k) * N + i] : 0;
   in.V = vol[i];
   in.dl = (A.steady && !A.first) ? dtl[i] : 0.0;
-  if (lx && A.first) ld4(sx, threadIdx.x, in.u0);
-  else ld4(Ua, i, in.u0);
+  if (!(x_given && A.first)) ld4(Ua, i, in.u0);
   if (A.mode == M_SSP3) {
-    if (A.first) for (int v = 0; v < 4; ++v) in.x[v] = in.u0[v];
-    else if (lx) ld4(sx, threadIdx.x, in.x);
-    else ld4(Ux, i, in.x);
+    if (x_given) {
+    } else if (A.first) {
+      for (int v = 0; v < 4; ++v) in.x[v] = in.u0[v];
+    } else {
+      ld4(Ux, i, in.x);
+    }
   } else {
     if (A.first) for (int v = 0; v < 4; ++v) in.x[v] = 0.0;
     else ld4(Facc, i, in.x);
   }
+}
+
+// The launch-block list of a batch with masked instances (nbl entries, the
+// blocks that hold a cell of an active instance) is followed by 8 words per
+// block: bit l set when the block's cell l belongs to an active instance.
+__device__ __forceinline__ const unsigned *active_mask(const int *blist, int nbl) {
+  return reinterpret_cast<const unsigned *>(blist + nbl);
 }
 
 template <int NT>
@@ -286,9 +293,15 @@ __device__ __forceinline__ void block_update_finish(int N, int lb, const Upd &A,
                                                     double *__restrict__ Ua, double *__restrict__ Ub,
                                                     double *__restrict__ Facc, double *__restrict__ Wout,
                                                     double *__restrict__ dtl, double *__restrict__ part,
-                                                    double *__restrict__ d2out, double *__restrict__ red) {
+                                                    double *__restrict__ d2out, double *__restrict__ red,
+                                                    const unsigned *__restrict__ amask) {
   const int i = lb * kBlock + threadIdx.x;
   double d2[4] = {0, 0, 0, 0};
+  // amask (batches with masked instances): bit l of the block's eight words
+  // is set when its cell l belongs to an active instance; a masked cell
+  // keeps its state (the update is computed, nothing is stored) and
+  // contributes zero increments
+  const bool on = !amask || threadIdx.x >= kBlock || ((amask[8 * lb + (threadIdx.x >> 5)] >> (threadIdx.x & 31)) & 1u);
   if (threadIdx.x < kBlock && i < N) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0}, wacc = 0.0;
 #pragma unroll
@@ -310,7 +323,7 @@ __device__ __forceinline__ void block_update_finish(int N, int lb, const Upd &A,
       double dl;
       if (A.first) {
         dl = A.cfl * V / (0.5 * wacc);
-        dtl[i] = dl;
+        if (on) dtl[i] = dl;
       } else {
         dl = in.dl;
       }
@@ -324,7 +337,7 @@ __device__ __forceinline__ void block_update_finish(int N, int lb, const Upd &A,
     } else if (A.mode == M_SSP42) {
       const double *fo = in.x;
       for (int v = 0; v < 4; ++v) un[v] = u0[v] + hs * (A.coef * R[v] + fo[v]);
-      if (!A.last) {
+      if (!A.last && on) {
         double fn[4];
         for (int v = 0; v < 4; ++v) fn[v] = fo[v] + R[v];
         st4(Facc, i, fn);
@@ -335,19 +348,21 @@ __device__ __forceinline__ void block_update_finish(int N, int lb, const Upd &A,
       for (int v = 0; v < 4; ++v) fn[v] = fo[v] + A.coef * R[v];
       if (A.use_acc) for (int v = 0; v < 4; ++v) un[v] = u0[v] + hs * fn[v];
       else for (int v = 0; v < 4; ++v) un[v] = u0[v] + hs * R[v];
-      if (!A.last) st4(Facc, i, fn);
+      if (!A.last && on) st4(Facc, i, fn);
     }
     if (A.last) {
-      st4(Ua, i, un);
-      for (int v = 0; v < 4; ++v) {
-        const double d = fabs(un[v] - u0[v]);
-        d2[v] = d * d;
+      if (on) {
+        st4(Ua, i, un);
+        for (int v = 0; v < 4; ++v) {
+          const double d = fabs(un[v] - u0[v]);
+          d2[v] = d * d;
+        }
       }
       if (d2out) st4(d2out, i, d2);
-    } else if (A.mode == M_SSP3) {
+    } else if (A.mode == M_SSP3 && on) {
       st4(Ub, i, un);
     }
-    if (Wout) {
+    if (Wout && on) {
       double w[4];
       cons2prim(un, A.gamma, w);
       st4(Wout, i, w);
@@ -452,9 +467,17 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
   }
   UpdIn in;
   const int ic = i0 + threadIdx.x;
-  if (ic < N) block_update_load(N, ic, gw, A, lgat, vol, Ua, cons ? Win : Ub, Facc, dtl, in, cons != 0, sW);
+  if (ic < N) {
+    block_update_load(N, ic, gw, A, lgat, vol, Ua, Ub, Facc, dtl, in, cons != 0);
+    if (cons) {  // the stage input staged in LDS (the step-start state too on the first stage)
+      ld4(sW, threadIdx.x, in.x);
+      if (A.first)
+        for (int v = 0; v < 4; ++v) in.u0[v] = in.x[v];
+    }
+  }
   __syncthreads();
-  block_update_finish<kBlock>(N, lb, A, in, lF, lS, Ua, Ub, Facc, Wout, dtl, part, d2out, lF);
+  block_update_finish<kBlock>(N, lb, A, in, lF, lS, Ua, Ub, Facc, Wout, dtl, part, d2out, lF,
+                              blist ? active_mask(blist, nbl) : nullptr);
 }
 
 // Whole RK stage in one kernel per 256-cell block: the block stages, in LDS,
@@ -702,7 +725,8 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(
     UpdIn in;
     const int ic = lb * kBlock + threadIdx.x;
     if (threadIdx.x < kBlock && ic < N) block_update_load(N, ic, gw, A, lgat, vol, Ua, Ub, Facc, dtl, in);
-    block_update_finish<kStageThreads>(N, lb, A, in, lF, lS, Ua, Ub, Facc, Wout, dtl, part, d2out, lds);
+    block_update_finish<kStageThreads>(N, lb, A, in, lF, lS, Ua, Ub, Facc, Wout, dtl, part, d2out, lds,
+                                       blist ? active_mask(blist, nbl) : nullptr);
   }
 }
 

@@ -739,6 +739,8 @@ struct fvs_ctx {
   std::vector<int> inst_of_real;     // reference order
   std::vector<int> inst_off_dev;     // device order, ninst+1
   std::vector<double> mach_aoa;      // 2 per instance
+  // blist: nblk(N) entries of launch-block list, then 8 words per block of
+  // active-cell bits (masked batches)
   int *dinst_off = nullptr, *dbinst = nullptr, *dwoff = nullptr, *blist = nullptr;
   double *dpinf_inst = nullptr, *d2buf = nullptr, *dmach_aoa = nullptr, *dforces = nullptr;
   int nact = 0;                      // active blocks (0 = all, no list)
@@ -1367,7 +1369,7 @@ int setup_device(fvs_ctx &C) {
     HIPC(hipMemcpyAsync(C.wall, wsorted.data(), wsorted.size() * sizeof(int), hipMemcpyHostToDevice, C.stream));
     if (int rc = C.alloc(C.d2buf, n4)) return rc;
     if (int rc = C.alloc(C.dforces, 4 * size_t(C.ninst))) return rc;
-    if (int rc = C.alloc(C.blist, size_t(nblk(N)))) return rc;
+    if (int rc = C.alloc(C.blist, 9 * size_t(nblk(N)))) return rc;
   }
   // phi = 1 (no limiter) or 0 (first order); grad = 0 for first order
   const double phi0 = (C.opt.recon == 1) ? 0.0 : 1.0;
@@ -2194,18 +2196,32 @@ int fvs_batch_set_active(fvs_ctx *C, const int *active) {
   bool all = true;
   for (int k = 0; k < C->ninst; ++k) all = all && active[k];
   if (!all) {
+    // launch the blocks that hold a cell of an active instance; a block that
+    // also holds masked cells leaves them alone (its mask bits are clear)
+    std::vector<unsigned> mask(8 * size_t(nb), 0u);
+    const std::vector<int> &off = C->inst_off_dev;
     for (int b = 0; b < nb; ++b) {
-      const int i0 = b * kBlock, i1 = std::min(C->N, i0 + kBlock) - 1;
-      const int k0 = int(std::upper_bound(C->inst_off_dev.begin(), C->inst_off_dev.end(), i0) - C->inst_off_dev.begin()) - 1;
-      const int k1 = int(std::upper_bound(C->inst_off_dev.begin(), C->inst_off_dev.end(), i1) - C->inst_off_dev.begin()) - 1;
+      const int i0 = b * kBlock, i1 = std::min(C->N, i0 + kBlock);
+      const int k0 = int(std::upper_bound(off.begin(), off.end(), i0) - off.begin()) - 1;
+      const int k1 = int(std::upper_bound(off.begin(), off.end(), i1 - 1) - off.begin()) - 1;
       bool on = false;
-      for (int k = k0; k <= k1 && !on; ++k) on = active[k] != 0;
+      for (int k = k0; k <= k1; ++k) {
+        if (!active[k]) continue;
+        on = true;
+        for (int i = std::max(off[k], i0); i < std::min(off[k + 1], i1); ++i)
+          mask[8 * size_t(b) + (i - i0) / 32] |= 1u << ((i - i0) % 32);
+      }
       if (on) list.push_back(b);
     }
     if (list.empty()) list.push_back(0);
     HIPC(hipMemcpyAsync(C->blist, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
-    HIPC(hipStreamSynchronize(C->stream));
+    HIPC(hipMemcpyAsync(C->blist + nb, mask.data(), mask.size() * sizeof(unsigned), hipMemcpyHostToDevice, C->stream));
   }
+  // masked cells are not updated, so on the primitive-state paths their W
+  // lags the stage parity; a resumed instance starts from its own state
+  hipLaunchKernelGGL(k_prim, dim3(nb), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(C->stream));
   C->nact = all ? 0 : int(list.size());
   return 0;
 }

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from fvs2d_amd import cases
+from fvs2d_amd._lib import RK_SSP3, RK_SSP42
 from fvs2d_amd.solver import BatchSolver, Solver
 from tests.common import rel_err, vortex_scheme
 
@@ -22,9 +23,9 @@ INST = [((48, 24), (0.75, 1.0)), ((64, 32), (0.8, 1.25)), ((32, 16), (0.6, -2.0)
 INST_RAGGED = [((40, 20), (0.75, 1.0)), ((52, 26), (0.8, 1.25)), ((36, 18), (0.6, -2.0))]
 
 
-def single(k, steps, freeze_after=None, inst=INST):
+def single(k, steps, freeze_after=None, inst=INST, rk=RK_SSP42, cfl=1.0):
     (n, l), (M, a) = inst[k]
-    m, sch = cases.naca(n, l, mach=M, aoa=a, cfl=1.0)
+    m, sch = cases.naca(n, l, mach=M, aoa=a, cfl=cfl, rk_scheme=rk)
     s = Solver(m, sch)
     s.initialize_solution(1, 0.0)
     if freeze_after is not None:
@@ -36,9 +37,9 @@ def single(k, steps, freeze_after=None, inst=INST):
     return s
 
 
-def batch(inst=INST, kernel_path=0):
-    meshes = [cases.naca(n, l, cfl=1.0)[0] for (n, l), _ in inst]
-    _, sch = cases.naca(16, 8, cfl=1.0, kernel_path=kernel_path)
+def batch(inst=INST, kernel_path=0, rk=RK_SSP42, cfl=1.0):
+    meshes = [cases.naca(n, l, cfl=cfl)[0] for (n, l), _ in inst]
+    _, sch = cases.naca(16, 8, cfl=cfl, kernel_path=kernel_path, rk_scheme=rk)
     b = BatchSolver(meshes, [ma for _, ma in inst], sch)
     b.initialize_freestream()
     return b
@@ -75,6 +76,35 @@ def test_masked_instance_is_frozen():
     for k in (0, 2):
         s = single(k, 25)
         assert np.array_equal(after[o[k]:o[k + 1]], s.cvar), k
+
+
+@pytest.mark.parametrize("kernel_path", [0, 2])
+@pytest.mark.parametrize("rk", [RK_SSP42, RK_SSP3], ids=["ssp42", "ssp3"])
+def test_masked_instance_in_shared_blocks(rk, kernel_path):
+    """Instance boundaries inside 256-cell launch blocks: a block that holds
+    cells of a masked and of an active instance is still launched, and the
+    masked cells keep their state bit for bit (SSP-RK3 runs its stages on
+    conservative states, SSP-RK(4,2) on primitive ones; kernel_path 2 is the
+    whole-stage kernel).  SSP-RK3 runs at CFL 0.3: the local time step is
+    four times the common convention and SSP-RK3 diverges at 1.0."""
+    cfl = 0.3 if rk == RK_SSP3 else 1.0
+    b = batch(INST_RAGGED, kernel_path=kernel_path, rk=rk, cfl=cfl)
+    b.batch_time_integration(0.0, 10)
+    before = b.cvar
+    assert np.all(np.isfinite(before))
+    b.set_active([1, 0, 1])
+    b.batch_time_integration(0.0, 15)
+    after = b.cvar
+    o = b.cell_offsets
+    assert np.array_equal(after[o[1]:o[2]], before[o[1]:o[2]])
+    for k in (0, 2):
+        s = single(k, 25, inst=INST_RAGGED, rk=rk, cfl=cfl)
+        assert np.array_equal(after[o[k]:o[k + 1]], s.cvar), k
+    # unmasked again, the instance resumes from where it stopped
+    b.set_active([1, 1, 1])
+    b.batch_time_integration(0.0, 5)
+    s = single(1, 15, inst=INST_RAGGED, rk=rk, cfl=cfl)
+    assert np.array_equal(b.instance_state(1), s.cvar)
 
 
 @pytest.mark.parametrize("case", ["naca_ssp42", "vortex_periodic_ssp3"])
