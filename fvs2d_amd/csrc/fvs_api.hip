@@ -244,35 +244,32 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggcb(int nbx, int N, int nfw, c
   }
 }
 
-// GGNB gradient (src/gradient_ggnb.f90:351-378):
-// grad_i = (coef0 q_i + sum_incidences (coefedg*coefnb) q_j) / vol
-template <int LIM>
-__global__ __launch_bounds__(kBlock) void k_grad_ggnb(int nbx, int N, const double *__restrict__ W,
-                                                      const double2 *__restrict__ c0, const int *__restrict__ goff,
-                                                      const int *__restrict__ gwid, const int *__restrict__ gnb,
-                                                      const double2 *__restrict__ gc, const double *__restrict__ vol,
-                                                      double *__restrict__ grad, double *__restrict__ phi, int lt,
-                                                      const int *__restrict__ nvrt, const double2 *__restrict__ loff,
-                                                      const int *__restrict__ soff, const int *__restrict__ swid,
-                                                      const int *__restrict__ snb) {
-  const int i = xcd_block(nbx) * kBlock + threadIdx.x;
-  if (i >= N) return;
-  const int s = i / kSlice, l = i % kSlice;
-  double w[4], gx[4], gy[4];
-  ld4(W, i, w);
-  const double2 a = c0[i];
-  for (int v = 0; v < 4; ++v) gx[v] = a.x * w[v], gy[v] = a.y * w[v];
-  const int base = goff[s], wid = gwid[s];
-  // four entries per round: ids and coefficients, then states, in flight
-  // together; the sum keeps the entry order
+// GGNB gradient (src/gradient_ggnb.f90:351-378) in two passes.  The
+// reference's per-cell sum coef0 q_i + sum_nodes coefedg(in) sum_j coefnb q_j
+// regroups by node into sum_nodes coefedg(in) F(node), F(v) = sum over the
+// cells j around v of idw(v)/d(j, v) q_j: an inverse-distance average
+// formed once per node, 12 B per incidence, instead of 20 B per cell-node-
+// cell incidence.  Agrees with the reference to rounding (the grouping).
+//
+// Pass 1: node values, node incidences in 64-node slices (entry k of lane l
+// of slice s at nsoff[s] + 64 k + l; padding: a cell of the node, weight 0).
+__global__ __launch_bounds__(kBlock) void k_ggnb_nodes(int NV, const double *__restrict__ W,
+                                                       const int *__restrict__ nsoff, const int *__restrict__ nswid,
+                                                       const int *__restrict__ nscell,
+                                                       const double *__restrict__ nscoef, double *__restrict__ F) {
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= NV) return;
+  const int s = v / kSlice, l = v % kSlice;
+  const int base = nsoff[s], wid = nswid[s];
+  double f[4] = {0.0, 0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < wid; k0 += 4) {
     int j[4];
-    double2 c[4];
+    double c[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const long e = base + long(k0 + u) * kSlice + l;
-      j[u] = (k0 + u < wid) ? gnb[e] : i;
-      c[u] = (k0 + u < wid) ? gc[e] : make_double2(0.0, 0.0);
+      j[u] = (k0 + u < wid) ? nscell[e] : nscell[base + l];
+      c[u] = (k0 + u < wid) ? nscoef[e] : 0.0;
     }
     double q[4][4];
 #pragma unroll
@@ -280,17 +277,50 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggnb(int nbx, int N, const doub
 #pragma unroll
     for (int u = 0; u < 4; ++u)
       if (k0 + u < wid)
-        for (int v = 0; v < 4; ++v) {
-          gx[v] = gx[v] + c[u].x * q[u][v];
-          gy[v] = gy[v] + c[u].y * q[u][v];
-        }
+        for (int v2 = 0; v2 < 4; ++v2) f[v2] = f[v2] + c[u] * q[u][v2];
   }
+  st4(F, v, f);
+}
+
+// Pass 2: grad_i = sum over the cell's nodes (gw slots, planes [k][N]; a
+// padded slot has weight 0) of coefedg * F(node), / vol; with LIM the limiter
+// over the node-neighbour stencil.
+template <int LIM>
+__global__ __launch_bounds__(kBlock) void k_grad_ggnb(int nbx, int N, int gw, const double *__restrict__ W,
+                                                      const int *__restrict__ cnode, const double2 *__restrict__ cedg,
+                                                      const double *__restrict__ F, const double *__restrict__ vol,
+                                                      double *__restrict__ grad, double *__restrict__ phi, int lt,
+                                                      const int *__restrict__ nvrt, const double2 *__restrict__ loff,
+                                                      const int *__restrict__ soff, const int *__restrict__ swid,
+                                                      const int *__restrict__ snb) {
+  const int i = xcd_block(nbx) * kBlock + threadIdx.x;
+  if (i >= N) return;
+  int nd[4];
+  double2 c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    nd[k] = (k < gw) ? cnode[long(k) * N + i] : 0;
+    c[k] = (k < gw) ? cedg[long(k) * N + i] : make_double2(0.0, 0.0);
+  }
+  double f[4][4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < gw) ld4(F, nd[k], f[k]);
+  double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < gw)
+      for (int v = 0; v < 4; ++v) {
+        gx[v] = gx[v] + c[k].x * f[k][v];
+        gy[v] = gy[v] + c[k].y * f[k][v];
+      }
   const double V = vol[i];
   for (int v = 0; v < 4; ++v) gx[v] = gx[v] / V, gy[v] = gy[v] / V;
   double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
   st8(grad, i, g8);
   if (LIM) {
-    double mn[4], mx[4];
+    double w[4], mn[4], mx[4];
+    ld4(W, i, w);
     stencil_minmax(i, soff, swid, snb, W, w, mn, mx);
     phi[i] = cell_limiter(lt, nvrt[i], N, i, loff, V, w, mn, mx, gx, gy);
   }
@@ -683,8 +713,13 @@ struct fvs_ctx {
   double2 *scw = nullptr;
   double2 *ggcb0 = nullptr, *ggcbc = nullptr;
   int *ggcbnb = nullptr;
-  double2 *ggnb0 = nullptr, *ggnbc = nullptr;
-  int *goff = nullptr, *gwid = nullptr, *gnb = nullptr;
+  // GGNB: node values F (NV nodes, ordered by their first device cell),
+  // node incidences in slices, per cell gw node slots and coefedg
+  int NV = 0, gwn = 0;
+  int *nsoff = nullptr, *nswid = nullptr, *nscell = nullptr, *gcnode = nullptr;
+  double *nscoef = nullptr, *gF = nullptr;
+  double2 *gcedg = nullptr;
+  long nnz_node = 0;
   int *cell_intr = nullptr, *wall = nullptr;
   int ncell_intr = 0, nwall = 0;
   // centroid-based LSQ gradient (k_grad_lsqc)
@@ -1304,17 +1339,70 @@ int setup_device(fvs_ctx &C) {
     if (int rc = C.upload(C.ggcbc, cn)) return rc;
   }
   if (C.opt.grad_method == 2) {
-    std::vector<double2> c0(N);
-    for (int i = 0; i < N; ++i) c0[i] = make_double2(m.ggnb_coef0[2 * P[i]], m.ggnb_coef0[2 * P[i] + 1]);
-    std::vector<int> off, wid, nb;
-    std::vector<double2> c2;
-    if (!pack_sell(N, P, IP, m.ggnb_ptr, m.ggnb_cell, m.ggnb_c.data(), off, wid, nb, c2))
-      return fail(FVS_EINVAL, "mesh too large: GGNB stencil entries exceed 32-bit offsets");
-    if (int rc = C.upload(C.ggnb0, c0)) return rc;
-    if (int rc = C.upload(C.goff, off)) return rc;
-    if (int rc = C.upload(C.gwid, wid)) return rc;
-    if (int rc = C.upload(C.gnb, nb)) return rc;
-    if (int rc = C.upload(C.ggnbc, c2)) return rc;
+    // nodes that touch a cell, ordered by their first device cell so that a
+    // slice of nodes reads a compact range of cells and vice versa
+    const int Vn = m.nnodes;
+    std::vector<int> first(Vn, INT32_MAX);
+    for (int v = 0; v < Vn; ++v)
+      for (int q = m.n2c_ptr[v]; q < m.n2c_ptr[v + 1]; ++q) first[v] = std::min(first[v], IP[m.n2c[q]]);
+    std::vector<int> nodes;
+    for (int v = 0; v < Vn; ++v)
+      if (first[v] < INT32_MAX) nodes.push_back(v);
+    std::stable_sort(nodes.begin(), nodes.end(), [&](int a, int b) { return first[a] < first[b]; });
+    const int NV = int(nodes.size());
+    std::vector<int> dnode(Vn, -1);
+    for (int k = 0; k < NV; ++k) dnode[nodes[k]] = k;
+    // node incidences in 64-node slices, the reference's cell order per node
+    const int ns = (NV + kSlice - 1) / kSlice;
+    std::vector<int> noff(ns + 1, 0), nwid(ns, 0);
+    long total = 0;
+    for (int sl = 0; sl < ns; ++sl) {
+      int w = 0;
+      for (int l = 0; l < kSlice && sl * kSlice + l < NV; ++l) {
+        const int v = nodes[sl * kSlice + l];
+        w = std::max(w, m.n2c_ptr[v + 1] - m.n2c_ptr[v]);
+      }
+      nwid[sl] = w;
+      total += long(w) * kSlice;
+      if (total > long(INT32_MAX)) return fail(FVS_EINVAL, "mesh too large: GGNB node incidences exceed 32-bit offsets");
+      noff[sl + 1] = int(total);
+    }
+    std::vector<int> ncell(size_t(total), 0);
+    std::vector<double> ncoef(size_t(total), 0.0);
+    for (int sl = 0; sl < ns; ++sl)
+      for (int l = 0; l < kSlice; ++l) {
+        const int k = sl * kSlice + l;
+        const int v = k < NV ? nodes[k] : nodes[0];
+        const int b = m.n2c_ptr[v], n = m.n2c_ptr[v + 1] - b;
+        for (int e = 0; e < nwid[sl]; ++e) {
+          const size_t at = size_t(noff[sl]) + size_t(e) * kSlice + l;
+          ncell[at] = IP[m.n2c[b + (e < n ? e : 0)]];  // padding: a cell of the node, weight 0
+          ncoef[at] = (k < NV && e < n) ? m.ggnb_ncoef[b + e] : 0.0;
+        }
+      }
+    // per cell: its nodes (the reference's local order) and coefedg
+    int gwn = 0;
+    for (int c = 0; c < N; ++c) gwn = std::max(gwn, m.nvrt[c]);
+    std::vector<int> cnode(size_t(gwn) * N, 0);
+    std::vector<double2> cedg(size_t(gwn) * N, make_double2(0.0, 0.0));
+    for (int i = 0; i < N; ++i) {
+      const int c = P[i];
+      for (int k = 0; k < gwn; ++k) {
+        const bool real = k < m.nvrt[c];
+        cnode[size_t(k) * N + i] = dnode[m.cnodec[4 * c + (real ? k : 0)]];
+        if (real) cedg[size_t(k) * N + i] = make_double2(m.ggnb_edg[8 * size_t(c) + 2 * k], m.ggnb_edg[8 * size_t(c) + 2 * k + 1]);
+      }
+    }
+    C.NV = NV;
+    C.gwn = gwn;
+    C.nnz_node = long(m.n2c.size());
+    if (int rc = C.upload(C.nsoff, noff)) return rc;
+    if (int rc = C.upload(C.nswid, nwid)) return rc;
+    if (int rc = C.upload(C.nscell, ncell)) return rc;
+    if (int rc = C.upload(C.nscoef, ncoef)) return rc;
+    if (int rc = C.upload(C.gcnode, cnode)) return rc;
+    if (int rc = C.upload(C.gcedg, cedg)) return rc;
+    if (int rc = C.alloc(C.gF, 4 * size_t(NV))) return rc;
   }
   // interior cells (vortex error), device ids, reference order
   std::vector<int> ci(m.cell_intr.size());
@@ -1473,12 +1561,14 @@ int launch_gradient(fvs_ctx &C, bool with_limiter, const double *src = nullptr) 
       hipLaunchKernelGGL(k_grad_ggcb<0>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.NFW, C.W, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
                          C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
   } else {
+    hipLaunchKernelGGL(k_ggnb_nodes, dim3(nblk(C.NV)), b, 0, C.stream, C.NV, C.W, C.nsoff, C.nswid, C.nscell,
+                       C.nscoef, C.gF);
     if (lim)
-      hipLaunchKernelGGL(k_grad_ggnb<1>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.W, C.ggnb0, C.goff, C.gwid, C.gnb, C.ggnbc, C.vol,
-                         C.grad, C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
+      hipLaunchKernelGGL(k_grad_ggnb<1>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.gwn, C.W, C.gcnode,
+                         C.gcedg, C.gF, C.vol, C.grad, C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
     else
-      hipLaunchKernelGGL(k_grad_ggnb<0>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.W, C.ggnb0, C.goff, C.gwid, C.gnb, C.ggnbc, C.vol,
-                         C.grad, C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
+      hipLaunchKernelGGL(k_grad_ggnb<0>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.gwn, C.W, C.gcnode,
+                         C.gcedg, C.gF, C.vol, C.grad, C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
   }
   HIPC(hipGetLastError());
   return 0;
@@ -2345,7 +2435,9 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   else if (C->lsqc) bg = N * (32 + 16 + 24 + 64) + 4.0 * m.lsq_ptr.back();
   else if (C->opt.grad_method == 3) bg = N * (32 + 64) + 20.0 * m.lsq_ptr.back();
   else if (C->opt.grad_method == 1) bg = N * (32 + 16 + 8 + 64) + N * C->NFW * 20.0;
-  else bg = N * (32 + 16 + 8 + 64) + 20.0 * m.ggnb_ptr.back();
+  // GGNB: pass 1 reads the node incidences and the states, writes F; pass 2
+  // reads its node slots and coefedg, F, vol, writes grad
+  else bg = 12.0 * double(C->nnz_node) + N * 32 + 2.0 * 32 * C->NV + N * (20.0 * C->gwn + 8 + 64);
   if (lim) {
     bg += N * (8 + 4) + 16.0 * (3 * m.ntri + 4 * m.nquad);
     if (C->opt.grad_method == 3) bg += 0.0;

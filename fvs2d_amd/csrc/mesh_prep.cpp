@@ -422,8 +422,10 @@ void ggcb_setup(MeshPrep &m) {
 
 void ggnb_setup(MeshPrep &m) {
   // src/gradient_ggnb.f90:217-345: Green-Gauss over node values that are
-  // inverse-distance averages of the cells around each node, folded into
-  // cell-to-cell coefficients
+  // inverse-distance averages of the cells around each node.  The reference
+  // folds the node values into cell-to-cell coefficients (its coef0 is the
+  // cell's own share of its node values); kept here as the two factors:
+  // per node incidence idw(v)/d(cell, v), per cell and node coefedg.
   const int N = m.ncells, V = m.nnodes;
   std::vector<double> idw(V, 0.0);
   for (int v = 0; v < V; ++v) {
@@ -438,26 +440,19 @@ void ggnb_setup(MeshPrep &m) {
     }
     idw[v] = 1.0 / idt;
   }
-  m.ggnb_coef0.assign(2 * N, 0.0);
-  std::vector<double> coefedg(8 * size_t(N), 0.0);
-  for (int c = 0; c < N; ++c) {
-    for (int e = 0; e < m.nvrt[c]; ++e) {
-      int je = m.cedge[4 * c + e];
-      double af = m.ea[je], nxf = m.enx[je] * m.nrmlsign[4 * c + e], nyf = m.eny[je] * m.nrmlsign[4 * c + e];
-      int v1 = m.canon[m.en1[je]], v2 = m.canon[m.en2[je]];
+  m.ggnb_ncoef.assign(m.n2c.size(), 0.0);
+  for (int v = 0; v < V; ++v)
+    for (int q = m.n2c_ptr[v]; q < m.n2c_ptr[v + 1]; ++q) {
+      int c = m.n2c[q];
       double xv, yv;
-      node_in_cell(m, c, v1, xv, yv);
+      node_in_cell(m, c, v, xv, yv);
       double dx = m.cx[c] - xv, dy = m.cy[c] - yv;
-      double w1 = 1.0 / std::sqrt(dx * dx + dy * dy);
-      node_in_cell(m, c, v2, xv, yv);
-      dx = m.cx[c] - xv;
-      dy = m.cy[c] - yv;
-      double w2 = 1.0 / std::sqrt(dx * dx + dy * dy);
-      m.ggnb_coef0[2 * c] = m.ggnb_coef0[2 * c] + af * nxf / 2.0 * (w1 * idw[v1] + w2 * idw[v2]);
-      m.ggnb_coef0[2 * c + 1] = m.ggnb_coef0[2 * c + 1] + af * nyf / 2.0 * (w1 * idw[v1] + w2 * idw[v2]);
+      m.ggnb_ncoef[q] = idw[v] / std::sqrt(dx * dx + dy * dy);
     }
-    // coefedg(in) = sum over the cell's edges touching node in of af*n/2,
-    // edge ends visited n1 then n2 (src/gradient_ggnb.f90:321-342)
+  // coefedg(in) = sum over the cell's edges touching node in of af*n/2,
+  // edge ends visited n1 then n2 (src/gradient_ggnb.f90:321-342)
+  m.ggnb_edg.assign(8 * size_t(N), 0.0);
+  for (int c = 0; c < N; ++c)
     for (int in = 0; in < m.nvrt[c]; ++in) {
       int iv = m.cnodec[4 * c + in];
       for (int e = 0; e < m.nvrt[c]; ++e) {
@@ -466,34 +461,11 @@ void ggnb_setup(MeshPrep &m) {
           int jv = m.canon[j == 0 ? m.en1[je] : m.en2[je]];
           if (jv != iv) continue;
           double af = m.ea[je], nxf = m.enx[je] * m.nrmlsign[4 * c + e], nyf = m.eny[je] * m.nrmlsign[4 * c + e];
-          coefedg[8 * size_t(c) + 2 * in] = coefedg[8 * size_t(c) + 2 * in] + af * nxf / 2.0;
-          coefedg[8 * size_t(c) + 2 * in + 1] = coefedg[8 * size_t(c) + 2 * in + 1] + af * nyf / 2.0;
+          m.ggnb_edg[8 * size_t(c) + 2 * in] = m.ggnb_edg[8 * size_t(c) + 2 * in] + af * nxf / 2.0;
+          m.ggnb_edg[8 * size_t(c) + 2 * in + 1] = m.ggnb_edg[8 * size_t(c) + 2 * in + 1] + af * nyf / 2.0;
         }
       }
     }
-  }
-  // incidences (cell, coefedg(in)*coefnb) in node-then-cell order, the cell
-  // itself (coefnb = 0) dropped
-  m.ggnb_ptr.assign(N + 1, 0);
-  m.ggnb_cell.clear();
-  m.ggnb_c.clear();
-  for (int c = 0; c < N; ++c) {
-    for (int in = 0; in < m.nvrt[c]; ++in) {
-      int iv = m.cnodec[4 * c + in];
-      for (int q = m.n2c_ptr[iv]; q < m.n2c_ptr[iv + 1]; ++q) {
-        int j = m.n2c[q];
-        if (j == c) continue;
-        double xv, yv;
-        node_in_cell(m, j, iv, xv, yv);
-        double dx = m.cx[j] - xv, dy = m.cy[j] - yv;
-        double cnb = idw[iv] / std::sqrt(dx * dx + dy * dy);
-        m.ggnb_cell.push_back(j);
-        m.ggnb_c.push_back(coefedg[8 * size_t(c) + 2 * in] * cnb);
-        m.ggnb_c.push_back(coefedg[8 * size_t(c) + 2 * in + 1] * cnb);
-      }
-    }
-    m.ggnb_ptr[c + 1] = int(m.ggnb_cell.size());
-  }
 }
 
 }  // namespace

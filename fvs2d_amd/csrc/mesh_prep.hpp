@@ -42,11 +42,9 @@ struct MeshPrep {
   // GGCB: per local edge (ptr, coefnb(2)); coef0(2) per cell
   std::vector<int> ggcb_ptr;
   std::vector<double> ggcb_coef0, ggcb_coefnb;
-  // GGNB: coef0(2) per cell; per node incidence (cell, coefedg*coefnb (2)),
-  // incidences with coefnb == 0 (the cell itself) dropped
-  std::vector<double> ggnb_coef0;
-  std::vector<int> ggnb_ptr, ggnb_cell;
-  std::vector<double> ggnb_c;
+  // GGNB: per node incidence (aligned with n2c) idw(v)/d(cell, v); per cell
+  // and local node (4 slots) coefedg (2)
+  std::vector<double> ggnb_ncoef, ggnb_edg;
 };
 
 // Builds everything; returns "" on success or an error message.
