@@ -519,13 +519,28 @@ __global__ __launch_bounds__(kBlock) void k_update(int nbx, int N, int gw, Upd A
 // residual norms of one step: sqrt(sum_i (U-U0)^2 / N) per variable
 // (src/runge_kutta.f90:169-184), partials summed in block order
 constexpr int kNormThreads = 1024;
+// (bump: also advance the step counter, which the caller then skips)
+constexpr int kNormLoads = 8;
 __global__ __launch_bounds__(kNormThreads) void k_norm_final(int nparts, int N, const double *__restrict__ part,
-                                                             double *__restrict__ out, const int *__restrict__ ctr) {
+                                                             double *__restrict__ out, int *__restrict__ ctr,
+                                                             int bump) {
   __shared__ double red[4 * kNormThreads];
   const int t = threadIdx.x;
   double s[4] = {0, 0, 0, 0};
-  for (int p = t; p < nparts; p += kNormThreads)
-    for (int v = 0; v < 4; ++v) s[v] = s[v] + part[4 * p + v];
+  // thread t adds partials t, t + 1024, ... in that order; eight loads are in
+  // flight before the adds (absent ones are exact zeros)
+  for (int p0 = t; p0 < nparts; p0 += kNormLoads * kNormThreads) {
+    double q[kNormLoads][4];
+#pragma unroll
+    for (int u = 0; u < kNormLoads; ++u) {
+      const int p = p0 + u * kNormThreads;
+      if (p < nparts) ld4(part, p, q[u]);
+      else q[u][0] = q[u][1] = q[u][2] = q[u][3] = 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kNormLoads; ++u)
+      for (int v = 0; v < 4; ++v) s[v] = s[v] + q[u][v];
+  }
   for (int v = 0; v < 4; ++v) red[v * kNormThreads + t] = s[v];
   __syncthreads();
   for (int h = kNormThreads / 2; h > 0; h >>= 1) {
@@ -536,6 +551,7 @@ __global__ __launch_bounds__(kNormThreads) void k_norm_final(int nparts, int N, 
   if (t == 0) {
     double *o = out + 4 * (ctr ? ctr[0] : 0);
     for (int v = 0; v < 4; ++v) o[v] = sqrt(red[v * kNormThreads] / double(N));
+    if (bump && ctr) ctr[0] = ctr[0] + 1;
   }
 }
 
@@ -2150,14 +2166,17 @@ int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {
     }
     if (rc) return rc;
   }
+  // the step counter (the norms' output row) advances after its last reader
+  const bool inst = dinst && C.ninst > 0;
   if (!C.nact) {
     ProfScope ps(C, KC_NORM);
-    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kNormThreads), 0, C.stream, nblk(C.N), C.N, C.part, dn, C.dctr);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kNormThreads), 0, C.stream, nblk(C.N), C.N, C.part, dn, C.dctr,
+                       inst ? 0 : 1);
   }
-  if (dinst && C.ninst > 0)
+  if (inst)
     hipLaunchKernelGGL(k_inst_norms, dim3(C.ninst), dim3(kBlock), 0, C.stream, C.ninst, C.dinst_off, C.d2buf, C.dctr,
                        dinst);
-  hipLaunchKernelGGL(k_counter, dim3(1), dim3(1), 0, C.stream, C.dctr, 1, 1);
+  if (C.nact || inst) hipLaunchKernelGGL(k_counter, dim3(1), dim3(1), 0, C.stream, C.dctr, 1, 1);
   HIPC(hipGetLastError());
   return 0;
 }
