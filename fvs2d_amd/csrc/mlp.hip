@@ -337,6 +337,23 @@ __device__ __forceinline__ void put_pairs(u16 *__restrict__ T, int col0, int mp,
   }
 }
 
+// Workgroup -> (k-tile, row slice) of the weight-gradient kernels.  Workgroups
+// go to the eight XCDs round-robin by index; the nkt k-tiles of one slice read
+// the same D rows, so they are given indices eight apart (one XCD, whose L2
+// then serves D to all of them) and dispatched close together.  Needs the
+// slice count to be a multiple of 8, else the plain order is used.
+__device__ __forceinline__ void tn_tile(int nkt, int nsplit, int &kt, int &s) {
+  const int b = int(blockIdx.x);
+  if (nsplit % 8 == 0) {
+    const int r = b & 7, q = b >> 3;
+    kt = q % nkt;
+    s = (q / nkt) * 8 + r;
+  } else {
+    kt = b % nkt;
+    s = b / nkt;
+  }
+}
+
 // Weight-gradient partials: for k-tile kt (64 rows of dW) and row slice s,
 // dW[k][n] = sum over m in slice of A[m][k] * D[m][n], A[M][K], D[M][256].
 // Both operands are transposed into LDS (m contiguous); a thread moves two
@@ -351,7 +368,8 @@ __global__ __launch_bounds__(256) void k_gemm_tn(int K, int Kreal, int chunk, co
   __shared__ __attribute__((aligned(16))) u16 Bs[kW * kLds];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int nkt = K / kRows;
-  const int kt = blockIdx.x % nkt, s = blockIdx.x / nkt;
+  int kt, s;
+  tn_tile(nkt, int(gridDim.x) / nkt, kt, s);
   const long m0 = long(s) * chunk;
   const int kb = kt * kRows;
   f32x4 acc[4][4];
@@ -427,7 +445,8 @@ __global__ __launch_bounds__(256 * kTnGroups) void k_gemm_tn_g(int K, int Kreal,
   const int tid = threadIdx.x & 255, grp = threadIdx.x >> 8, lane = tid & 63, w = tid >> 6;
   u16 *As = smem + grp * kTnBuf, *Bs = As + kRows * kLds;
   const int nkt = K / kRows;
-  const int kt = blockIdx.x % nkt, s = blockIdx.x / nkt;
+  int kt, s;
+  tn_tile(nkt, int(gridDim.x) / nkt, kt, s);
   const long m0 = long(s) * chunk + grp * kBK;
   const int kb = kt * kRows;
   f32x4 acc[4][4];
