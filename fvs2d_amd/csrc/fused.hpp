@@ -268,11 +268,9 @@ __device__ __forceinline__ void block_update_load(int N, int i, int gw, const Up
   in.dl = (A.steady && !A.first) ? dtl[i] : 0.0;
   if (!(x_given && A.first)) ld4(Ua, i, in.u0);
   if (A.mode == M_SSP3) {
-    if (x_given) {
-    } else if (A.first) {
-      for (int v = 0; v < 4; ++v) in.x[v] = in.u0[v];
-    } else {
-      ld4(Ux, i, in.x);
+    if (!x_given) {
+      if (A.first) for (int v = 0; v < 4; ++v) in.x[v] = in.u0[v];
+      else ld4(Ux, i, in.x);
     }
   } else {
     if (A.first) for (int v = 0; v < 4; ++v) in.x[v] = 0.0;
@@ -280,9 +278,10 @@ __device__ __forceinline__ void block_update_load(int N, int i, int gw, const Up
   }
 }
 
-// The launch-block list of a batch with masked instances (nbl entries, the
-// blocks that hold a cell of an active instance) is followed by 8 words per
-// block: bit l set when the block's cell l belongs to an active instance.
+// The launch-block list of a batch with masked instances (nbl slots, the
+// first gridDim.x of them the launched blocks: those holding a cell of an
+// active instance) is followed by 8 words per block: bit l set when the
+// block's cell l belongs to an active instance.
 __device__ __forceinline__ const unsigned *active_mask(const int *blist, int nbl) {
   return reinterpret_cast<const unsigned *>(blist + nbl);
 }
