@@ -203,6 +203,7 @@ __device__ __forceinline__ void stencil_minmax(int i, const int *__restrict__ so
 // grad_i = (coef0 q_i + sum_f coefnb_f q_nb(f)) / vol
 template <int LIM>
 __global__ __launch_bounds__(kBlock) void k_grad_ggcb(int nbx, int N, int nfw, const double *__restrict__ W,
+                                                      int cons, double gamma,
                                                       const double2 *__restrict__ c0, const int *__restrict__ nb,
                                                       const double2 *__restrict__ cn, const double *__restrict__ vol,
                                                       double *__restrict__ grad, double *__restrict__ phi, int lt,
@@ -226,6 +227,12 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggcb(int nbx, int N, int nfw, c
   const double V = vol[i];
 #pragma unroll
   for (int k = 0; k < 4; ++k) ld4(W, j[k], q[k]);
+  // without a limiter W may hold conservative states (SSP-RK3 stages)
+  const bool cv = !LIM && cons;
+  make_prim(cv, gamma, w);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < nfw) make_prim(cv, gamma, q[k]);
   for (int v = 0; v < 4; ++v) gx[v] = a.x * w[v], gy[v] = a.y * w[v];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
@@ -253,8 +260,9 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggcb(int nbx, int N, int nfw, c
 //
 // Pass 1: node values, node incidences in 64-node slices (entry k of lane l
 // of slice s at nsoff[s] + 64 k + l; padding: a cell of the node, weight 0).
-__global__ __launch_bounds__(kBlock) void k_ggnb_nodes(int NV, const double *__restrict__ W,
-                                                       const int *__restrict__ nsoff, const int *__restrict__ nswid,
+__global__ __launch_bounds__(kBlock) void k_ggnb_nodes(int NV, const double *__restrict__ W, int cons,
+                                                       double gamma, const int *__restrict__ nsoff,
+                                                       const int *__restrict__ nswid,
                                                        const int *__restrict__ nscell,
                                                        const double *__restrict__ nscoef, double *__restrict__ F) {
   const int v = blockIdx.x * kBlock + threadIdx.x;
@@ -274,6 +282,8 @@ __global__ __launch_bounds__(kBlock) void k_ggnb_nodes(int NV, const double *__r
     double q[4][4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) ld4(W, j[u], q[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) make_prim(cons, gamma, q[u]);  // conservative states (SSP-RK3 stages)
 #pragma unroll
     for (int u = 0; u < 4; ++u)
       if (k0 + u < wid)
@@ -1518,7 +1528,11 @@ int launch_gradient(fvs_ctx &C, bool with_limiter, const double *src = nullptr) 
   const bool lim = with_limiter && C.opt.limiter > 0 && C.opt.recon != 1 && !C.limiter_frozen;
   const int lt = C.opt.limiter;
   dim3 g(xcd_grid(nblk(N))), b(kBlock);
-  if (src && !(C.halo && C.use_halo)) return fail(FVS_ESTATE, "conservative gradient input needs the staged halo");
+  const bool gg = C.opt.grad_method == 1 || C.opt.grad_method == 2;
+  if (src && !(C.halo && C.use_halo) && !(gg && !lim))
+    return fail(FVS_ESTATE, "conservative gradient input needs the staged halo or an unlimited Green-Gauss gradient");
+  const double *Wg = src ? src : C.W;
+  const int cons = src ? 1 : 0;
   ProfScope ps(C, KC_GRAD);
   if (C.halo && C.use_halo) {
     if (C.nact) g = dim3(C.nact);
@@ -1571,14 +1585,14 @@ int launch_gradient(fvs_ctx &C, bool with_limiter, const double *src = nullptr) 
                          C.nvrt, C.loff, C.vol);
   } else if (C.opt.grad_method == 1) {
     if (lim)
-      hipLaunchKernelGGL(k_grad_ggcb<1>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.NFW, C.W, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
+      hipLaunchKernelGGL(k_grad_ggcb<1>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.NFW, C.W, 0, C.opt.gamma, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
                          C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
     else
-      hipLaunchKernelGGL(k_grad_ggcb<0>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.NFW, C.W, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
+      hipLaunchKernelGGL(k_grad_ggcb<0>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.NFW, Wg, cons, C.opt.gamma, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
                          C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
   } else {
-    hipLaunchKernelGGL(k_ggnb_nodes, dim3(nblk(C.NV)), b, 0, C.stream, C.NV, C.W, C.nsoff, C.nswid, C.nscell,
-                       C.nscoef, C.gF);
+    hipLaunchKernelGGL(k_ggnb_nodes, dim3(nblk(C.NV)), b, 0, C.stream, C.NV, Wg, lim ? 0 : cons, C.opt.gamma,
+                       C.nsoff, C.nswid, C.nscell, C.nscoef, C.gF);
     if (lim)
       hipLaunchKernelGGL(k_grad_ggnb<1>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.gwn, C.W, C.gcnode,
                          C.gcedg, C.gF, C.vol, C.grad, C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
@@ -2106,10 +2120,13 @@ namespace {
 // held in the device counter, which the step advances; so the same launch
 // sequence can be captured once in a HIP graph and replayed.
 // SSP-RK3 stages on conservative states (no primitive-state traffic): the
-// fused path with the staged-halo gradient or a first-order reconstruction
+// fused path with the staged-halo LSQ gradient, an unlimited Green-Gauss
+// gradient (kernel_path 3 keeps primitive states) or a first-order
+// reconstruction
 bool cons_stages(const fvs_ctx &C) {
+  const bool gg = (C.opt.grad_method == 1 || C.opt.grad_method == 2) && C.opt.limiter == 0 && C.opt.kernel_path != 3;
   return C.fused && !C.stage && C.opt.rk_scheme == FVS_RK_SSP3 &&
-         (C.opt.recon == 1 || (C.halo && C.use_halo));
+         (C.opt.recon == 1 || (C.halo && C.use_halo) || gg);
 }
 
 int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {

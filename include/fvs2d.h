@@ -57,13 +57,15 @@ typedef struct {
   int reorder;          /* device cell order: 0 reference order, 1 Morton blocks */
   int kernel_path;      /* 0: fastest available (fused LDS flux+update, centroid LSQ;
                               SSP-RK3 stages read conservative stage states and
-                              write no primitive state);
+                              write no primitive state, with the LSQ gradient or an
+                              unlimited Green-Gauss one);
                            1: face-flux buffer kernels only;
                            2: whole stage in one kernel (gradients recomputed per
                               block from LDS; LSQ, centroid form, triangle-sized
                               blocks; falls back to 0 where it does not apply);
                            3: as 0 without the per-block halo data (the LSQ
-                              gradient gathers its stencil cell by cell) */
+                              gradient gathers its stencil cell by cell) and with
+                              SSP-RK3 stages on primitive states */
 } fvs_options;
 
 const char *fvs_last_error(void);
@@ -96,7 +98,8 @@ int fvs_create_batch(fvs_ctx **out, int ninst, const int *cell_offsets, const do
  * NULL) receives each instance's log_res.plt norms per step */
 int fvs_batch_time_integration(fvs_ctx *ctx, double t1, int nsteps, double *inst_norms);
 /* active[k] != 0 keeps instance k stepping; the others are left out of the
- * launches (blocks holding only inactive instances are skipped) */
+ * launches (blocks holding only inactive instances are skipped, and a block
+ * shared with an active instance leaves their cells untouched) */
 int fvs_batch_set_active(fvs_ctx *ctx, const int *active);
 /* C_L, C_D, C_L(cell), C_D(cell) per instance (4*ninst), src/io.f90:340-449 */
 int fvs_batch_wall_forces(fvs_ctx *ctx, double *out);

@@ -1,8 +1,9 @@
 """SSP-RK3 on the default path runs its stages on conservative states: the
 gradient and flux kernels read the stage's conservative state and convert it
 to primitive variables as they stage it, and no primitive state is written
-between stages (U0 -> Ub -> Uc -> U0).  kernel_path 3 (no block halo) keeps
-the primitive-state arrangement, so the two must agree bitwise: each face
+between stages (U0 -> Ub -> Uc -> U0), with the staged-halo LSQ gradient and
+with unlimited Green-Gauss gradients.  kernel_path 3 keeps the primitive-state
+arrangement, so the two must agree bitwise: each face
 flux, gradient and cell sum is the same arithmetic on the same bits, and the
 conversion is compiled without multiply-add contraction wherever it runs.
 
@@ -51,6 +52,23 @@ def test_conservative_stages_match_primitive_path(square, limiter):
     # limited schemes amplify rounding (the oracle's own 1-ulp spread), so
     # the bar there is looser than the unlimited 1e-10
     assert rel_err(ua, orc.cvar) <= (1e-10 if limiter == 0 else 1e-8)
+
+
+@pytest.mark.parametrize("gm", [1, 2], ids=["ggcb", "ggnb"])
+def test_green_gauss_conservative_stages(square, gm):
+    """Unlimited Green-Gauss gradients take the conservative states too
+    (converted in their gathers); kernel_path 3 keeps primitive states."""
+    sch = vortex_scheme(dt=0.01, grad_method=gm)
+    a = run(square, sch, 25)
+    b = run(square, sch, 25, kernel_path=3)
+    ua, ub = a.cvar, b.cvar
+    a.close()
+    b.close()
+    assert np.array_equal(ua, ub)
+    orc = Oracle(square, **sch.oracle_kwargs())
+    orc.initialize(1, 0.0)
+    orc.time_integration(0.0, 25)
+    assert rel_err(ua, orc.cvar) <= 1e-10
 
 
 def test_graph_replay_of_conservative_stages(square):
