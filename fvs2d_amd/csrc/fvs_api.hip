@@ -251,6 +251,97 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggcb(int nbx, int N, int nfw, c
   }
 }
 
+// Limiter bounds of a Green-Gauss gradient from a staged halo: the block's
+// own states (local 0..255) and the outside cells of its node-neighbour
+// stencils (from 256) are staged in LDS, and each cell walks its stencil as
+// 16-bit local codes (two per 32-bit word, the LSQ staged-halo layout) in LDS
+// instead of gathering the states from HBM.  min and max are exact, so the
+// bounds are those of the stencil walk in HBM (stencil_minmax).
+__device__ __forceinline__ void gg_stage_halo(int lb, int N, const double *__restrict__ W,
+                                              const int *__restrict__ ghoff, const int *__restrict__ ghlist,
+                                              double *__restrict__ lW) {
+  const int i0 = lb * kBlock, id = i0 + threadIdx.x;
+  if (id < N) {
+    double w[4];
+    ld4(W, id, w);
+    st4(lW, threadIdx.x, w);
+  }
+  const int o2 = ghoff[lb], nh = ghoff[lb + 1] - o2;
+  for (int q = threadIdx.x; q < nh; q += kBlock) {
+    double w[4];
+    ld4(W, ghlist[o2 + q], w);
+    st4(lW, kBlock + q, w);
+  }
+}
+__device__ __forceinline__ void gg_halo_minmax(const unsigned *__restrict__ lc, int wid,
+                                               const double *__restrict__ lW, const double w[4], double mn[4],
+                                               double mx[4]) {
+  for (int v = 0; v < 4; ++v) mn[v] = mx[v] = w[v];
+  for (int k = 0; k < wid; ++k) {
+    const int j = int(((lc[(k >> 1) * kSlice] >> (16 * (k & 1))) & 0xffffu) >> 4);
+    double q[4];
+    ld4(lW, j, q);
+    for (int v = 0; v < 4; ++v) {
+      mn[v] = (q[v] < mn[v]) ? q[v] : mn[v];
+      mx[v] = (q[v] > mx[v]) ? q[v] : mx[v];
+    }
+  }
+}
+
+// GGCB gradient with the limiter, bounds from the staged halo (one block of
+// 256 cells per workgroup; LDS 32 B per local cell)
+__global__ __launch_bounds__(kBlock) void k_grad_ggcb_h(int nbx, int N, int nfw, const double *__restrict__ W,
+                                                        const double2 *__restrict__ c0, const int *__restrict__ nb,
+                                                        const double2 *__restrict__ cn, const double *__restrict__ vol,
+                                                        double *__restrict__ grad, double *__restrict__ phi, int lt,
+                                                        const int *__restrict__ nvrt, const double2 *__restrict__ loff,
+                                                        const int *__restrict__ ghoff, const int *__restrict__ ghlist,
+                                                        const int *__restrict__ poff, const int *__restrict__ swid,
+                                                        const unsigned *__restrict__ lcode) {
+  extern __shared__ double lds[];
+  const int lb = xcd_block(nbx);
+  if (lb >= (nbx < 0 ? -nbx : nbx)) return;  // the whole workgroup leaves before the barrier
+  const int i = lb * kBlock + threadIdx.x;
+  const bool act = i < N;
+  // the gradient's global loads are in flight while the block is staged
+  int j[4];
+  double2 c[4];
+  double w[4], q[4][4];
+  double2 a = make_double2(0.0, 0.0);
+  double V = 1.0;
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      j[k] = (k < nfw) ? nb[long(k) * N + i] : i;
+      c[k] = (k < nfw) ? cn[long(k) * N + i] : make_double2(0.0, 0.0);
+    }
+    ld4(W, i, w);
+    a = c0[i];
+    V = vol[i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ld4(W, j[k], q[k]);
+  }
+  gg_stage_halo(lb, N, W, ghoff, ghlist, lds);
+  __syncthreads();
+  if (!act) return;
+  double gx[4], gy[4];
+  for (int v = 0; v < 4; ++v) gx[v] = a.x * w[v], gy[v] = a.y * w[v];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < nfw)
+      for (int v = 0; v < 4; ++v) {
+        gx[v] = gx[v] + c[k].x * q[k][v];
+        gy[v] = gy[v] + c[k].y * q[k][v];
+      }
+  for (int v = 0; v < 4; ++v) gx[v] = gx[v] / V, gy[v] = gy[v] / V;
+  double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
+  st8(grad, i, g8);
+  const int s = i / kSlice;
+  double mn[4], mx[4];
+  gg_halo_minmax(lcode + poff[s] + threadIdx.x % kSlice, swid[s], lds, w, mn, mx);
+  phi[i] = cell_limiter(lt, nvrt[i], N, i, loff, V, w, mn, mx, gx, gy);
+}
+
 // GGNB gradient (src/gradient_ggnb.f90:351-378) in two passes.  The
 // reference's per-cell sum coef0 q_i + sum_nodes coefedg(in) sum_j coefnb q_j
 // regroups by node into sum_nodes coefedg(in) F(node), F(v) = sum over the
@@ -334,6 +425,56 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggnb(int nbx, int N, int gw, co
     stencil_minmax(i, soff, swid, snb, W, w, mn, mx);
     phi[i] = cell_limiter(lt, nvrt[i], N, i, loff, V, w, mn, mx, gx, gy);
   }
+}
+
+// GGNB pass 2 with the limiter, bounds from the staged halo
+__global__ __launch_bounds__(kBlock) void k_grad_ggnb_h(int nbx, int N, int gw, const double *__restrict__ W,
+                                                        const int *__restrict__ cnode, const double2 *__restrict__ cedg,
+                                                        const double *__restrict__ F, const double *__restrict__ vol,
+                                                        double *__restrict__ grad, double *__restrict__ phi, int lt,
+                                                        const int *__restrict__ nvrt, const double2 *__restrict__ loff,
+                                                        const int *__restrict__ ghoff, const int *__restrict__ ghlist,
+                                                        const int *__restrict__ poff, const int *__restrict__ swid,
+                                                        const unsigned *__restrict__ lcode) {
+  extern __shared__ double lds[];
+  const int lb = xcd_block(nbx);
+  if (lb >= (nbx < 0 ? -nbx : nbx)) return;  // the whole workgroup leaves before the barrier
+  const int i = lb * kBlock + threadIdx.x;
+  const bool act = i < N;
+  int nd[4];
+  double2 c[4];
+  double f[4][4], w[4];
+  double V = 1.0;
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      nd[k] = (k < gw) ? cnode[long(k) * N + i] : 0;
+      c[k] = (k < gw) ? cedg[long(k) * N + i] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < gw) ld4(F, nd[k], f[k]);
+    ld4(W, i, w);
+    V = vol[i];
+  }
+  gg_stage_halo(lb, N, W, ghoff, ghlist, lds);
+  __syncthreads();
+  if (!act) return;
+  double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < gw)
+      for (int v = 0; v < 4; ++v) {
+        gx[v] = gx[v] + c[k].x * f[k][v];
+        gy[v] = gy[v] + c[k].y * f[k][v];
+      }
+  for (int v = 0; v < 4; ++v) gx[v] = gx[v] / V, gy[v] = gy[v] / V;
+  double g8[8] = {gx[0], gx[1], gx[2], gx[3], gy[0], gy[1], gy[2], gy[3]};
+  st8(grad, i, g8);
+  const int s = i / kSlice;
+  double mn[4], mx[4];
+  gg_halo_minmax(lcode + poff[s] + threadIdx.x % kSlice, swid[s], lds, w, mn, mx);
+  phi[i] = cell_limiter(lt, nvrt[i], N, i, loff, V, w, mn, mx, gx, gy);
 }
 
 // interior faces: reconstruction (src/residual.f90:87-92) + Roe (95)
@@ -1344,6 +1485,14 @@ int setup_device(fvs_ctx &C) {
           if (int rc = setup_stage(C, h_boff, h_bfaces, fic, fbc, off, wid, nb, code)) return rc;
       }
     }
+    // a limited Green-Gauss gradient takes its bounds from a staged halo of
+    // the same (node-neighbour) stencil: codes are the device ids (the
+    // bounds need no periodic shift)
+    if ((C.opt.grad_method == 1 || C.opt.grad_method == 2) && C.opt.limiter > 0 && C.opt.recon != 1 &&
+        C.opt.kernel_path != 3 && N < (1 << 27)) {
+      std::vector<unsigned> code(nb.begin(), nb.end());
+      if (int rc = setup_grad_halo(C, off, wid, code)) return rc;
+    }
   }
   if (C.opt.grad_method == 1) {
     std::vector<double2> c0(N), cn(size_t(C.NFW) * N, make_double2(0.0, 0.0));
@@ -1529,12 +1678,14 @@ int launch_gradient(fvs_ctx &C, bool with_limiter, const double *src = nullptr) 
   const int lt = C.opt.limiter;
   dim3 g(xcd_grid(nblk(N))), b(kBlock);
   const bool gg = C.opt.grad_method == 1 || C.opt.grad_method == 2;
-  if (src && !(C.halo && C.use_halo) && !(gg && !lim))
+  if (src && !(C.opt.grad_method == 3 && C.halo && C.use_halo) && !(gg && !lim))
     return fail(FVS_ESTATE, "conservative gradient input needs the staged halo or an unlimited Green-Gauss gradient");
   const double *Wg = src ? src : C.W;
   const int cons = src ? 1 : 0;
   ProfScope ps(C, KC_GRAD);
-  if (C.halo && C.use_halo) {
+  const bool ghalo = gg && lim && C.halo && C.use_halo;  // limited Green-Gauss, bounds from the staged halo
+  const size_t glds = size_t(4) * (kBlock + C.maxh) * sizeof(double);
+  if (C.opt.grad_method == 3 && C.halo && C.use_halo) {
     if (C.nact) g = dim3(C.nact);
     const int nb = nblk(N);
     const int nl = kBlock + C.maxh;
@@ -1584,7 +1735,11 @@ int launch_gradient(fvs_ctx &C, bool with_limiter, const double *src = nullptr) 
       hipLaunchKernelGGL(k_grad_lsq<0>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.W, C.soff, C.swid, C.snb, C.scw, C.grad, C.phi, lt,
                          C.nvrt, C.loff, C.vol);
   } else if (C.opt.grad_method == 1) {
-    if (lim)
+    if (ghalo)
+      hipLaunchKernelGGL(k_grad_ggcb_h, g, b, glds, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.NFW, C.W, C.ggcb0,
+                         C.ggcbnb, C.ggcbc, C.vol, C.grad, C.phi, lt, C.nvrt, C.loff, C.ghoff, C.ghlist, C.poff,
+                         C.swid, C.lcode);
+    else if (lim)
       hipLaunchKernelGGL(k_grad_ggcb<1>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.NFW, C.W, 0, C.opt.gamma, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
                          C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
     else
@@ -1593,7 +1748,11 @@ int launch_gradient(fvs_ctx &C, bool with_limiter, const double *src = nullptr) 
   } else {
     hipLaunchKernelGGL(k_ggnb_nodes, dim3(nblk(C.NV)), b, 0, C.stream, C.NV, Wg, lim ? 0 : cons, C.opt.gamma,
                        C.nsoff, C.nswid, C.nscell, C.nscoef, C.gF);
-    if (lim)
+    if (ghalo)
+      hipLaunchKernelGGL(k_grad_ggnb_h, g, b, glds, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.gwn, C.W,
+                         C.gcnode, C.gcedg, C.gF, C.vol, C.grad, C.phi, lt, C.nvrt, C.loff, C.ghoff, C.ghlist,
+                         C.poff, C.swid, C.lcode);
+    else if (lim)
       hipLaunchKernelGGL(k_grad_ggnb<1>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.gwn, C.W, C.gcnode,
                          C.gcedg, C.gF, C.vol, C.grad, C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
     else
@@ -2126,7 +2285,7 @@ namespace {
 bool cons_stages(const fvs_ctx &C) {
   const bool gg = (C.opt.grad_method == 1 || C.opt.grad_method == 2) && C.opt.limiter == 0 && C.opt.kernel_path != 3;
   return C.fused && !C.stage && C.opt.rk_scheme == FVS_RK_SSP3 &&
-         (C.opt.recon == 1 || (C.halo && C.use_halo) || gg);
+         (C.opt.recon == 1 || (C.opt.grad_method == 3 && C.halo && C.use_halo) || gg);
 }
 
 int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {
@@ -2467,7 +2626,8 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   const double N = C->N, Fi = C->Fi, Fb = C->Fb;
   const bool lim = C->opt.limiter > 0 && C->opt.recon != 1;
   double bg = 0.0;
-  if (C->halo && C->use_halo) bg = N * (32 + 16 + 24 + 64) + 4.0 * double(C->nghl) + 4.0 * double(C->nlcode);
+  if (C->opt.grad_method == 3 && C->halo && C->use_halo)
+    bg = N * (32 + 16 + 24 + 64) + 4.0 * double(C->nghl) + 4.0 * double(C->nlcode);
   else if (C->lsqc) bg = N * (32 + 16 + 24 + 64) + 4.0 * m.lsq_ptr.back();
   else if (C->opt.grad_method == 3) bg = N * (32 + 64) + 20.0 * m.lsq_ptr.back();
   else if (C->opt.grad_method == 1) bg = N * (32 + 16 + 8 + 64) + N * C->NFW * 20.0;
@@ -2477,6 +2637,7 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   if (lim) {
     bg += N * (8 + 4) + 16.0 * (3 * m.ntri + 4 * m.nquad);
     if (C->opt.grad_method == 3) bg += 0.0;
+    else if (C->halo && C->use_halo) bg += 4.0 * double(C->nghl) + 4.0 * double(C->nlcode);  // staged halo
     else bg += 4.0 * m.lsq_ptr.back() + 8.0 * (C->opt.grad_method == 1 ? N : 0.0);
   }
   const double bfi = Fi * (8 + 56 + 40) + N * (32 + 64 + 8);
