@@ -409,8 +409,9 @@ class Solver:
 
     def autotune(self, nsteps: int = 10):
         """Pick plain or XCD-aware workgroup numbering per kernel class (bit
-        0) and, for the LSQ gradient, whether it reads its stencil from a
-        per-block halo staged in LDS (bit 1); the state is left as it was.
+        0) and, for the LSQ gradient or a limited Green-Gauss gradient's
+        limiter, whether it reads its stencil from a per-block halo staged in
+        LDS (bit 1); the state is left as it was.
         Returns (choices, best ms per step)."""
         ch = np.zeros(4, np.int32)
         best = np.zeros(1)

@@ -209,9 +209,10 @@ int fvs_batch_freeze_limiter(fvs_ctx *ctx, const int *frozen);
  * and keeps, per kernel class [gradient, face flux, gather+update, fused
  * flux+update], the faster of plain and XCD-aware workgroup numbering.  The
  * state is restored afterwards.  choice4 (may be NULL) receives the choices
- * (1 = XCD-aware; for the gradient, bit 1 set = the gradient reads its stencil
- * from a per-block halo staged in LDS, the other candidate it tries), best_ms
- * (may be NULL) the best time per step. */
+ * (1 = XCD-aware; for the gradient, bit 1 set = the LSQ gradient, or a
+ * limited Green-Gauss gradient's limiter, reads its stencil from a per-block
+ * halo staged in LDS, the other candidate it tries), best_ms (may be NULL) the
+ * best time per step. */
 int fvs_autotune(fvs_ctx *ctx, int nsteps, int *choice4, double *best_ms);
 /* wall-clock of the last fvs_time_integration call's device work (ms) */
 double fvs_last_elapsed_ms(const fvs_ctx *ctx);
