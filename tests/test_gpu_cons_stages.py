@@ -102,3 +102,19 @@ def test_primitive_state_after_integration(square):
     g_fresh = fresh.compute_gradient_cellcntr()
     fresh.close()
     assert np.array_equal(g_after, g_fresh)
+
+
+@pytest.mark.parametrize("gm", [1, 2], ids=["ggcb", "ggnb"])
+@pytest.mark.parametrize("limiter", [1, 2], ids=["venk", "barth"])
+def test_green_gauss_limiter_halo_bitwise(square, gm, limiter):
+    """A limited Green-Gauss gradient takes its limiter bounds from the
+    block's staged halo on the default path and from the stencil walk in HBM
+    on kernel_path 3; min and max are exact, so the states agree bitwise."""
+    sch = vortex_scheme(dt=0.01, grad_method=gm, limiter=limiter)
+    a = run(square, sch, 20)
+    b = run(square, sch, 20, kernel_path=3)
+    ua, ub = a.cvar, b.cvar
+    a.close()
+    b.close()
+    assert np.all(np.isfinite(ua))
+    assert np.array_equal(ua, ub)
