@@ -1,7 +1,8 @@
 /*
  * fvs_oracle.c -- CPU restatement of the fvs2d residual / time-step path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker, and the timed CPU baseline).  The
+ * TEST INFRASTRUCTURE ONLY (the results the tests compare with, and the timed
+ * CPU baseline).  The
  * product never links, loads or calls this file.
  *
  * It restates, loop for loop and in the reference's summation order, the

@@ -2,8 +2,8 @@
  * fvs_oracle.h -- CPU restatement of the fvs2d residual / time-step path.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker
- * (or the timed CPU baseline).  The product (fvs2d_amd, libfvs2d_hip.so) never
+ * cpu_baseline leg of bench.py may load this library, and only for the results
+ * the tests compare with (or as the timed CPU baseline).  The product (fvs2d_amd, libfvs2d_hip.so) never
  * links, loads or calls it.
  *
  * Every routine restates one reference routine, cited file:line against the

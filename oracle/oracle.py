@@ -1,7 +1,8 @@
 """ctypes wrapper of the CPU oracle (test infrastructure only).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
-this module, as the checker or as the timed CPU baseline.
+this module: as the reference results the tests compare with, or as the timed
+CPU baseline.
 """
 from __future__ import annotations
 

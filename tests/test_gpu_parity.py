@@ -4,7 +4,7 @@ Tolerances (FP64; the GPU contracts multiply-adds into FMAs, the oracle does
 not, so results agree to rounding, not bitwise):
   * one residual evaluation: max|d| <= 1e-12 * max|ref| per variable;
   * trajectories: |d| / |ref| <= 1e-10 per cell and variable after the
-    stated number of steps (the north-star parity bar);
+    stated number of steps (the parity bar of this path);
   * limiter values: |d| <= 1e-10 (Barth divides by a face increment that
     carries the rounding difference), scaled by max|phi| where the Van Albada
     form returns values far outside [0, 1].
