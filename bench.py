@@ -194,7 +194,7 @@ def main():
                          "multiply), 3 times, and print its rate: the calibration of the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the multi-GPU "
+        print(f"bench.py: this benchmark runs on one GPU only (asked for {args.gpus}); the multi-GPU "
               "decomposition is a design, not an implementation", file=sys.stderr)
         sys.exit(2)
     if args.copy_mib:

@@ -13,7 +13,7 @@ def pytest_configure(config):
 
 
 def _has_gpu():
-    """A GPU box exposes /dev/kfd; probing through HIP would initialise the
+    """A machine with an AMD GPU exposes /dev/kfd; probing through HIP would initialise the
     runtime in the test process, so only the device node is checked."""
     return os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK | os.W_OK)
 

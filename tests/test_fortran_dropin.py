@@ -1,6 +1,6 @@
 """The Fortran drop-in modules (fvs2d_amd/fortran) driven like the reference
 drives its own: data_solution_init, gradient_init, runge_kutta_init,
-compute_residual, time_integration, gradient_cellcntr_1var.  The driver is a
+compute_residual, time_integration, gradient_cellcntr_1var.  The Fortran test program is a
 shared library (tests/fortran, built by __graft_entry__.build) called in this
 process; its results must equal the Python binding's bit for bit (same library,
 same options), and the Python binding is checked against the oracle elsewhere."""
