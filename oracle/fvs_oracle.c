@@ -36,6 +36,24 @@
  * than one; every cell's arithmetic is the serial one */
 #define ORC_PAR _Pragma("omp parallel for if(o->nthr > 1) num_threads(o->nthr > 1 ? o->nthr : 1) schedule(static)")
 
+/* Planted mistakes, one per build, for the test that shows each pin fails on
+ * a plausible error (tests/test_oracle_mutants.py compiles throw-away
+ * variants with -DORC_MUTANT=k into a temporary directory; the oracle library
+ * itself is built with ORC_MUTANT 0, i.e. none of them). */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define MUT_GGCB_SWAP_WEIGHTS 1   /* d0/(d0+d1) on the cell, d1/(d0+d1) on the neighbour */
+#define MUT_GGNB_IDW_SQUARE 2     /* node weights 1/d^2 instead of 1/d */
+#define MUT_GGNB_SELF_TWICE 3     /* the self incidence kept in coefnb as well as in coef0 */
+#define MUT_LIM_FACE_STENCIL 4    /* limiter bounds over face neighbours only */
+#define MUT_LIM_GRAD_SWAP 5       /* face extrapolation with d/dx and d/dy exchanged */
+#define MUT_LIM_RHO_ONLY 6        /* phi of the density alone instead of the min over variables */
+#define MUT_SLIP_NO_FACTOR2 7     /* slip-wall ghost with zero instead of reversed normal velocity */
+#define MUT_RK2_SWAP_BC 8         /* order-2 table with b and c exchanged */
+#define MUT_RK1_SWAP_AD 9         /* order-1 table with a and d exchanged */
+#define MUT_RK3_H3 10             /* third classical stage advanced by dt/2 instead of dt */
+
 static char g_err[512];
 const char *orc_last_error(void) { return g_err; }
 static void set_err(const char *fmt, ...) {
@@ -557,6 +575,9 @@ static void setup_ggcb(orc_t *o) {
       }
       dx = xf - xc1; dy = yf - yc1;
       double d1 = sqrt(dx * dx + dy * dy);
+#if ORC_MUTANT == MUT_GGCB_SWAP_WEIGHTS
+      { double tmp = d0; d0 = d1; d1 = tmp; }
+#endif
       o->ggcb_coef0[2 * ic] = o->ggcb_coef0[2 * ic] + d1 / (d0 + d1) * nxf * af;
       o->ggcb_coef0[2 * ic + 1] = o->ggcb_coef0[2 * ic + 1] + d1 / (d0 + d1) * nyf * af;
       o->ggcb_coefnb[8 * ic + 2 * ie] = d0 / (d0 + d1) * nxf * af;
@@ -578,6 +599,9 @@ static void setup_ggnb(orc_t *o) {
       cell_node_xy(o, ic, v, &xv, &yv);
       double dx = o->cx[ic] - xv, dy = o->cy[ic] - yv;
       double d = sqrt(dx * dx + dy * dy);
+#if ORC_MUTANT == MUT_GGNB_IDW_SQUARE
+      d = d * d;
+#endif
       idt = idt + 1.0 / d;
     }
     idw[v] = 1.0 / idt;
@@ -598,6 +622,10 @@ static void setup_ggnb(orc_t *o) {
       cell_node_xy(o, ic, iv2, &xv, &yv);
       dx = o->cx[ic] - xv; dy = o->cy[ic] - yv;
       double w2 = 1.0 / sqrt(dx * dx + dy * dy);
+#if ORC_MUTANT == MUT_GGNB_IDW_SQUARE
+      w1 = w1 * w1;
+      w2 = w2 * w2;
+#endif
       double wt1 = idw[iv1], wt2 = idw[iv2];
       o->ggnb_coef0[2 * ic] = o->ggnb_coef0[2 * ic] + af * nxf / 2.0 * (w1 * wt1 + w2 * wt2);
       o->ggnb_coef0[2 * ic + 1] = o->ggnb_coef0[2 * ic + 1] + af * nyf / 2.0 * (w1 * wt1 + w2 * wt2);
@@ -626,10 +654,15 @@ static void setup_ggnb(orc_t *o) {
         cell_node_xy(o, jc, iv, &xv, &yv);
         double dx = o->cx[jc] - xv, dy = o->cy[jc] - yv;
         double d = sqrt(dx * dx + dy * dy);
+#if ORC_MUTANT == MUT_GGNB_IDW_SQUARE
+        d = d * d;
+#endif
         o->ggnb_cell[i] = jc;
         o->ggnb_in[i] = in;
         o->ggnb_coefnb[i] = idw[iv] / d;
+#if ORC_MUTANT != MUT_GGNB_SELF_TWICE
         if (jc == ic) o->ggnb_coefnb[i] = 0.0;
+#endif
         i++;
       }
     }
@@ -926,17 +959,32 @@ void orc_limiter(orc_t *o) {
     for (int ic = 0; ic < N; ++ic) {
       double q = o->pvar[4 * ic + iv];
       double pmin = q, pmax = q;
+#if ORC_MUTANT == MUT_LIM_FACE_STENCIL
+      for (int ie = 0; ie < o->nvrt[ic]; ++ie) {
+        int je = o->cedge[4 * ic + ie];
+        int jc = (o->ec1[je] == ic) ? o->ec2[je] : o->ec1[je];
+        if (jc < 0) continue;
+        double qj = o->pvar[4 * jc + iv];
+        pmin = (qj < pmin) ? qj : pmin;
+        pmax = (qj > pmax) ? qj : pmax;
+      }
+#else
       for (int i = o->lsq_ptr[ic]; i < o->lsq_ptr[ic + 1]; ++i) {
         double qj = o->pvar[4 * o->lsq_cell[i] + iv];
         pmin = (qj < pmin) ? qj : pmin;
         pmax = (qj > pmax) ? qj : pmax;
       }
+#endif
       double pe[4] = {100.0, 100.0, 100.0, 100.0};
       for (int ie = 0; ie < o->nvrt[ic]; ++ie) {
         int je = o->cedge[4 * ic + ie];
         double xc, yc;
         side_centroid(o, je, ic, &xc, &yc);
+#if ORC_MUTANT == MUT_LIM_GRAD_SWAP
+        double pf = q + (o->ex[je] - xc) * gy[4 * ic + iv] + (o->ey[je] - yc) * gx[4 * ic + iv];
+#else
         double pf = q + (o->ex[je] - xc) * gx[4 * ic + iv] + (o->ey[je] - yc) * gy[4 * ic + iv];
+#endif
         double dmax = pmax - q, dmin = pmin - q, diff = pf - q;
         if (diff > 0.0) pe[ie] = orc_limiter_fn(o->op.limiter, dmax, diff, o->vol[ic]);
         else if (diff < 0.0) pe[ie] = orc_limiter_fn(o->op.limiter, dmin, diff, o->vol[ic]);
@@ -950,7 +998,9 @@ void orc_limiter(orc_t *o) {
   ORC_PAR
   for (int ic = 0; ic < N; ++ic) {
     double m = phi4[4 * ic];
+#if ORC_MUTANT != MUT_LIM_RHO_ONLY
     for (int k = 1; k < 4; ++k) m = (phi4[4 * ic + k] < m) ? phi4[4 * ic + k] : m;
+#endif
     o->phi[ic] = m;
   }
   free(phi4);
@@ -1089,8 +1139,13 @@ int orc_bc_state(const orc_t *o, int bc_type, double t, double x, double y, doub
   } else if (bc_type == ORC_BC_SLIP_WALL) {
     for (int k = 0; k < 4; ++k) pfR[k] = pfL[k];
     double un = pfL[1] * nx + pfL[2] * ny;
+#if ORC_MUTANT == MUT_SLIP_NO_FACTOR2
+    pfR[1] = pfL[1] - 1.0 * un * nx;
+    pfR[2] = pfL[2] - 1.0 * un * ny;
+#else
     pfR[1] = pfL[1] - 2.0 * un * nx;
     pfR[2] = pfL[2] - 2.0 * un * ny;
+#endif
   } else if (bc_type == ORC_BC_DIRICHLET) {
     if (o->op.lvortex) orc_vortex_state(o, t, x, y, pfR);
     else orc_mms_state(o, x, y, pfR, NULL);
@@ -1317,7 +1372,15 @@ void orc_time_integration(orc_t *o, double t1, int nsteps, double *norms, double
     case 3: ak = 0.65; bk = 2.7; ck = 2.0; dk = 0.65; break;
     default: break;
   }
+#if ORC_MUTANT == MUT_RK2_SWAP_BC
+  if (o->op.rk_order == 2) { double tmp = bk; bk = ck; ck = tmp; }
+#elif ORC_MUTANT == MUT_RK1_SWAP_AD
+  if (o->op.rk_order == 1) { double tmp = ak; ak = dk; dk = tmp; }
+#endif
   double h_rk[4] = {dt / 2.0, dt / 2.0, dt, dt / 6.0}, rk_coef[4] = {ak, bk, ck, dk};
+#if ORC_MUTANT == MUT_RK3_H3
+  if (o->op.rk_order == 3) h_rk[2] = dt / 2.0;
+#endif
   double dts[4] = {0, 0, 0, 0}, dte[4] = {0, 0, 0, 0};
   if (o->op.rk_scheme == ORC_RK_SSP42) {
     h_rk[0] = h_rk[1] = h_rk[2] = dt / 3.0; h_rk[3] = dt / 4.0;

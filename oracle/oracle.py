@@ -47,15 +47,24 @@ def build() -> str:
     return LIB
 
 
-_lib = None
+def build_variant(mutant: int, outdir: str) -> str:
+    """Compile the oracle with planted mistake `mutant` (the MUT_* switches of
+    fvs_oracle.c) into outdir; for the test that shows the pins catch them."""
+    out = os.path.join(outdir, f"libfvs_oracle_mut{mutant}.so")
+    cmd = ["gcc", "-O1", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-shared",
+           f"-DORC_MUTANT={int(mutant)}", "-o", out, os.path.join(HERE, "fvs_oracle.c"), "-lm", "-fopenmp"]
+    subprocess.run(cmd, check=True)
+    return out
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+_libs = {}
+
+
+def lib(path: str = LIB):
+    if path not in _libs:
+        if path == LIB and not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         P = C.c_void_p
         dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
         ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -85,8 +94,8 @@ def lib():
         L.orc_gradient_1var.argtypes = [P, dp, dp]
         L.orc_lsq_nnz.restype = C.c_int
         L.orc_lsq_nnz.argtypes = [P]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def make_opts(**kw) -> Opts:
@@ -104,10 +113,11 @@ def make_opts(**kw) -> Opts:
 
 
 class Oracle:
-    """One oracle instance over a mesh (fvs2d_amd.meshgen.Mesh)."""
+    """One oracle instance over a mesh (fvs2d_amd.meshgen.Mesh); lib_path
+    selects another build of the oracle (a planted-mistake variant)."""
 
-    def __init__(self, mesh, **opts):
-        L = lib()
+    def __init__(self, mesh, lib_path: str = LIB, **opts):
+        L = lib(lib_path)
         self.L = L
         self.opts = make_opts(**opts)
         self.mesh = mesh

@@ -11,13 +11,24 @@ different form, invariants or brute force:
   gradient_lsq        exactness for random linear fields (fn, nn, p=0,1,2),
                       weighted least squares by numpy lstsq
   gradient_ggcb/ggnb  zero for constants, exact for linear fields on an
-                      equilateral mesh, first-order convergence on a sine
+                      equilateral mesh, first-order convergence on a sine;
+                      GGCB exact for linear fields on graded rectangles (face
+                      midpoints on the centre segment at unequal distances);
+                      both equal to their textbook forms computed from raw
+                      node data on a jittered mesh (tests/pin_checks.py)
   limiter             Venkatakrishnan's published form, Barth's min(1,a/b),
-                      constant-field branch, Van Albada special values
+                      constant-field branch, Van Albada special values; the
+                      assembly (bounds, face extrapolation, min over faces and
+                      variables) against a raw-data restatement
   compute_residual    free-stream preservation, discrete conservation, MMS
-                      truncation-error convergence
+                      truncation-error convergence; slip-wall ghost by the
+                      closed-form mirrored-state Roe flux
   runge_kutta         one step of each scheme equals its stability polynomial
-                      applied through the linearised residual
+                      applied through the linearised residual (classical
+                      orders 1-4 from the stage structure of the table)
+
+tests/test_oracle_mutants.py builds the oracle with one planted mistake at a
+time and shows the checks of tests/pin_checks.py fail on each.
   mms / vortex        closed forms of the isentropic vortex
   wall forces         closed-form C_N / C_A of a linear pressure field
 """
@@ -28,6 +39,7 @@ import pytest
 
 from fvs2d_amd import meshgen
 from oracle.oracle import Oracle, limiter_fn, roe
+from tests import pin_checks
 from tests.common import vortex_scheme
 from tests.golden_meshes import load_example
 
@@ -275,7 +287,25 @@ def test_gradient_converges(gm, jitter, rate):
     assert np.all(rates > rate), (errs, rates)
 
 
+def test_ggcb_exact_on_graded_rectangles():
+    pin_checks.check_ggcb_graded_exact()
+
+
+@pytest.mark.parametrize("gm", [1, 2])
+def test_gg_equals_raw_data_textbook_form(gm):
+    (pin_checks.check_ggcb_brute if gm == 1 else pin_checks.check_ggnb_brute)()
+
+
 # ----------------------------------------------------------------------------- limiter
+@pytest.mark.parametrize("gm,lim", [(3, 1), (3, 2), (1, 1), (2, 2)])
+def test_limiter_assembly_raw_data(gm, lim):
+    pin_checks.check_limiter_brute(grad_method=gm, limiter=lim)
+
+
+def test_slip_wall_closed_form():
+    pin_checks.check_slip_wall_closed_form()
+
+
 def test_venkatakrishnan_published_form():
     """phi = [(D+^2 + e^2) D- + 2 D-^2 D+] / [D+^2 + 2 D-^2 + D+ D- + e^2] / D-
     (Venkatakrishnan 1993), e^2 = (K h)^3, K = 5, h = 2 sqrt(V/pi)."""
@@ -408,6 +438,11 @@ def test_rk_stability_polynomial(scheme, coeffs):
     assert np.max(np.abs(got - pred)) / np.max(np.abs(pred)) < 1e-4
     # the next polynomial coefficient matters: (dt J)^3 d is not negligible
     assert np.max(np.abs(v)) > 1e-3 * np.max(np.abs(pred))
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_classical_rk_tables(order):
+    pin_checks.check_rk_polynomial(order=order)
 
 
 # ----------------------------------------------------------------------------- vortex, forces
