@@ -303,6 +303,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "device_ms_per_step": round(dev / args.steps * 1e3, 4),
+        "autotune_ms_per_step": round(tuned_ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -949,6 +949,27 @@ struct fvs_ctx {
   // step counter for norm outputs (graph replays), graph mode switch
   int *dctr = nullptr;
   bool use_graph = false;
+  // device outputs of fvs_time_integration (per-step norms, vortex errors,
+  // instance norms), kept across calls and grown on demand, so a call
+  // allocates nothing once its step count has been seen
+  double *dn_buf = nullptr, *dv_buf = nullptr, *di_buf = nullptr;
+  size_t dn_cap = 0, dv_cap = 0, di_cap = 0;
+  // the instantiated graph of the last graph-mode call and what its capture
+  // baked in; launch_gen changes whenever a call changes what a step launches
+  // (time step, limiter freezing, active blocks, autotuned launch choices)
+  struct GraphKey {
+    unsigned gen = 0;
+    int gsteps = 0;
+    const void *dn = nullptr, *di = nullptr, *w = nullptr, *walt = nullptr;
+    bool operator==(const GraphKey &o) const {
+      return gen == o.gen && gsteps == o.gsteps && dn == o.dn && di == o.di && w == o.w && walt == o.walt;
+    }
+  } gkey;
+  hipGraphExec_t gexec = nullptr;
+  unsigned launch_gen = 1;
+  // W does not hold the primitive state of U0 (the conservative-stage path
+  // writes none); refreshed by the first caller that reads W
+  bool w_stale = false;
   int nparts = 0;
   bool have_state = false;
   double last_ms = 0.0;
@@ -973,8 +994,31 @@ struct fvs_ctx {
     if (!h.empty()) HIPC(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream));
     return 0;
   }
+  // device buffer of at least n doubles, reallocated (doubling) only when n
+  // exceeds what it holds
+  int grow(double *&p, size_t &cap, size_t n) {
+    if (n <= cap && p) return 0;
+    const size_t want = std::max(n, 2 * cap);
+    if (p) {
+      HIPC(hipStreamSynchronize(stream));
+      (void)hipFree(p);
+      p = nullptr;
+      cap = 0;
+    }
+    HIPC(hipMalloc(&p, std::max<size_t>(want, 1) * sizeof(double)));
+    cap = want;
+    return 0;
+  }
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    gkey = GraphKey{};
+  }
   ~fvs_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
+    drop_graph();
+    for (double *q : {dn_buf, dv_buf, di_buf})
+      if (q) (void)hipFree(q);
     for (void *q : allocs) (void)hipFree(q);
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -1879,6 +1923,16 @@ int need_state(const fvs_ctx *C) {
   return 0;
 }
 
+// W := primitive state of U0
+void refresh_prim(fvs_ctx &C) {
+  hipLaunchKernelGGL(k_prim, dim3(nblk(C.N)), dim3(kBlock), 0, C.stream, C.N, C.opt.gamma, C.U0, C.W);
+  C.w_stale = false;
+}
+// the primitive state of U0 in W, where a conservative-stage run left it stale
+void ensure_prim(fvs_ctx &C) {
+  if (C.w_stale) refresh_prim(C);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2088,7 +2142,7 @@ int fvs_set_state(fvs_ctx *C, const double *cvar) {
   std::vector<double> h;
   to_device_order(*C, cvar, h, 4);
   HIPC(hipMemcpyAsync(C->U0, h.data(), h.size() * 8, hipMemcpyHostToDevice, C->stream));
-  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  refresh_prim(*C);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(C->stream));
   C->have_state = true;
@@ -2144,7 +2198,7 @@ int fvs_initialize(fvs_ctx *C, int mode, double t0) {
 
 int fvs_primitives(fvs_ctx *C, double *pvar) {
   if (int rc = need_state(C)) return rc;
-  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  refresh_prim(*C);
   HIPC(hipGetLastError());
   if (pvar) {
     std::vector<double> h(4 * size_t(C->N));
@@ -2158,6 +2212,7 @@ int fvs_primitives(fvs_ctx *C, double *pvar) {
 
 int fvs_gradient(fvs_ctx *C, double *grad) {
   if (int rc = need_state(C)) return rc;
+  ensure_prim(*C);
   if (C->opt.recon != 1)
     if (int rc = launch_gradient(*C, false)) return rc;
   if (grad) {
@@ -2200,6 +2255,7 @@ int fvs_gradient_1var(fvs_ctx *C, const double *fc, double *dfc) {
 
 int fvs_limiter(fvs_ctx *C, double *phi) {
   if (int rc = need_state(C)) return rc;
+  ensure_prim(*C);
   if (C->opt.recon != 1 && C->opt.limiter > 0)
     if (int rc = launch_gradient(*C, true)) return rc;
   if (phi) {
@@ -2214,7 +2270,7 @@ int fvs_limiter(fvs_ctx *C, double *phi) {
 
 int fvs_compute_residual(fvs_ctx *C, double time, double *resid, double *ws) {
   if (int rc = need_state(C)) return rc;
-  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  refresh_prim(*C);
   if (int rc = launch_fluxes(*C, time)) return rc;
   Upd A{};
   A.mode = M_RESID;
@@ -2364,9 +2420,45 @@ bool time_dependent(const fvs_ctx &C) {
   return false;
 }
 
+// capture gsteps steps into the context's graph, unless the cached one was
+// captured with the same launches, pointers and grid sizes
+int ensure_graph(fvs_ctx &C, int gsteps, double t1, double *dn, double *dinst) {
+  fvs_ctx::GraphKey key;
+  key.gen = C.launch_gen;
+  key.gsteps = gsteps;
+  key.dn = dn;
+  key.di = dinst;
+  key.w = C.W;
+  key.walt = C.Walt;
+  if (C.gexec && key == C.gkey) return 0;
+  C.drop_graph();
+  hipGraph_t graph = nullptr;
+  HIPC(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  // t1 is baked into the captured launches; only time-dependent boundary
+  // data read it, and those never run from a graph
+  for (int g = 0; g < gsteps && !rc; ++g) rc = launch_step(C, t1, dn, dinst);
+  hipError_t e = hipStreamEndCapture(C.stream, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(FVS_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&C.gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    C.gexec = nullptr;
+    return fail(FVS_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  }
+  C.gkey = key;
+  return 0;
+}
+
 int run_steps(fvs_ctx &C, double t1, int nsteps, double *dn, double *dv, double *dinst) {
   const double dt = C.opt.dt;
   const int nst = (C.opt.rk_scheme == FVS_RK_SSP3) ? 3 : 4;
+  const bool cons = cons_stages(C);
+  if (!cons) ensure_prim(C);  // the stages read W
   hipLaunchKernelGGL(k_counter, dim3(1), dim3(1), 0, C.stream, C.dctr, 0, 0);
   HIPC(hipGetLastError());
   int s = 0;
@@ -2375,28 +2467,9 @@ int run_steps(fvs_ctx &C, double t1, int nsteps, double *dn, double *dv, double 
   if (C.use_graph && !C.profiling && !dv && !time_dependent(C)) {
     const int gsteps = (nst % 2 == 0) ? 1 : 2;
     if (nsteps >= 2 * gsteps) {
-      hipGraph_t graph = nullptr;
-      hipGraphExec_t exec = nullptr;
-      HIPC(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
-      int rc = 0;
-      for (int g = 0; g < gsteps && !rc; ++g) rc = launch_step(C, t1, dn, dinst);
-      hipError_t e = hipStreamEndCapture(C.stream, &graph);
-      if (rc) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return rc;
-      }
-      if (e != hipSuccess) return fail(FVS_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
-      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      if (e != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        return fail(FVS_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-      }
-      for (; s + gsteps <= nsteps; s += gsteps) {
-        e = hipGraphLaunch(exec, C.stream);
-        if (e != hipSuccess) break;
-      }
-      (void)hipGraphExecDestroy(exec);
-      (void)hipGraphDestroy(graph);
+      if (int rc = ensure_graph(C, gsteps, t1, dn, dinst)) return rc;
+      hipError_t e = hipSuccess;
+      for (; s + gsteps <= nsteps && e == hipSuccess; s += gsteps) e = hipGraphLaunch(C.gexec, C.stream);
       if (e != hipSuccess) return fail(FVS_EHIP, std::string("graph launch: ") + hipGetErrorString(e));
     }
   }
@@ -2406,10 +2479,9 @@ int run_steps(fvs_ctx &C, double t1, int nsteps, double *dn, double *dv, double 
     if (dv)
       if (int rc = launch_vortex_err(C, told + dt, dv + 13 * size_t(s))) return rc;
   }
-  if (cons_stages(C)) {  // the stages wrote no primitive state
-    hipLaunchKernelGGL(k_prim, dim3(nblk(C.N)), dim3(kBlock), 0, C.stream, C.N, C.opt.gamma, C.U0, C.W);
-    HIPC(hipGetLastError());
-  }
+  // the conservative stages wrote no primitive state: W is refreshed by the
+  // next caller that reads it, not inside the timed steps
+  if (cons) C.w_stale = true;
   return 0;
 }
 
@@ -2417,33 +2489,27 @@ int integrate(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr, do
   if (int rc = need_state(C)) return rc;
   if (nsteps < 0) return fail(FVS_EINVAL, "nsteps < 0");
   if (nsteps == 0) return 0;
-  double *dn = nullptr, *dv = nullptr, *di = nullptr;
-  HIPC(hipMalloc(&dn, size_t(nsteps) * 4 * sizeof(double)));
-  int rc = 0;
-  if (verr && hipMalloc(&dv, size_t(nsteps) * 13 * sizeof(double)) != hipSuccess) rc = fail(FVS_EHIP, "hipMalloc");
-  if (!rc && inst_norms && C->ninst > 0 &&
-      hipMalloc(&di, size_t(nsteps) * C->ninst * 4 * sizeof(double)) != hipSuccess)
-    rc = fail(FVS_EHIP, "hipMalloc");
-  if (!rc && hipEventRecord(C->ev0, C->stream) != hipSuccess) rc = fail(FVS_EHIP, "hipEventRecord");
-  if (!rc) rc = run_steps(*C, t1, nsteps, dn, dv, di);
-  if (!rc) {
-    hipError_t e = hipEventRecord(C->ev1, C->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(C->stream);
-    float ms = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, C->ev0, C->ev1);
-    if (e != hipSuccess) rc = fail(FVS_EHIP, std::string("time integration: ") + hipGetErrorString(e));
-    C->last_ms = ms;
-  }
-  if (!rc && norms && hipMemcpy(norms, dn, size_t(nsteps) * 4 * 8, hipMemcpyDeviceToHost) != hipSuccess)
-    rc = fail(FVS_EHIP, "copy of residual norms failed");
-  if (!rc && verr && hipMemcpy(verr, dv, size_t(nsteps) * 13 * 8, hipMemcpyDeviceToHost) != hipSuccess)
-    rc = fail(FVS_EHIP, "copy of vortex errors failed");
-  if (!rc && di && hipMemcpy(inst_norms, di, size_t(nsteps) * C->ninst * 4 * 8, hipMemcpyDeviceToHost) != hipSuccess)
-    rc = fail(FVS_EHIP, "copy of instance norms failed");
-  (void)hipFree(dn);
-  if (dv) (void)hipFree(dv);
-  if (di) (void)hipFree(di);
-  return rc;
+  // device outputs: the context's buffers (allocated only when a call needs
+  // more than any earlier one)
+  if (int rc = C->grow(C->dn_buf, C->dn_cap, size_t(nsteps) * 4)) return rc;
+  if (verr)
+    if (int rc = C->grow(C->dv_buf, C->dv_cap, size_t(nsteps) * 13)) return rc;
+  const bool want_inst = inst_norms && C->ninst > 0;
+  if (want_inst)
+    if (int rc = C->grow(C->di_buf, C->di_cap, size_t(nsteps) * C->ninst * 4)) return rc;
+  double *dn = C->dn_buf, *dv = verr ? C->dv_buf : nullptr, *di = want_inst ? C->di_buf : nullptr;
+  HIPC(hipEventRecord(C->ev0, C->stream));
+  if (int rc = run_steps(*C, t1, nsteps, dn, dv, di)) return rc;
+  HIPC(hipEventRecord(C->ev1, C->stream));
+  // the small outputs queue behind the steps; one wait for all of them
+  if (norms) HIPC(hipMemcpyAsync(norms, dn, size_t(nsteps) * 4 * 8, hipMemcpyDeviceToHost, C->stream));
+  if (dv) HIPC(hipMemcpyAsync(verr, dv, size_t(nsteps) * 13 * 8, hipMemcpyDeviceToHost, C->stream));
+  if (di) HIPC(hipMemcpyAsync(inst_norms, di, size_t(nsteps) * C->ninst * 4 * 8, hipMemcpyDeviceToHost, C->stream));
+  HIPC(hipStreamSynchronize(C->stream));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, C->ev0, C->ev1));
+  C->last_ms = ms;
+  return 0;
 }
 
 }  // namespace
@@ -2462,6 +2528,7 @@ int fvs_time_integration(fvs_ctx *C, double t1, int nsteps, double *norms, doubl
 int fvs_set_graph(fvs_ctx *C, int on) {
   if (!C) return fail(FVS_EINVAL, "null context");
   C->use_graph = on != 0;
+  if (!C->use_graph) C->drop_graph();
   return 0;
 }
 
@@ -2504,10 +2571,11 @@ int fvs_batch_set_active(fvs_ctx *C, const int *active) {
   }
   // masked cells are not updated, so on the primitive-state paths their W
   // lags the stage parity; a resumed instance starts from its own state
-  hipLaunchKernelGGL(k_prim, dim3(nb), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  refresh_prim(*C);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(C->stream));
   C->nact = all ? 0 : int(list.size());
+  C->launch_gen++;
   return 0;
 }
 
@@ -2515,7 +2583,7 @@ int fvs_batch_wall_forces(fvs_ctx *C, double *out) {
   if (int rc = need_state(C)) return rc;
   if (C->ninst <= 0) return fail(FVS_ESTATE, "not a batch context (use fvs_create_batch)");
   if (!out) return fail(FVS_EINVAL, "null argument");
-  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  refresh_prim(*C);
   const int save = C->nact;
   C->nact = 0;
   int rc = launch_gradient(*C, false);
@@ -2542,7 +2610,7 @@ int fvs_wall_forces(fvs_ctx *C, double *out4) {
   if (int rc = need_state(C)) return rc;
   if (!out4) return fail(FVS_EINVAL, "null argument");
   const double kPi = 3.14159265358979323846;
-  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  refresh_prim(*C);
   if (int rc = launch_gradient(*C, false)) return rc;
   const double al = C->opt.aoa_deg * (kPi / 180.0);
   hipLaunchKernelGGL(k_wall_forces, dim3(1), dim3(64), 0, C->stream, C->nwall, C->wall, C->fb_cell, C->fb_geo, C->W,
@@ -2559,7 +2627,7 @@ int fvs_wall_cp(fvs_ctx *C, int *nwall, double *out4, double *un3) {
   *nwall = C->nwall;
   if (un3) un3[0] = un3[1] = un3[2] = 0.0;
   if (C->nwall == 0 || (!out4 && !un3)) return 0;
-  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  refresh_prim(*C);
   const int save = C->nact;
   C->nact = 0;
   int rc = launch_gradient(*C, false);
@@ -2600,6 +2668,7 @@ int fvs_set_timestep(fvs_ctx *C, double dt, double cfl) {
   if (!(dt > 0.0) || !(cfl > 0.0)) return fail(FVS_EINVAL, "dt and cfl must be positive");
   C->opt.dt = dt;
   C->opt.cfl = cfl;
+  C->launch_gen++;
   return 0;
 }
 
@@ -2684,6 +2753,7 @@ int fvs_freeze_limiter(fvs_ctx *C, int frozen) {
   if (!C) return fail(FVS_EINVAL, "null context");
   C->limiter_frozen = frozen != 0;
   C->frz_some = false;
+  C->launch_gen++;
   return 0;
 }
 
@@ -2695,6 +2765,7 @@ int fvs_batch_freeze_limiter(fvs_ctx *C, const int *frozen) {
   for (int k = 0; k < C->ninst; ++k) nf += frozen[k] != 0;
   C->limiter_frozen = nf == C->ninst;
   C->frz_some = nf > 0 && nf < C->ninst;
+  C->launch_gen++;
   if (C->frz_some) {
     if (!C->dfrz)
       if (int rc = C->alloc(C->dfrz, size_t(C->ninst))) return rc;
@@ -2719,7 +2790,8 @@ int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) {
   auto trial = [&](double &ms) -> int {
     if (hipMemcpyAsync(C->U0, save, n4 * 8, hipMemcpyDeviceToDevice, C->stream) != hipSuccess)
       return fail(FVS_EHIP, "autotune: state restore failed");
-    hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+    refresh_prim(*C);
+    C->launch_gen++;  // the launch choices may have changed
     if (int r = fvs_time_integration(C, 0.0, nsteps, nullptr, nullptr)) return r;
     ms = C->last_ms / nsteps;
     return 0;
@@ -2740,9 +2812,10 @@ int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) {
   }
   if (hipMemcpyAsync(C->U0, save, n4 * 8, hipMemcpyDeviceToDevice, C->stream) != hipSuccess && !rc)
     rc = fail(FVS_EHIP, "autotune: state restore failed");
-  hipLaunchKernelGGL(k_prim, dim3(nblk(C->N)), dim3(kBlock), 0, C->stream, C->N, C->opt.gamma, C->U0, C->W);
+  refresh_prim(*C);
   if (hipStreamSynchronize(C->stream) != hipSuccess && !rc) rc = fail(FVS_EHIP, "autotune: sync failed");
   (void)hipFree(save);
+  C->launch_gen++;
   if (rc) return rc;
   if (choice4) {
     for (int k = 0; k < 4; ++k) choice4[k] = *knobs[k] ? 1 : 0;

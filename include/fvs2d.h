@@ -104,9 +104,11 @@ int fvs_batch_set_active(fvs_ctx *ctx, const int *active);
 /* C_L, C_D, C_L(cell), C_D(cell) per instance (4*ninst), src/io.f90:340-449 */
 int fvs_batch_wall_forces(fvs_ctx *ctx, double *out);
 /* Launch mode, no reference counterpart: on != 0 captures the launches of one
- * time step (two for three-stage schemes) in a HIP graph once per
- * fvs_time_integration call and replays it; time-dependent boundary data
- * (the Dirichlet vortex) always use stream launches. */
+ * time step (two for three-stage schemes) in a HIP graph and replays it; the
+ * instantiated graph is kept for later fvs_time_integration calls until a call
+ * changes what a step launches (time step, limiter freezing, active
+ * instances, autotuned choices).  Time-dependent boundary data (the
+ * Dirichlet vortex) always use stream launches. */
 int fvs_set_graph(fvs_ctx *ctx, int on);
 /* replaces data_sol_close / data_grid_close (src/data_solution.f90:130-140) */
 int fvs_destroy(fvs_ctx *ctx);
