@@ -221,14 +221,22 @@ def make_dataset(conds, group, cfl, max_steps, drop, chunk, freeze_after, freeze
 
 
 def run(n_train, n_test, res, seed, steps, batch, lr, depth, cfl, max_steps, drop, chunk, freeze_after, freeze_drop,
-        freeze_stall, diverge, fallback, group, n_seq, limiter=1, log=print):
+        freeze_stall, diverge, fallback, group, n_seq, limiter=1, log=print, test_res=None, stall_scale=False):
+    """Train on n_train converged solves at resolution ``res``, then solve
+    n_test held-out conditions at ``test_res`` (default ``res``; the network's
+    inputs do not depend on the grid) cold and warm.  With ``stall_scale``
+    the stall window scales with n_around / 64 (cases.stall_rule_for)."""
+    test_res = tuple(res if test_res is None else test_res)
     kw = dict(cfl=cfl, max_steps=max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after,
               freeze_drop=freeze_drop, freeze_stall=freeze_stall, limiter=limiter)
-    out = dict(resolution=list(res), n_train=n_train, n_test=n_test, seed=seed, cfl=cfl, drop=drop,
-               freeze_after=freeze_after, freeze_drop=freeze_drop, freeze_stall=freeze_stall, fallback=fallback,
-               depth=depth, batch=batch, lr=lr, steps=steps, limiter=limiter)
+    out = dict(resolution=list(res), test_resolution=list(test_res), n_train=n_train, n_test=n_test, seed=seed,
+               cfl=cfl, drop=drop, freeze_after=freeze_after, freeze_drop=freeze_drop, freeze_stall=freeze_stall,
+               fallback=fallback, depth=depth, batch=batch, lr=lr, steps=steps, limiter=limiter)
     train_c = draw(n_train, seed, res)
-    test_c = draw(n_test, seed + 1_000_003, res)
+    test_c = draw(n_test, seed + 1_000_003, test_res)
+    kw_test = dict(kw)
+    if stall_scale and freeze_stall is not None:
+        kw_test["freeze_stall"] = (int(freeze_stall[0] * test_res[0] / 64), freeze_stall[1])
     raw, du, its, gen_s = make_dataset(train_c, group, diverge=diverge, log=log, **kw)
     out["train_converged"] = len(its)
     out["train_rows"] = int(raw.shape[0])
@@ -249,11 +257,11 @@ def run(n_train, n_test, res, seed, steps, batch, lr, depth, cfl, max_steps, dro
     out["loss_first_last"] = [float(loss[:10].mean()), float(loss[-50:].mean())]
     del raw, du
     # held-out conditions: cold, then warm on the same batch solver
-    b, rc, cold_s, setup = cold_solve(test_c, diverge=diverge, **kw)
+    b, rc, cold_s, setup = cold_solve(test_c, diverge=diverge, **kw_test)
     cv_cold = b.solver.cvar
     f_cold = b.solver.wall_forces_batch()
     log(f"test cold: {int((rc['iterations'] > 0).sum())}/{n_test} converged in {cold_s:.2f} s")
-    rw, warm_s, fell = warm_solve(b, net, scale, diverge=diverge, fallback=fallback, **kw)
+    rw, warm_s, fell = warm_solve(b, net, scale, diverge=diverge, fallback=fallback, **kw_test)
     cv_warm = b.solver.cvar
     f_warm = b.solver.wall_forces_batch()
     log(f"test warm: {int((rw['iterations'] > 0).sum())}/{n_test} converged in {warm_s:.2f} s, "
@@ -281,6 +289,10 @@ def run(n_train, n_test, res, seed, steps, batch, lr, depth, cfl, max_steps, dro
         cl_abs_diff_p50=float(np.median(np.abs(f_warm[both, 0] - f_cold[both, 0]))) if np.any(both) else None,
         cd_abs_diff_p50=float(np.median(np.abs(f_warm[both, 1] - f_cold[both, 1]))) if np.any(both) else None,
         state_rel_diff_max=float(np.max(dstate[both])) if np.any(both) else None,
+        # instances converged both ways whose warm C_L and C_D are within 1e-5 of the cold ones
+        forces_within_1e5=(float(np.mean((np.abs(f_warm[both, 0] - f_cold[both, 0]) < 1e-5) &
+                                         (np.abs(f_warm[both, 1] - f_cold[both, 1]) < 1e-5)))
+                           if np.any(both) else None),
         setup_seconds=setup,
     )
     out["per_instance"] = [dict(mach=c["mach"], aoa=c["aoa"], thickness=c["thickness"], cold=int(ic[k]),
@@ -292,7 +304,7 @@ def run(n_train, n_test, res, seed, steps, batch, lr, depth, cfl, max_steps, dro
                            for k, c in enumerate(test_c)]
     b.solver.close()
     if n_seq:
-        sq, ok, its_s = sequential_cold(test_c[:n_seq], **kw)
+        sq, ok, its_s = sequential_cold(test_c[:n_seq], **kw_test)
         out["plain_cold"] = dict(instances=n_seq, seconds=sq, converged=ok, solves_per_hour=3600.0 * ok / sq)
         out["warm_batched_vs_plain_cold"] = out["warm"]["solves_per_hour"] / out["plain_cold"]["solves_per_hour"]
         log(f"plain cold: {ok}/{n_seq} in {sq:.2f} s")
@@ -304,6 +316,8 @@ def main(argv=None):
     ap.add_argument("--train", type=int, default=2000)
     ap.add_argument("--test", type=int, default=256)
     ap.add_argument("--res", type=str, default="64x32")
+    ap.add_argument("--test-res", type=str, default=None, help="resolution of the held-out solves (default --res)")
+    ap.add_argument("--stall-scale", type=int, default=0, help="scale the stall window with n_around / 64")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=65536)
@@ -329,7 +343,9 @@ def main(argv=None):
                                                     float(a.freeze_stall.split(",")[1]))
     out = run(a.train, a.test, res, a.seed, a.steps, a.batch, a.lr, a.depth, a.cfl, a.max_steps, a.drop, a.chunk,
               a.freeze_after, a.freeze_drop, stall, a.diverge, a.fallback, a.group, a.seq, a.limiter,
-              log=lambda s: print(s, flush=True))
+              log=lambda s: print(s, flush=True),
+              test_res=None if a.test_res is None else tuple(int(v) for v in a.test_res.split("x")),
+              stall_scale=bool(a.stall_scale))
     print(json.dumps(out))
     return 0
 
