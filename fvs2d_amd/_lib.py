@@ -24,8 +24,9 @@ EXPORTS = (
     "fvs_ctx_mesh", "fvs_mesh_sizes", "fvs_mesh_topology", "fvs_mesh_lists", "fvs_mesh_lsq_nnz", "fvs_mesh_lsq",
     "fvs_set_state", "fvs_get_state", "fvs_initialize", "fvs_primitives", "fvs_gradient", "fvs_gradient_1var",
     "fvs_limiter", "fvs_compute_residual", "fvs_roe_flux_batch", "fvs_local_dt", "fvs_time_integration",
-    "fvs_vortex_error", "fvs_wall_forces", "fvs_set_timestep", "fvs_kernel_times", "fvs_autotune", "fvs_freeze_limiter",
-    "fvs_batch_freeze_limiter", "fvs_wall_cp", "fvs_last_elapsed_ms",
+    "fvs_vortex_error", "fvs_wall_forces", "fvs_set_timestep", "fvs_kernel_times",
+    "fvs_autotune", "fvs_freeze_limiter", "fvs_batch_freeze_limiter", "fvs_wall_cp", "fvs_last_elapsed_ms",
+    "fvs_set_phase_timers", "fvs_phase_times", "fvs_set_persistent",
 )
 
 
@@ -65,6 +66,9 @@ def load():
     L.fvs_batch_set_active.argtypes = [V, V]
     L.fvs_batch_wall_forces.argtypes = [V, V]
     L.fvs_set_graph.argtypes = [V, I]
+    L.fvs_set_phase_timers.argtypes = [V, I]
+    L.fvs_set_persistent.argtypes = [V, I]
+    L.fvs_phase_times.argtypes = [V, V]
     L.fvs_mesh_create.argtypes = [C.POINTER(V), I, V, I, I, V, V, I, V, V, V, I, I, D, I]
     for n in ("fvs_destroy", "fvs_mesh_destroy"):
         getattr(L, n).argtypes = [V]

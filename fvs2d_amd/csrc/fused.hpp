@@ -654,66 +654,91 @@ __global__ __launch_bounds__(kBlock, LIM ? 1 : 7) void k_grad_lsqh(int N, int nb
 
 constexpr int kStageThreads = 2 * kBlock;  // two threads per cell: each phase fits one round
 
+// Everything k_stage reads besides the stage's own update constants and its
+// input / output primitive states (one struct, so the same block body serves
+// the per-stage launch and the persistent whole-step kernel).
+struct StageArgs {
+  int N, nb, gw, maxf, max1, max2;
+  Phys P;
+  double t;
+  const int *boff, *bfaces;
+  const unsigned *blf;
+  const short *lgat;
+  int Fi;
+  const double *fgeo;
+  const int *btype;
+  const double *bgeo;
+  const int *s2off, *s2n1, *s2list, *sl0, *slw;
+  const long *sleo;
+  const unsigned short *slid;
+  const double2 *cxy;
+  const double *ginv;
+  ShiftTable st;
+  double p;
+  int phisrc;
+  double *phi;
+  int lt;
+  const int *nvrt;
+  const double2 *loff;
+  const double *vol;
+  const int *ioff, *frz;
+  int ninst;
+  double *Ua, *Ub, *Facc, *dtl, *part;
+  const int *blist, *binst;
+  const double *pinf_inst;
+};
+
+// One 256-cell block lb of a whole RK stage (see above).
 template <int LIM, int PMODE>
-__global__ __launch_bounds__(kStageThreads) void k_stage(
-    int N, int nb, int gw, int maxf, int max1, int max2, Upd A, Phys P, double t, const int *__restrict__ boff,
-    const int *__restrict__ bfaces, const unsigned *__restrict__ blf, const short *__restrict__ lgat, int Fi,
-    const double *__restrict__ fgeo, const int *__restrict__ btype, const double *__restrict__ bgeo,
-    const int *__restrict__ s2off, const int *__restrict__ s2n1, const int *__restrict__ s2list,
-    const int *__restrict__ sl0, const int *__restrict__ slw, const long *__restrict__ sleo,
-    const unsigned short *__restrict__ slid, const double2 *__restrict__ cxy, const double *__restrict__ ginv,
-    ShiftTable st, double p, int phisrc, double *__restrict__ phi, int lt, const int *__restrict__ nvrt,
-    const double2 *__restrict__ loff, const double *__restrict__ vol, const int *__restrict__ ioff,
-    const int *__restrict__ frz, int ninst, const double *__restrict__ Win, double *__restrict__ Ua,
-    double *__restrict__ Ub, double *__restrict__ Facc, double *__restrict__ Wout, double *__restrict__ dtl,
-    double *__restrict__ part, const int *__restrict__ blist, const int *__restrict__ binst,
-    const double *__restrict__ pinf_inst, double *__restrict__ d2out) {
+__device__ __forceinline__ void stage_block(const StageArgs &a, int lb, const Upd &A, const double *__restrict__ Win,
+                                            double *__restrict__ Wout, double *__restrict__ d2out) {
   extern __shared__ double lds[];
+  const int N = a.N, max1 = a.max1, max2 = a.max2, maxf = a.maxf;
   double *lW = lds;                                   // [max2][4]
   double2 *lX = reinterpret_cast<double2 *>(lds + 4 * max2);  // [max2]
   double *lG = lds + 6 * max2;                        // [max1][8]
   double *lP = lG + 8 * max1;                         // [max1]
   double *lF = lP + max1;                             // [maxf][4]
   double *lS = lF + 4 * maxf;                         // [maxf]
-  const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
-  const int nbl = nb < 0 ? -nb : nb;
-  if (lb >= nbl) return;  // the whole workgroup leaves before any barrier
-  const int o2 = s2off[lb], n2 = s2off[lb + 1] - o2, n1 = s2n1[lb];
+  const int nbl = a.nb < 0 ? -a.nb : a.nb;
+  const int o2 = a.s2off[lb], n2 = a.s2off[lb + 1] - o2, n1 = a.s2n1[lb];
   const int i0 = lb * kBlock, nown = (N - i0 < kBlock) ? N - i0 : kBlock;
   // stage the state and centres of C2
   for (int q = threadIdx.x; q < n2; q += kStageThreads) {
-    const int id = s2list[o2 + q];
+    const int id = a.s2list[o2 + q];
     double w[4];
     ld4(Win, id, w);
     st4(lW, q, w);
-    lX[q] = cxy[id];
+    lX[q] = a.cxy[id];
   }
   __syncthreads();
   // gradients (and limiter values) of C1 from LDS
   for (int q = threadIdx.x; q < n1; q += kStageThreads) {
-    const int id = s2list[o2 + q];
-    const int si = sl0[lb] + q / kSlice;
+    const int id = a.s2list[o2 + q];
+    const int si = a.sl0[lb] + q / kSlice;
     double g8[8];
-    const double ph = lds_gradient<LIM, PMODE>(q, id, N, lW, lX, slid + sleo[si], slw[si], st, p, ginv, g8, phisrc,
-                                               phi, lt, nvrt, loff, vol, ioff, frz, ninst, q < nown);
+    const double ph = lds_gradient<LIM, PMODE>(q, id, N, lW, lX, a.slid + a.sleo[si], a.slw[si], a.st, a.p, a.ginv,
+                                               g8, a.phisrc, a.phi, a.lt, a.nvrt, a.loff, a.vol, a.ioff, a.frz,
+                                               a.ninst, q < nown);
     st8(lG, q, g8);
     lP[q] = ph;
   }
   __syncthreads();
   // face fluxes from LDS
-  const int f0 = boff[lb], nf = boff[lb + 1] - f0;
+  const int f0 = a.boff[lb], nf = a.boff[lb + 1] - f0;
   for (int f = threadIdx.x; f < nf; f += kStageThreads) {
-    const int slot = bfaces[f0 + f];
-    const unsigned pr = blf[f0 + f];
+    const int slot = a.bfaces[f0 + f];
+    const unsigned pr = a.blf[f0 + f];
     const int la = int(pr & 0xffffu), lc = int(pr >> 16);
     double fa[4], wsa, g[8];
-    if (slot < Fi) {
-      ld8(fgeo, slot, g);
-      interior_flux(P, make_int2(la, lc), g, lW, lG, lP, fa, wsa);
+    if (slot < a.Fi) {
+      ld8(a.fgeo, slot, g);
+      interior_flux(a.P, make_int2(la, lc), g, lW, lG, lP, fa, wsa);
     } else {
-      const int b = slot - Fi;
-      ld8(bgeo, b, g);
-      boundary_flux(P, t, la, btype[b], g, lW, lG, lP, fa, wsa, binst ? pinf_inst + 4 * binst[b] : P.pinf);
+      const int b = slot - a.Fi;
+      ld8(a.bgeo, b, g);
+      boundary_flux(a.P, a.t, la, a.btype[b], g, lW, lG, lP, fa, wsa,
+                    a.binst ? a.pinf_inst + 4 * a.binst[b] : a.P.pinf);
     }
     for (int v = 0; v < 4; ++v) lF[4 * f + v] = fa[v];
     lS[f] = wsa;
@@ -723,10 +748,20 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(
   {
     UpdIn in;
     const int ic = lb * kBlock + threadIdx.x;
-    if (threadIdx.x < kBlock && ic < N) block_update_load(N, ic, gw, A, lgat, vol, Ua, Ub, Facc, dtl, in);
-    block_update_finish<kStageThreads>(N, lb, A, in, lF, lS, Ua, Ub, Facc, Wout, dtl, part, d2out, lds,
-                                       blist ? active_mask(blist, nbl) : nullptr);
+    if (threadIdx.x < kBlock && ic < N)
+      block_update_load(N, ic, a.gw, A, a.lgat, a.vol, a.Ua, a.Ub, a.Facc, a.dtl, in);
+    block_update_finish<kStageThreads>(N, lb, A, in, lF, lS, a.Ua, a.Ub, a.Facc, Wout, a.dtl, a.part, d2out, lds,
+                                       a.blist ? active_mask(a.blist, nbl) : nullptr);
   }
+}
+
+template <int LIM, int PMODE>
+__global__ __launch_bounds__(kStageThreads) void k_stage(StageArgs a, Upd A, const double *__restrict__ Win,
+                                                         double *__restrict__ Wout, double *__restrict__ d2out) {
+  const int lb = a.blist ? a.blist[blockIdx.x] : xcd_block(a.nb);
+  const int nbl = a.nb < 0 ? -a.nb : a.nb;
+  if (lb >= nbl) return;  // the whole workgroup leaves before any barrier
+  stage_block<LIM, PMODE>(a, lb, A, Win, Wout, d2out);
 }
 
 // per-instance residual norms of one step from the per-cell squared
@@ -785,4 +820,170 @@ __global__ void k_wall_forces_inst(int ninst, const int *__restrict__ woff, cons
   out[4 * k + 1] = cn * sn + ca * c;
   out[4 * k + 2] = cn1 * c - ca1 * sn;
   out[4 * k + 3] = cn1 * sn + ca1 * c;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent whole-step kernel: one cooperative launch runs nsteps time steps
+// of the whole-stage block body over the launch-block list (the ragged batch's
+// active blocks, or every block), with a grid-wide barrier after each stage
+// and the step's residual norms reduced in the same launch.
+// ---------------------------------------------------------------------------
+
+// Grid-wide barrier of a co-resident launch: thread 0 of each workgroup
+// counts itself in bar[0]; the last one resets the count and advances the
+// generation bar[1]; the others poll the generation.  Each poll is bounded
+// (spin polls, or as soon as another workgroup has given up and set *err):
+// a workgroup that gives up records it in *err and returns false, so every
+// wave of the launch ends.
+__device__ __forceinline__ bool grid_barrier(unsigned *bar, unsigned nwg, unsigned &gen, unsigned spin, int *err) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ok = 1;
+    __threadfence();
+    const unsigned arrived = atomicAdd(&bar[0], 1u);
+    if (arrived == nwg - 1) {
+      atomicExch(&bar[0], 0u);
+      __threadfence();
+      atomicAdd(&bar[1], 1u);
+    } else {
+      unsigned n = 0;
+      while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++n >= spin || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          atomicExch(err, 1);
+          ok = 0;
+          break;
+        }
+      }
+    }
+    gen = gen + 1;
+    __threadfence();
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// The whole-mesh residual norms of one step (k_norm_final's summation:
+// kNormThreads partial sums over strided partials, then its tree), computed
+// by NT threads that each play kNormThreads / NT of its threads, so the bits
+// are those of k_norm_final.  red: 4 kNormThreads doubles of LDS.
+template <int NT>
+__device__ __forceinline__ void norm_sum_block(int nparts, int N, const double *__restrict__ part,
+                                               double *__restrict__ out, double *red) {
+  constexpr int R = kNormThreads / NT;
+  for (int r = 0; r < R; ++r) {
+    const int vt = threadIdx.x + r * NT;
+    double s[4] = {0, 0, 0, 0};
+    for (int p0 = vt; p0 < nparts; p0 += kNormLoads * kNormThreads) {
+      double q[kNormLoads][4];
+#pragma unroll
+      for (int u = 0; u < kNormLoads; ++u) {
+        const int p = p0 + u * kNormThreads;
+        if (p < nparts) ld4(part, p, q[u]);
+        else q[u][0] = q[u][1] = q[u][2] = q[u][3] = 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kNormLoads; ++u)
+        for (int v = 0; v < 4; ++v) s[v] = s[v] + q[u][v];
+    }
+    for (int v = 0; v < 4; ++v) red[v * kNormThreads + vt] = s[v];
+  }
+  __syncthreads();
+  for (int h = kNormThreads / 2; h > 0; h >>= 1) {
+    for (int r = 0; r < R; ++r) {
+      const int vt = threadIdx.x + r * NT;
+      if (vt < h)
+        for (int v = 0; v < 4; ++v)
+          red[v * kNormThreads + vt] = red[v * kNormThreads + vt] + red[v * kNormThreads + vt + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int v = 0; v < 4; ++v) out[v] = sqrt(red[v * kNormThreads] / double(N));
+  __syncthreads();
+}
+
+// Instance k's residual norms of one step with k_inst_norms' summation (the
+// first kBlock threads of the workgroup play its threads).  red: 4 kBlock
+// doubles of LDS.
+__device__ __forceinline__ void inst_norm_block(int k, const int *__restrict__ off, const double *__restrict__ d2,
+                                                double *__restrict__ out, double *red) {
+  const int t = threadIdx.x;
+  const int a = off[k], b = off[k + 1];
+  if (t < kBlock) {
+    double s[4] = {0, 0, 0, 0};
+    for (int i = a + t; i < b; i += kBlock) {
+      double q[4];
+      ld4(d2, i, q);
+      for (int v = 0; v < 4; ++v) s[v] = s[v] + q[v];
+    }
+    for (int v = 0; v < 4; ++v) red[v * kBlock + t] = s[v];
+  }
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if (t < h)
+      for (int v = 0; v < 4; ++v) red[v * kBlock + t] = red[v * kBlock + t] + red[v * kBlock + t + h];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int v = 0; v < 4; ++v) out[v] = sqrt(red[v * kBlock] / double(b - a));
+  __syncthreads();
+}
+
+struct PersistCtl {
+  Upd A[4];                 // the stages of one step
+  int nst, nsteps, nlaunch;  // stages per step, steps, launch blocks
+  double *W0, *W1;          // primitive states: stage k of the launch reads W0 when k is even
+  double *d2buf;            // per-cell squared increments of a step's last stage
+  double *dn;               // whole-mesh norms, one row per step (NULL: none)
+  int nparts;
+  double *dinst;            // per-instance norms (NULL: none)
+  int ninst;
+  const int *inst_off;
+  int *ctr;                 // output row of the first step; advanced by nsteps at the end
+  unsigned *bar;
+  int *err;
+  unsigned spin;
+};
+
+template <int LIM, int PMODE>
+__global__ __launch_bounds__(kStageThreads) void k_steps_persistent(StageArgs a, PersistCtl c) {
+  extern __shared__ double lds[];
+  unsigned gen = 0;
+  if (threadIdx.x == 0) gen = __hip_atomic_load(&c.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int row0 = c.ctr[0];  // block 0 writes the counter only after the last barrier
+  int par = 0;
+  bool ok = true;
+  for (int s = 0; s < c.nsteps && ok; ++s) {
+    for (int k = 0; k < c.nst && ok; ++k) {
+      const Upd A = c.A[k];
+      const double *Win = par ? c.W1 : c.W0;
+      double *Wout = par ? c.W0 : c.W1;
+      for (int q = blockIdx.x; q < c.nlaunch; q += gridDim.x) {
+        const int lb = a.blist ? a.blist[q] : q;
+        stage_block<LIM, PMODE>(a, lb, A, Win, Wout, A.last ? c.d2buf : nullptr);
+        __syncthreads();  // the next block's staging overwrites this one's LDS
+      }
+      ok = grid_barrier(c.bar, gridDim.x, gen, c.spin, c.err);
+      par ^= 1;
+    }
+    if (!ok) break;
+    // this step's norms; the partials and increments they read are next
+    // written by the last stage of the next step, two or more barriers later
+    const long row = long(row0) + s;
+    if (c.dn && blockIdx.x == 0) norm_sum_block<kStageThreads>(c.nparts, a.N, a.part, c.dn + 4 * row, lds);
+    if (c.dinst)
+      for (int k = blockIdx.x; k < c.ninst; k += gridDim.x)
+        inst_norm_block(k, c.inst_off, c.d2buf, c.dinst + 4 * (row * c.ninst + k), lds);
+  }
+  if (ok) ok = grid_barrier(c.bar, gridDim.x, gen, c.spin, c.err);
+  if (ok && blockIdx.x == 0 && threadIdx.x == 0) c.ctr[0] = row0 + c.nsteps;
+}
+
+__global__ void k_barrier_reset(unsigned *bar, int *err) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    bar[0] = 0u;
+    err[0] = 0;
+  }
 }

@@ -47,6 +47,7 @@ int fail(int code, const std::string &msg) {
   } while (0)
 
 constexpr int kBlock = 256;
+constexpr int kVerr = 15;  // vortex-error record: 13 norms + location of the largest density error
 constexpr int kSlice = 64;  // one wavefront
 
 __host__ __device__ inline int nblk(long n, int b = kBlock) { return int((n + b - 1) / b); }
@@ -256,20 +257,23 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggcb(int nbx, int N, int nfw, c
 // stencils (from 256) are staged in LDS, and each cell walks its stencil as
 // 16-bit local codes (two per 32-bit word, the LSQ staged-halo layout) in LDS
 // instead of gathering the states from HBM.  min and max are exact, so the
-// bounds are those of the stencil walk in HBM (stencil_minmax).
-__device__ __forceinline__ void gg_stage_halo(int lb, int N, const double *__restrict__ W,
+// bounds are those of the stencil walk in HBM (stencil_minmax).  With cons, W
+// holds conservative states (SSP-RK3 stages), converted as they are staged.
+__device__ __forceinline__ void gg_stage_halo(int lb, int N, const double *__restrict__ W, int cons, double gamma,
                                               const int *__restrict__ ghoff, const int *__restrict__ ghlist,
                                               double *__restrict__ lW) {
   const int i0 = lb * kBlock, id = i0 + threadIdx.x;
   if (id < N) {
     double w[4];
     ld4(W, id, w);
+    make_prim(cons, gamma, w);
     st4(lW, threadIdx.x, w);
   }
   const int o2 = ghoff[lb], nh = ghoff[lb + 1] - o2;
   for (int q = threadIdx.x; q < nh; q += kBlock) {
     double w[4];
     ld4(W, ghlist[o2 + q], w);
+    make_prim(cons, gamma, w);
     st4(lW, kBlock + q, w);
   }
 }
@@ -289,8 +293,11 @@ __device__ __forceinline__ void gg_halo_minmax(const unsigned *__restrict__ lc, 
 }
 
 // GGCB gradient with the limiter, bounds from the staged halo (one block of
-// 256 cells per workgroup; LDS 32 B per local cell)
+// 256 cells per workgroup; LDS 32 B per local cell).  With cons, W holds
+// conservative states: the staged ones and the face neighbours are converted
+// after their loads are issued.
 __global__ __launch_bounds__(kBlock) void k_grad_ggcb_h(int nbx, int N, int nfw, const double *__restrict__ W,
+                                                        int cons, double gamma,
                                                         const double2 *__restrict__ c0, const int *__restrict__ nb,
                                                         const double2 *__restrict__ cn, const double *__restrict__ vol,
                                                         double *__restrict__ grad, double *__restrict__ phi, int lt,
@@ -321,7 +328,13 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggcb_h(int nbx, int N, int nfw,
 #pragma unroll
     for (int k = 0; k < 4; ++k) ld4(W, j[k], q[k]);
   }
-  gg_stage_halo(lb, N, W, ghoff, ghlist, lds);
+  gg_stage_halo(lb, N, W, cons, gamma, ghoff, ghlist, lds);
+  if (act) {
+    make_prim(cons, gamma, w);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < nfw) make_prim(cons, gamma, q[k]);
+  }
   __syncthreads();
   if (!act) return;
   double gx[4], gy[4];
@@ -429,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggnb(int nbx, int N, int gw, co
 
 // GGNB pass 2 with the limiter, bounds from the staged halo
 __global__ __launch_bounds__(kBlock) void k_grad_ggnb_h(int nbx, int N, int gw, const double *__restrict__ W,
-                                                        const int *__restrict__ cnode, const double2 *__restrict__ cedg,
+                                                        int cons, double gamma, const int *__restrict__ cnode, const double2 *__restrict__ cedg,
                                                         const double *__restrict__ F, const double *__restrict__ vol,
                                                         double *__restrict__ grad, double *__restrict__ phi, int lt,
                                                         const int *__restrict__ nvrt, const double2 *__restrict__ loff,
@@ -457,7 +470,8 @@ __global__ __launch_bounds__(kBlock) void k_grad_ggnb_h(int nbx, int N, int gw, 
     ld4(W, i, w);
     V = vol[i];
   }
-  gg_stage_halo(lb, N, W, ghoff, ghlist, lds);
+  gg_stage_halo(lb, N, W, cons, gamma, ghoff, ghlist, lds);
+  if (act) make_prim(cons, gamma, w);
   __syncthreads();
   if (!act) return;
   double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
@@ -707,10 +721,15 @@ __global__ __launch_bounds__(kNormThreads) void k_norm_final(int nparts, int N, 
 }
 
 // isentropic-vortex error over interior cells (src/mms.f90:315-361)
+// error_isentropic_vortex (src/mms.f90:271-365) over the interior cells,
+// listed in the reference's ascending order: per block the max, sum and sum
+// of squares of |U - U_exact| per variable, and the list position of the
+// largest density error (the first one on ties, like maxloc)
 __global__ __launch_bounds__(kBlock) void k_vortex_err(int n, Phys P, double t, const int *__restrict__ cells,
                                                        const double *__restrict__ cx, const double *__restrict__ cy,
                                                        const double *__restrict__ U, double *__restrict__ part) {
   __shared__ double red[4 * kBlock];
+  __shared__ int arg[kBlock];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   double d[4] = {0, 0, 0, 0};
   if (k < n) {
@@ -724,18 +743,24 @@ __global__ __launch_bounds__(kBlock) void k_vortex_err(int n, Phys P, double t, 
     ld4(U, i, u);
     for (int v = 0; v < 4; ++v) d[v] = fabs(u[v] - ex[v]);
   }
-  // max
+  // max, and for the density the first position holding it
   double m[4] = {d[0], d[1], d[2], d[3]};
+  int am = k < n ? k : n;
   {
     const int t2 = threadIdx.x;
     for (int q = 0; q < 4; ++q) red[q * kBlock + t2] = m[q];
+    arg[t2] = am;
     __syncthreads();
     for (int s = kBlock / 2; s > 0; s >>= 1) {
-      if (t2 < s)
+      if (t2 < s) {
+        const double a = red[t2], b = red[t2 + s];
+        if (b > a || (b == a && arg[t2 + s] < arg[t2])) arg[t2] = arg[t2 + s];
         for (int q = 0; q < 4; ++q) red[q * kBlock + t2] = fmax(red[q * kBlock + t2], red[q * kBlock + t2 + s]);
+      }
       __syncthreads();
     }
     for (int q = 0; q < 4; ++q) m[q] = red[q * kBlock];
+    am = arg[0];
     __syncthreads();
   }
   double l1[4] = {d[0], d[1], d[2], d[3]};
@@ -743,23 +768,34 @@ __global__ __launch_bounds__(kBlock) void k_vortex_err(int n, Phys P, double t, 
   __syncthreads();
   double l2[4] = {d[0] * d[0], d[1] * d[1], d[2] * d[2], d[3] * d[3]};
   block_sum4(l2, red);
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
     for (int q = 0; q < 4; ++q) {
-      part[12 * blockIdx.x + q] = m[q];
-      part[12 * blockIdx.x + 4 + q] = l1[q];
-      part[12 * blockIdx.x + 8 + q] = l2[q];
+      part[13 * blockIdx.x + q] = m[q];
+      part[13 * blockIdx.x + 4 + q] = l1[q];
+      part[13 * blockIdx.x + 8 + q] = l2[q];
     }
+    part[13 * blockIdx.x + 12] = double(am);
+  }
 }
 
-__global__ void k_vortex_final(int nparts, int n, const double *__restrict__ part, double *__restrict__ out) {
+// the 13 norms of src/mms.f90:351-361 and the centre of the cell with the
+// largest density error (src/mms.f90:363: maxloc over all cells of an array
+// that is zero outside the interior cells, so cell `first` when every
+// interior error is zero)
+__global__ void k_vortex_final(int nparts, int n, const double *__restrict__ part, const int *__restrict__ cells,
+                               const double *__restrict__ cx, const double *__restrict__ cy, int first,
+                               double *__restrict__ out) {
   if (threadIdx.x != 0) return;
   double m[4] = {0, 0, 0, 0}, l1[4] = {0, 0, 0, 0}, l2[4] = {0, 0, 0, 0};
-  for (int p = 0; p < nparts; ++p)
+  int am = 0;
+  for (int p = 0; p < nparts; ++p) {
+    if (part[13 * p] > m[0]) am = int(part[13 * p + 12]);
     for (int q = 0; q < 4; ++q) {
-      m[q] = fmax(m[q], part[12 * p + q]);
-      l1[q] = l1[q] + part[12 * p + 4 + q];
-      l2[q] = l2[q] + part[12 * p + 8 + q];
+      m[q] = fmax(m[q], part[13 * p + q]);
+      l1[q] = l1[q] + part[13 * p + 4 + q];
+      l2[q] = l2[q] + part[13 * p + 8 + q];
     }
+  }
   const double dn = double(n);
   double tot = 0.0;
   for (int q = 0; q < 4; ++q) {
@@ -769,6 +805,9 @@ __global__ void k_vortex_final(int nparts, int n, const double *__restrict__ par
     tot = tot + l2[q];
   }
   out[12] = sqrt(tot / dn);
+  const int c = m[0] > 0.0 ? cells[am] : first;
+  out[13] = cx[c];
+  out[14] = cy[c];
 }
 
 // C_L / C_D over wall faces (src/io.f90:401-442), one thread in face order
@@ -967,15 +1006,28 @@ struct fvs_ctx {
   } gkey;
   hipGraphExec_t gexec = nullptr;
   unsigned launch_gen = 1;
+  // persistent whole-step launches (fvs_set_persistent): grid barrier state
+  // (count, generation) and its failure flag; compute units of the device
+  bool use_persistent = false;
+  unsigned *pbar = nullptr;
+  int *perr = nullptr;
+  int ncu = 0;
+  bool persist_ran = false;
   // W does not hold the primitive state of U0 (the conservative-stage path
   // writes none); refreshed by the first caller that reads W
   bool w_stale = false;
   int nparts = 0;
   bool have_state = false;
   double last_ms = 0.0;
+  // the device the context was created on
+  int device = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // per-launch event pairs while profiling (kernel class, start, stop)
   bool profiling = false;
+  // phase timers (fvs_set_phase_timers): device ms per kernel class, summed
+  // over the time-integration calls made while they are on
+  bool phase_timing = false;
+  double phase_ms[6] = {0, 0, 0, 0, 0, 0};
   std::vector<hipEvent_t> evpool;
   std::vector<int> evclass;
   size_t evused = 0;
@@ -1644,9 +1696,11 @@ int setup_device(fvs_ctx &C) {
   if (int rc = C.alloc(C.rout, n4)) return rc;
   if (int rc = C.alloc(C.wsout, N)) return rc;
   C.nparts = std::max(nblk(N), nblk(std::max(C.ncell_intr, 1)));
-  if (int rc = C.alloc(C.part, 12 * size_t(C.nparts))) return rc;
+  if (int rc = C.alloc(C.part, 13 * size_t(C.nparts))) return rc;
   if (int rc = C.alloc(C.small, 64)) return rc;
   if (int rc = C.alloc(C.dctr, 1)) return rc;
+  if (int rc = C.alloc(C.pbar, 2)) return rc;
+  if (int rc = C.alloc(C.perr, 1)) return rc;
   if (C.ninst > 0) {
     // per-instance ranges in device order, free streams, wall-face ranges
     std::vector<int> off(C.ninst + 1, 0), binst(Fb), woff(C.ninst + 1, 0);
@@ -1722,7 +1776,7 @@ int launch_gradient(fvs_ctx &C, bool with_limiter, const double *src = nullptr) 
   const int lt = C.opt.limiter;
   dim3 g(xcd_grid(nblk(N))), b(kBlock);
   const bool gg = C.opt.grad_method == 1 || C.opt.grad_method == 2;
-  if (src && !(C.opt.grad_method == 3 && C.halo && C.use_halo) && !(gg && !lim))
+  if (src && !(C.halo && C.use_halo) && !(gg && !lim))
     return fail(FVS_ESTATE, "conservative gradient input needs the staged halo or an unlimited Green-Gauss gradient");
   const double *Wg = src ? src : C.W;
   const int cons = src ? 1 : 0;
@@ -1780,7 +1834,8 @@ int launch_gradient(fvs_ctx &C, bool with_limiter, const double *src = nullptr) 
                          C.nvrt, C.loff, C.vol);
   } else if (C.opt.grad_method == 1) {
     if (ghalo)
-      hipLaunchKernelGGL(k_grad_ggcb_h, g, b, glds, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.NFW, C.W, C.ggcb0,
+      hipLaunchKernelGGL(k_grad_ggcb_h, g, b, glds, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.NFW, Wg, cons,
+                         C.opt.gamma, C.ggcb0,
                          C.ggcbnb, C.ggcbc, C.vol, C.grad, C.phi, lt, C.nvrt, C.loff, C.ghoff, C.ghlist, C.poff,
                          C.swid, C.lcode);
     else if (lim)
@@ -1790,11 +1845,11 @@ int launch_gradient(fvs_ctx &C, bool with_limiter, const double *src = nullptr) 
       hipLaunchKernelGGL(k_grad_ggcb<0>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.NFW, Wg, cons, C.opt.gamma, C.ggcb0, C.ggcbnb, C.ggcbc, C.vol, C.grad,
                          C.phi, lt, C.nvrt, C.loff, C.soff, C.swid, C.snb);
   } else {
-    hipLaunchKernelGGL(k_ggnb_nodes, dim3(nblk(C.NV)), b, 0, C.stream, C.NV, Wg, lim ? 0 : cons, C.opt.gamma,
+    hipLaunchKernelGGL(k_ggnb_nodes, dim3(nblk(C.NV)), b, 0, C.stream, C.NV, Wg, cons, C.opt.gamma,
                        C.nsoff, C.nswid, C.nscell, C.nscoef, C.gF);
     if (ghalo)
-      hipLaunchKernelGGL(k_grad_ggnb_h, g, b, glds, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.gwn, C.W,
-                         C.gcnode, C.gcedg, C.gF, C.vol, C.grad, C.phi, lt, C.nvrt, C.loff, C.ghoff, C.ghlist,
+      hipLaunchKernelGGL(k_grad_ggnb_h, g, b, glds, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.gwn, Wg, cons,
+                         C.opt.gamma, C.gcnode, C.gcedg, C.gF, C.vol, C.grad, C.phi, lt, C.nvrt, C.loff, C.ghoff, C.ghlist,
                          C.poff, C.swid, C.lcode);
     else if (lim)
       hipLaunchKernelGGL(k_grad_ggnb<1>, g, b, 0, C.stream, xcd_arg(nblk(N), C.xcd_grad), N, C.gwn, C.W, C.gcnode,
@@ -1861,21 +1916,70 @@ int launch_flux_update(fvs_ctx &C, const Upd &A, double t, const double *uin = n
   return 0;
 }
 
+// arguments of the whole-stage block body for the current context state
+StageArgs stage_args(const fvs_ctx &C, double t) {
+  StageArgs a;
+  a.N = C.N;
+  a.nb = xcd_arg(nblk(C.N), C.xcd_fused);
+  a.gw = C.GW;
+  a.maxf = C.maxf;
+  a.max1 = C.max1;
+  a.max2 = C.max2;
+  a.P = C.P;
+  a.t = t;
+  a.boff = C.boff;
+  a.bfaces = C.bfaces;
+  a.blf = C.blf;
+  a.lgat = C.lgat;
+  a.Fi = C.Fi;
+  a.fgeo = C.fi_geo;
+  a.btype = C.fb_type;
+  a.bgeo = C.fb_geo;
+  a.s2off = C.s2off;
+  a.s2n1 = C.s2n1;
+  a.s2list = C.s2list;
+  a.sl0 = C.sl0;
+  a.slw = C.slw;
+  a.sleo = C.sleo;
+  a.slid = C.slid;
+  a.cxy = C.cxy;
+  a.ginv = C.ginv;
+  a.st = C.shifts;
+  a.p = C.opt.lsq_p;
+  a.phisrc = C.opt.limiter > 0 ? 1 : 0;
+  a.phi = C.phi;
+  a.lt = C.opt.limiter;
+  a.nvrt = C.nvrt;
+  a.loff = C.loff;
+  a.vol = C.vol;
+  a.ioff = C.frz_some ? C.dinst_off : nullptr;
+  a.frz = C.dfrz;
+  a.ninst = C.ninst;
+  a.Ua = C.U0;
+  a.Ub = C.Ub;
+  a.Facc = C.Facc;
+  a.dtl = C.dtl;
+  a.part = C.part;
+  a.blist = C.nact ? C.blist : nullptr;
+  a.binst = C.dbinst;
+  a.pinf_inst = C.dpinf_inst;
+  return a;
+}
+
+size_t stage_lds(const fvs_ctx &C) {
+  return (size_t(6) * C.max2 + size_t(9) * C.max1 + size_t(5) * C.maxf) * sizeof(double);
+}
+
 // whole stage in one kernel (k_stage); swaps the primitive buffers
 int launch_stage(fvs_ctx &C, const Upd &A, double t) {
   ProfScope ps(C, KC_FUSED);
   const int nb = nblk(C.N);
-  const size_t lds = (size_t(6) * C.max2 + size_t(9) * C.max1 + size_t(5) * C.maxf) * sizeof(double);
+  const size_t lds = stage_lds(C);
   const bool lim = C.opt.limiter > 0 && !C.limiter_frozen;
-  const int phisrc = C.opt.limiter > 0 ? 1 : 0;
   const dim3 g(C.nact ? C.nact : xcd_grid(nb)), b(kStageThreads);
-#define FVS_STAGE(L, PM)                                                                                           \
-  hipLaunchKernelGGL((k_stage<L, PM>), g, b, lds, C.stream, C.N, xcd_arg(nb, C.xcd_fused), C.GW, C.maxf, C.max1,  \
-                     C.max2, A, C.P, t, C.boff, C.bfaces, C.blf, C.lgat, C.Fi, C.fi_geo, C.fb_type, C.fb_geo,      \
-                     C.s2off, C.s2n1, C.s2list, C.sl0, C.slw, C.sleo, C.slid, C.cxy, C.ginv, C.shifts,           \
-                     C.opt.lsq_p, phisrc, C.phi, C.opt.limiter, C.nvrt, C.loff, C.vol,                           \
-                     C.frz_some ? C.dinst_off : nullptr, C.dfrz, C.ninst, C.W, C.U0, C.Ub, C.Facc, C.Walt,       \
-                     C.dtl, C.part, C.nact ? C.blist : nullptr, C.dbinst, C.dpinf_inst, A.last ? C.d2buf : nullptr)
+  const StageArgs a = stage_args(C, t);
+  double *d2 = A.last ? C.d2buf : nullptr;
+#define FVS_STAGE(L, PM) hipLaunchKernelGGL((k_stage<L, PM>), g, b, lds, C.stream, a, A, C.W, C.Walt, d2)
   if (lim) {
     if (C.pmode == 0) FVS_STAGE(1, 0);
     else if (C.pmode == 1) FVS_STAGE(1, 1);
@@ -1899,7 +2003,8 @@ int launch_vortex_err(fvs_ctx &C, double t, double *dst) {
   const int nb = nblk(n);
   hipLaunchKernelGGL(k_vortex_err, dim3(nb), dim3(kBlock), 0, C.stream, n, C.P, t, C.cell_intr, C.cx, C.cy, C.U0,
                      C.part);
-  hipLaunchKernelGGL(k_vortex_final, dim3(1), dim3(64), 0, C.stream, nb, n, C.part, dst);
+  hipLaunchKernelGGL(k_vortex_final, dim3(1), dim3(64), 0, C.stream, nb, n, C.part, C.cell_intr, C.cx, C.cy,
+                     C.iperm[0], dst);
   HIPC(hipGetLastError());
   return 0;
 }
@@ -2014,6 +2119,15 @@ static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, in
   }
   (void)hipEventCreate(&C->ev0);
   (void)hipEventCreate(&C->ev1);
+  int rc0 = hipGetDevice(&C->device) == hipSuccess ? 0 : fail(FVS_EHIP, "hipGetDevice failed");
+  if (!rc0 && hipDeviceGetAttribute(&C->ncu, hipDeviceAttributeMultiprocessorCount, C->device) != hipSuccess)
+    rc0 = fail(FVS_EHIP, "hipDeviceGetAttribute failed");
+  if (rc0) {
+    std::string e = g_err;
+    delete C;
+    g_err = e;
+    return rc0;
+  }
   if (int rc = setup_device(*C)) {
     std::string e = g_err;
     delete C;
@@ -2339,13 +2453,18 @@ namespace {
 // gradient (kernel_path 3 keeps primitive states) or a first-order
 // reconstruction
 bool cons_stages(const fvs_ctx &C) {
-  const bool gg = (C.opt.grad_method == 1 || C.opt.grad_method == 2) && C.opt.limiter == 0 && C.opt.kernel_path != 3;
+  // Green-Gauss gradients: unlimited, or limited with the bounds from the
+  // staged halo (where the stage states are converted once, as staged)
+  const bool gg = (C.opt.grad_method == 1 || C.opt.grad_method == 2) && C.opt.kernel_path != 3 &&
+                  (C.opt.limiter == 0 || (C.halo && C.use_halo));
   return C.fused && !C.stage && C.opt.rk_scheme == FVS_RK_SSP3 &&
          (C.opt.recon == 1 || (C.opt.grad_method == 3 && C.halo && C.use_halo) || gg);
 }
 
-int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {
-  const fvs_options &o = C.opt;
+// Update constants of stage k of a step starting at told, and the stage's
+// time ts (runge_kutta_init, src/runge_kutta.f90:36-65; SSP-RK3 stages at
+// t, t + dt, t + dt/2)
+Upd stage_upd(const fvs_options &o, int k, double told, double &ts) {
   const double dt = o.dt;
   double ak = 1, bk = 2, ck = 2, dk = 1;  // runge_kutta_init, src/runge_kutta.f90:36-51
   if (o.rk_order == 1) ak = 3.60897, bk = 2.04, ck = 0.34206, dk = 0.00897;
@@ -2356,34 +2475,42 @@ int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {
   const double dts[4] = {0.0, dt / 3.0, dt * 2.0 / 3.0, dt};  // src/runge_kutta.f90:56-65
   const double ssp3_a[3] = {0.0, 0.75, 1.0 / 3.0}, ssp3_b[3] = {1.0, 0.25, 2.0 / 3.0};
   const int nst = (o.rk_scheme == FVS_RK_SSP3) ? 3 : 4;
+  Upd A{};
+  A.steady = o.steady;
+  A.first = (k == 0);
+  A.last = (k == nst - 1);
+  A.cfl = o.cfl;
+  A.gamma = o.gamma;
+  ts = told;
+  if (o.rk_scheme == FVS_RK_SSP3) {
+    A.mode = M_SSP3;
+    A.a = ssp3_a[k];
+    A.b = ssp3_b[k];
+    A.h = o.steady ? 1.0 : dt;
+    ts = (k == 0) ? told : (k == 1 ? told + dt : told + 0.5 * dt);
+  } else if (o.rk_scheme == FVS_RK_SSP42) {
+    A.mode = M_SSP42;
+    A.coef = 1.0;
+    A.h = o.steady ? (k == 3 ? 1.0 / 4.0 : 1.0 / 3.0) : (k == 3 ? dt / 4.0 : dt / 3.0);
+    ts = told + dts[k];
+  } else {
+    A.mode = M_CLASSIC;
+    A.coef = rk_coef[k];
+    A.use_acc = (k == 3);
+    A.h = o.steady ? (k == 2 ? 1.0 : (k == 3 ? 1.0 / 6.0 : 1.0 / 2.0)) : h_rk[k];
+    const double t12 = told + 0.5 * dt, tnew = told + dt;
+    const double tst[4] = {told, t12, t12, tnew};
+    ts = tst[k];
+  }
+  return A;
+}
+
+int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {
+  const fvs_options &o = C.opt;
+  const int nst = (o.rk_scheme == FVS_RK_SSP3) ? 3 : 4;
   for (int k = 0; k < nst; ++k) {
-    Upd A{};
-    A.steady = o.steady;
-    A.first = (k == 0);
-    A.last = (k == nst - 1);
-    A.cfl = o.cfl;
-    A.gamma = o.gamma;
     double ts = told;
-    if (o.rk_scheme == FVS_RK_SSP3) {
-      A.mode = M_SSP3;
-      A.a = ssp3_a[k];
-      A.b = ssp3_b[k];
-      A.h = o.steady ? 1.0 : dt;
-      ts = (k == 0) ? told : (k == 1 ? told + dt : told + 0.5 * dt);
-    } else if (o.rk_scheme == FVS_RK_SSP42) {
-      A.mode = M_SSP42;
-      A.coef = 1.0;
-      A.h = o.steady ? (k == 3 ? 1.0 / 4.0 : 1.0 / 3.0) : (k == 3 ? dt / 4.0 : dt / 3.0);
-      ts = told + dts[k];
-    } else {
-      A.mode = M_CLASSIC;
-      A.coef = rk_coef[k];
-      A.use_acc = (k == 3);
-      A.h = o.steady ? (k == 2 ? 1.0 : (k == 3 ? 1.0 / 6.0 : 1.0 / 2.0)) : h_rk[k];
-      const double t12 = told + 0.5 * dt, tnew = told + dt;
-      const double tst[4] = {told, t12, t12, tnew};
-      ts = tst[k];
-    }
+    const Upd A = stage_upd(o, k, told, ts);
     int rc;
     if (C.stage) {
       rc = launch_stage(C, A, ts);
@@ -2454,17 +2581,76 @@ int ensure_graph(fvs_ctx &C, int gsteps, double t1, double *dn, double *dinst) {
   return 0;
 }
 
-int run_steps(fvs_ctx &C, double t1, int nsteps, double *dn, double *dv, double *dinst) {
+// nsteps whole steps in one cooperative launch of k_steps_persistent (the
+// whole-stage block body over the launch-block list, a grid barrier after
+// every stage, the step norms in the same launch); swaps the primitive
+// buffers like the per-stage launches would
+int launch_persistent(fvs_ctx &C, double t1, int nsteps, double *dn, double *dinst) {
+  const fvs_options &o = C.opt;
+  PersistCtl c{};
+  c.nst = (o.rk_scheme == FVS_RK_SSP3) ? 3 : 4;
+  for (int k = 0; k < c.nst; ++k) {
+    double ts = t1;
+    c.A[k] = stage_upd(o, k, t1, ts);
+  }
+  c.nsteps = nsteps;
+  c.nlaunch = C.nact ? C.nact : nblk(C.N);
+  c.W0 = C.W;
+  c.W1 = C.Walt;
+  c.d2buf = C.d2buf;
+  c.dn = C.nact ? nullptr : dn;
+  c.nparts = nblk(C.N);
+  const bool inst = dinst && C.ninst > 0;
+  c.dinst = inst ? dinst : nullptr;
+  c.ninst = C.ninst;
+  c.inst_off = C.dinst_off;
+  c.ctr = C.dctr;
+  c.bar = C.pbar;
+  c.err = C.perr;
+  c.spin = 1u << 22;
+  StageArgs a = stage_args(C, t1);
+  a.nb = nblk(C.N);  // plain numbering: the persistent loop walks the blocks itself
+  const size_t lds = std::max(stage_lds(C), size_t(4) * kNormThreads * sizeof(double));
+  const bool lim = o.limiter > 0 && !C.limiter_frozen;
+  const void *fn = nullptr;
+#define FVS_PK(L, PM) fn = reinterpret_cast<const void *>(&k_steps_persistent<L, PM>)
+  if (lim) {
+    if (C.pmode == 0) FVS_PK(1, 0);
+    else if (C.pmode == 1) FVS_PK(1, 1);
+    else if (C.pmode == 2) FVS_PK(1, 2);
+    else FVS_PK(1, 3);
+  } else {
+    if (C.pmode == 0) FVS_PK(0, 0);
+    else if (C.pmode == 1) FVS_PK(0, 1);
+    else if (C.pmode == 2) FVS_PK(0, 2);
+    else FVS_PK(0, 3);
+  }
+#undef FVS_PK
+  int per_cu = 0;
+  HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kStageThreads, lds));
+  if (per_cu < 1) return fail(FVS_ESTATE, "persistent launch: the whole-stage kernel does not fit on a compute unit");
+  const int grid = std::min(c.nlaunch, per_cu * C.ncu);
+  hipLaunchKernelGGL(k_barrier_reset, dim3(1), dim3(64), 0, C.stream, C.pbar, C.perr);
+  void *args[] = {&a, &c};
+  HIPC(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kStageThreads), args, unsigned(lds), C.stream));
+  C.persist_ran = true;
+  if ((nsteps * c.nst) % 2) std::swap(C.W, C.Walt);
+  return 0;
+}
+
+int run_steps(fvs_ctx &C, double t1, int nsteps, double *dn, double *dv, double *dinst, bool graph_ok = true) {
   const double dt = C.opt.dt;
   const int nst = (C.opt.rk_scheme == FVS_RK_SSP3) ? 3 : 4;
   const bool cons = cons_stages(C);
   if (!cons) ensure_prim(C);  // the stages read W
   hipLaunchKernelGGL(k_counter, dim3(1), dim3(1), 0, C.stream, C.dctr, 0, 0);
   HIPC(hipGetLastError());
+  if (graph_ok && C.use_persistent && C.stage && !C.profiling && !dv && !time_dependent(C))
+    return launch_persistent(C, t1, nsteps, dn, dinst);
   int s = 0;
   // graph replay: one captured graph covers an even number of stages so the
   // primitive-state buffers are back in place after each replay
-  if (C.use_graph && !C.profiling && !dv && !time_dependent(C)) {
+  if (graph_ok && C.use_graph && !C.profiling && !dv && !time_dependent(C)) {
     const int gsteps = (nst % 2 == 0) ? 1 : 2;
     if (nsteps >= 2 * gsteps) {
       if (int rc = ensure_graph(C, gsteps, t1, dn, dinst)) return rc;
@@ -2477,7 +2663,7 @@ int run_steps(fvs_ctx &C, double t1, int nsteps, double *dn, double *dv, double 
     const double told = t1 + double(s) * dt;
     if (int rc = launch_step(C, told, dn, dinst)) return rc;
     if (dv)
-      if (int rc = launch_vortex_err(C, told + dt, dv + 13 * size_t(s))) return rc;
+      if (int rc = launch_vortex_err(C, told + dt, dv + kVerr * size_t(s))) return rc;
   }
   // the conservative stages wrote no primitive state: W is refreshed by the
   // next caller that reads it, not inside the timed steps
@@ -2493,22 +2679,42 @@ int integrate(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr, do
   // more than any earlier one)
   if (int rc = C->grow(C->dn_buf, C->dn_cap, size_t(nsteps) * 4)) return rc;
   if (verr)
-    if (int rc = C->grow(C->dv_buf, C->dv_cap, size_t(nsteps) * 13)) return rc;
+    if (int rc = C->grow(C->dv_buf, C->dv_cap, size_t(nsteps) * kVerr)) return rc;
   const bool want_inst = inst_norms && C->ninst > 0;
   if (want_inst)
     if (int rc = C->grow(C->di_buf, C->di_cap, size_t(nsteps) * C->ninst * 4)) return rc;
   double *dn = C->dn_buf, *dv = verr ? C->dv_buf : nullptr, *di = want_inst ? C->di_buf : nullptr;
+  const bool timers = C->phase_timing && !C->profiling;
+  if (timers) {
+    C->profiling = true;
+    C->evused = 0;
+    C->evclass.clear();
+  }
   HIPC(hipEventRecord(C->ev0, C->stream));
-  if (int rc = run_steps(*C, t1, nsteps, dn, dv, di)) return rc;
+  const int rc_run = run_steps(*C, t1, nsteps, dn, dv, di);
+  if (timers) C->profiling = false;
+  if (rc_run) return rc_run;
   HIPC(hipEventRecord(C->ev1, C->stream));
   // the small outputs queue behind the steps; one wait for all of them
+  int perr = 0;
+  if (C->persist_ran) HIPC(hipMemcpyAsync(&perr, C->perr, sizeof(int), hipMemcpyDeviceToHost, C->stream));
   if (norms) HIPC(hipMemcpyAsync(norms, dn, size_t(nsteps) * 4 * 8, hipMemcpyDeviceToHost, C->stream));
-  if (dv) HIPC(hipMemcpyAsync(verr, dv, size_t(nsteps) * 13 * 8, hipMemcpyDeviceToHost, C->stream));
+  if (dv) HIPC(hipMemcpyAsync(verr, dv, size_t(nsteps) * kVerr * 8, hipMemcpyDeviceToHost, C->stream));
   if (di) HIPC(hipMemcpyAsync(inst_norms, di, size_t(nsteps) * C->ninst * 4 * 8, hipMemcpyDeviceToHost, C->stream));
   HIPC(hipStreamSynchronize(C->stream));
+  if (C->persist_ran) {
+    C->persist_ran = false;
+    if (perr) return fail(FVS_EHIP, "persistent launch: a workgroup gave up waiting at a grid barrier");
+  }
   float ms = 0.f;
   HIPC(hipEventElapsedTime(&ms, C->ev0, C->ev1));
   C->last_ms = ms;
+  if (timers)
+    for (size_t k = 0; k < C->evclass.size(); ++k) {
+      float t = 0.f;
+      HIPC(hipEventElapsedTime(&t, C->evpool[2 * k], C->evpool[2 * k + 1]));
+      C->phase_ms[C->evclass[k]] += t;
+    }
   return 0;
 }
 
@@ -2523,6 +2729,31 @@ int fvs_time_integration(fvs_ctx *C, double t1, int nsteps, double *norms, doubl
     return integrate(C, t1, nsteps, nullptr, verr, nullptr);
   }
   return integrate(C, t1, nsteps, norms, verr, nullptr);
+}
+
+int fvs_set_phase_timers(fvs_ctx *C, int on) {
+  if (!C) return fail(FVS_EINVAL, "null context");
+  C->phase_timing = on != 0;
+  return 0;
+}
+
+int fvs_phase_times(const fvs_ctx *C, double *sec4) {
+  if (!C || !sec4) return fail(FVS_EINVAL, "null argument");
+  const double *t = C->phase_ms;
+  sec4[0] = t[KC_GRAD] * 1e-3;                                    // gradient, limiter fused in
+  sec4[1] = 0.0;                                                  // limiter: inside the gradient launches
+  sec4[2] = (t[KC_FLUX_INT] + t[KC_FLUX_BND] + t[KC_FUSED]) * 1e-3;  // face fluxes (+ fused stage update)
+  sec4[3] = (t[KC_UPDATE] + t[KC_NORM]) * 1e-3;                   // stage updates, residual norms
+  return 0;
+}
+
+int fvs_set_persistent(fvs_ctx *C, int on) {
+  if (!C) return fail(FVS_EINVAL, "null context");
+  if (on && !C->stage)
+    return fail(FVS_ESTATE, "persistent launches run the whole-stage kernel: create the context with kernel_path 2 "
+                            "(LSQ gradient, triangle-sized blocks)");
+  C->use_persistent = on != 0;
+  return 0;
 }
 
 int fvs_set_graph(fvs_ctx *C, int on) {
@@ -2597,11 +2828,11 @@ int fvs_batch_wall_forces(fvs_ctx *C, double *out) {
   return 0;
 }
 
-int fvs_vortex_error(fvs_ctx *C, double t, double *out13) {
+int fvs_vortex_error(fvs_ctx *C, double t, double *out15) {
   if (int rc = need_state(C)) return rc;
-  if (!out13) return fail(FVS_EINVAL, "null argument");
+  if (!out15) return fail(FVS_EINVAL, "null argument");
   if (int rc = launch_vortex_err(*C, t, C->small)) return rc;
-  HIPC(hipMemcpyAsync(out13, C->small, 13 * 8, hipMemcpyDeviceToHost, C->stream));
+  HIPC(hipMemcpyAsync(out15, C->small, kVerr * 8, hipMemcpyDeviceToHost, C->stream));
   HIPC(hipStreamSynchronize(C->stream));
   return 0;
 }

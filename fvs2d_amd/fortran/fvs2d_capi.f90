@@ -132,11 +132,11 @@ module fvs2d_capi
       integer(c_int) :: rc
     end function
 
-    function fvs_vortex_error(ctx, t, out13) bind(C, name="fvs_vortex_error") result(rc)
+    function fvs_vortex_error(ctx, t, out15) bind(C, name="fvs_vortex_error") result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), value :: t
-      real(c_double), intent(out) :: out13(13)
+      real(c_double), intent(out) :: out15(15)
       integer(c_int) :: rc
     end function
 
@@ -196,6 +196,20 @@ module fvs2d_capi
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), value :: dt, cfl
+      integer(c_int) :: rc
+    end function
+
+    function fvs_set_phase_timers(ctx, on) bind(C, name="fvs_set_phase_timers") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
+      integer(c_int) :: rc
+    end function
+
+    function fvs_phase_times(ctx, sec4) bind(C, name="fvs_phase_times") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: sec4(4)
       integer(c_int) :: rc
     end function
 

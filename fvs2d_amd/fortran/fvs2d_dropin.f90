@@ -255,8 +255,8 @@ end module residual
 
 module runge_kutta
   use iso_c_binding
-  use mainparam,     only : nvar, iunit_res, file_res
-  use input,         only : lvortex, ntstart, lwrite_resid
+  use mainparam,     only : nvar, iunit_res, file_res, iunit_vortex, cput_grad, cput_lim, cput_flux, cput_rk
+  use input,         only : lvortex, ntstart, lwrite_resid, dt
   use data_grid,     only : ncells
   use data_solution, only : cvar
   use fvs2d_capi
@@ -268,24 +268,28 @@ module runge_kutta
   public :: runge_kutta_init, time_integration, istep_rk, cput_drdt, cput_uvwetc
 contains
 
-  ! src/runge_kutta.f90:25-88 (coefficient tables live in the library)
+  ! src/runge_kutta.f90:25-88 (coefficient tables live in the library); the
+  ! library's phase timers feed cput_grad, cput_lim, cput_flux and cput_rk
   subroutine runge_kutta_init
     if (lwrite_resid) then
       open(iunit_res, file=trim(file_res))
-      write(iunit_res,'(a)') 'variables = "iteration" "|r|2" "|ru|2" "|rv|2" "|rE|2"'
+      write(iunit_res,'(a)') 'variables = "iteration" "|<greek>\r</greek>|<sub>2</sub>",  "|<greek>\r</greek>u|<sub>2</sub>", "|<greek>\r</greek>v|<sub>2</sub>", "|<greek>\r</greek>E|<sub>2</sub>"'
     endif
     icont = 0
+    call fvs_check(fvs_set_phase_timers(fvs_ctx, 1_c_int), 'runge_kutta_init')
   end subroutine runge_kutta_init
 
   ! src/runge_kutta.f90:94-114: the whole interval on the device; the
-  ! per-step residual norms (169-184) and vortex errors (src/mms.f90:351-361)
+  ! per-step residual norms (169-184) and vortex errors (src/mms.f90:271-365)
   ! come back as small arrays and are logged as the reference logs them
   subroutine time_integration(t1, ntimes_sub)
     real, intent(in)    :: t1
     integer, intent(in) :: ntimes_sub
     real(c_double), allocatable, target :: norms(:,:), verr(:,:)
+    real(c_double) :: sec(4)
+    real :: tnew
     integer :: k
-    allocate(norms(4, ntimes_sub), verr(13, ntimes_sub))
+    allocate(norms(4, ntimes_sub), verr(15, ntimes_sub))
     call fvs_check(fvs_set_state(fvs_ctx, cvar), 'time_integration')
     if (lvortex) then
       call fvs_check(fvs_time_integration(fvs_ctx, t1, int(ntimes_sub,c_int), norms, c_loc(verr)), &
@@ -295,11 +299,38 @@ contains
                      'time_integration')
     endif
     call fvs_check(fvs_get_state(fvs_ctx, cvar), 'time_integration')
+    call fvs_check(fvs_phase_times(fvs_ctx, sec), 'time_integration')
+    cput_grad = sec(1); cput_lim = sec(2); cput_flux = sec(3); cput_rk = sec(4)
     do k = 1, ntimes_sub
       icont = icont + 1
+      if (lvortex) then
+        tnew = (t1 + dble(k-1)*dt) + dt
+        call log_vortex_error(tnew, verr(:,k))
+      endif
       if (lwrite_resid) write(iunit_res,'(i0,1x,4(e16.8))') icont+ntstart-1, norms(1:4,k)
     enddo
     deallocate(norms, verr)
   end subroutine time_integration
+
+  ! the output of error_isentropic_vortex (src/mms.f90:280-292, 351-363) for
+  ! one step's record: 13 error norms, then the centre of the cell with the
+  ! largest density error
+  subroutine log_vortex_error(time, v)
+    real, intent(in)           :: time
+    real(c_double), intent(in) :: v(15)
+    if (time==dt) then
+      open(iunit_vortex, file='log_vortex_err.plt')
+      write(iunit_vortex,'(a)') 'variables = "t", '
+      write(iunit_vortex,'(a)') '"<greek>r</greek><sub>max</sub>",  "<greek>r</greek><sub>L1</sub>", "<greek>r</greek><sub>L2</sub>",'
+      write(iunit_vortex,'(a)') '"<greek>r</greek>u<sub>max</sub>",  "<greek>r</greek>u<sub>L1</sub>", "<greek>r</greek>u<sub>L2</sub>",'
+      write(iunit_vortex,'(a)') '"<greek>r</greek>v<sub>max</sub>",  "<greek>r</greek>v<sub>L1</sub>", "<greek>r</greek>v<sub>L2</sub>",'
+      write(iunit_vortex,'(a)') '"<greek>r</greek>E<sub>max</sub>",  "<greek>r</greek>E<sub>L1</sub>", "<greek>r</greek>E<sub>L2</sub>",'
+      write(iunit_vortex,'(a)') '"Q<sub>L2</sub>"'
+      open(701, file='log_vortex_err_xy.plt')
+      write(701,'(a)') 'variables = "t", "x" "y"'
+    endif
+    write(iunit_vortex, '(14(e16.8,1x))') time, v(1:13)
+    write(701,*) time, v(14), v(15)
+  end subroutine log_vortex_error
 
 end module runge_kutta

@@ -291,7 +291,7 @@ class Solver:
 
     def time_integration(self, t1: float, nsteps: int, vortex_error: bool = False):
         norms = np.zeros((nsteps, 4))
-        verr = np.zeros((nsteps, 13)) if vortex_error else None
+        verr = np.zeros((nsteps, 15)) if vortex_error else None
         check(self._L.fvs_time_integration(self._h, t1, nsteps, ptr(norms), ptr(verr)))
         return (norms, verr) if vortex_error else norms
 
@@ -301,6 +301,21 @@ class Solver:
     def set_graph(self, on: bool = True):
         """Replay each time step from a captured HIP graph (fvs_set_graph)."""
         check(self._L.fvs_set_graph(self._h, int(on)))
+
+    def set_persistent(self, on: bool = True):
+        """Run each time-integration call as one persistent cooperative launch
+        (fvs_set_persistent; needs kernel_path 2)."""
+        check(self._L.fvs_set_persistent(self._h, int(on)))
+
+    def set_phase_timers(self, on: bool = True):
+        """Accumulate device time per phase (fvs_set_phase_timers)."""
+        check(self._L.fvs_set_phase_timers(self._h, int(on)))
+
+    def phase_times(self) -> np.ndarray:
+        """Seconds per phase [gradient, limiter, flux, Runge-Kutta] (fvs_phase_times)."""
+        out = np.zeros(4)
+        check(self._L.fvs_phase_times(self._h, ptr(out)))
+        return out
 
     def steady_solve(self, max_steps: int, drop: float = 1e-8, chunk: int = 200, t1: float = 0.0,
                      freeze_at: Optional[float] = None, freeze_after: Optional[int] = None,
@@ -371,7 +386,7 @@ class Solver:
         self.cvar = restart.read_cont(base, self.ncells, self.nnodes, imach)
 
     def error_isentropic_vortex(self, t: float) -> np.ndarray:
-        out = np.zeros(13)
+        out = np.zeros(15)
         check(self._L.fvs_vortex_error(self._h, t, ptr(out)))
         return out
 

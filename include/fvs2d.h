@@ -110,6 +110,28 @@ int fvs_batch_wall_forces(fvs_ctx *ctx, double *out);
  * instances, autotuned choices).  Time-dependent boundary data (the
  * Dirichlet vortex) always use stream launches. */
 int fvs_set_graph(fvs_ctx *ctx, int on);
+/* Phase timers, the counterpart of the reference's cput_grad, cput_lim,
+ * cput_flux and cput_rk (src/mainparam.f90:15-16, accumulated around each
+ * phase and printed by src/fvs2d.f90:175-182): while on != 0, every
+ * fvs_time_integration call records HIP events around each launch (and
+ * replays no graph) and adds each launch's device time to its phase. */
+int fvs_set_phase_timers(fvs_ctx *ctx, int on);
+/* Seconds accumulated by the phase timers: sec4 = gradient (the limiter is
+ * evaluated inside the gradient launches, so its time is here), limiter
+ * (always 0 for that reason), face fluxes (with the stage update where it is
+ * fused into the flux launch), Runge-Kutta (separate stage-update launches
+ * and the residual-norm reduction). */
+int fvs_phase_times(const fvs_ctx *ctx, double *sec4);
+/* Launch mode, no reference counterpart: on != 0 runs each
+ * fvs_time_integration call (or batch call) as one persistent cooperative
+ * launch -- workgroups resident on every compute unit walk the launch blocks
+ * (the active blocks of a masked batch) through each whole RK stage, meet at a
+ * grid barrier after every stage and reduce each step's residual norms in the
+ * same launch -- instead of per-stage launches.  Results are bit-identical to
+ * them.  Needs the whole-stage kernel (kernel_path 2); time-dependent boundary
+ * data and calls that want vortex errors use per-stage launches.  A barrier
+ * wait is bounded: a launch whose workgroups fail to meet returns FVS_EHIP. */
+int fvs_set_persistent(fvs_ctx *ctx, int on);
 /* replaces data_sol_close / data_grid_close (src/data_solution.f90:130-140) */
 int fvs_destroy(fvs_ctx *ctx);
 
@@ -172,11 +194,15 @@ int fvs_roe_flux_batch(int n, double gamma, const double *pL, const double *pR, 
 int fvs_local_dt(fvs_ctx *ctx, double *dt);
 /* time_integration (src/runge_kutta.f90:94-114): nsteps steps from t1.
  * norms (nsteps*4, may be NULL) receives the log_res.plt norms
- * (src/runge_kutta.f90:169-184); verr (nsteps*13, may be NULL) the
- * error_isentropic_vortex record (src/mms.f90:351-361) after each step. */
+ * (src/runge_kutta.f90:169-184); verr (nsteps*15, may be NULL) after each
+ * step the error_isentropic_vortex record: the 13 values of one
+ * log_vortex_err.plt row after its time (src/mms.f90:351-361), then x, y of
+ * the cell with the largest density error (log_vortex_err_xy.plt,
+ * src/mms.f90:363). */
 int fvs_time_integration(fvs_ctx *ctx, double t1, int nsteps, double *norms, double *verr);
-/* error_isentropic_vortex (src/mms.f90:271-365) on the current state */
-int fvs_vortex_error(fvs_ctx *ctx, double t, double *out13);
+/* error_isentropic_vortex (src/mms.f90:271-365) on the current state: out15
+ * as one step's verr record of fvs_time_integration */
+int fvs_vortex_error(fvs_ctx *ctx, double t, double *out15);
 /* write_inst_cp_un C_L/C_D (src/io.f90:340-449), summed over slip/solid wall
  * boundaries: out4 = cl, cd, cl1, cd1 */
 int fvs_wall_forces(fvs_ctx *ctx, double *out4);

@@ -1318,6 +1318,10 @@ void orc_initialize(orc_t *o, int mode, double t0) {
 void orc_vortex_error(const orc_t *o, double time, double *out) {
   double g = o->op.gamma;
   double mx[4] = {0, 0, 0, 0}, l1[4] = {0, 0, 0, 0}, l2[4] = {0, 0, 0, 0}, eL2 = 0.0;
+  /* erho (src/mms.f90:292,321) is zero outside the interior cells; its
+   * maxloc (363) is the first cell holding the largest value */
+  int imax = 0;
+  double emax = 0.0;
   for (int i = 0; i < o->ncells_intr; ++i) {
     int ic = o->cell_intr[i];
     double pv[4];
@@ -1329,6 +1333,7 @@ void orc_vortex_error(const orc_t *o, double time, double *out) {
     ex[3] = pv[3] / (g - 1.0) + 0.5 * ex[0] * (pv[1] * pv[1] + pv[2] * pv[2]);
     double d[4];
     for (int k = 0; k < 4; ++k) d[k] = fabs(o->cvar[4 * ic + k] - ex[k]);
+    if (d[0] > emax || (d[0] == emax && emax > 0.0 && ic < imax)) { emax = d[0]; imax = ic; }
     for (int k = 0; k < 4; ++k) {
       mx[k] = (d[k] > mx[k]) ? d[k] : mx[k];
       l1[k] = l1[k] + d[k];
@@ -1343,6 +1348,8 @@ void orc_vortex_error(const orc_t *o, double time, double *out) {
     out[3 * k + 2] = sqrt(l2[k] / n);
   }
   out[12] = sqrt(eL2 / n);
+  out[13] = o->cx[imax];
+  out[14] = o->cy[imax];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1469,7 +1476,7 @@ void orc_time_integration(orc_t *o, double t1, int nsteps, double *norms, double
       }
       tend = told + dt;
     }
-    if (verr) orc_vortex_error(o, tend, verr + 13 * (istep - 1));
+    if (verr) orc_vortex_error(o, tend, verr + 15 * (istep - 1));
     if (norms) residual_norms(o, cvar0, norms + 4 * (istep - 1));
   }
   free(cvar0); free(fcvar);

@@ -95,12 +95,14 @@ int  orc_bc_state(const orc_t *o, int bc_type, double t, double x, double y, dou
 
 /* initial solution: mode 1 free stream (+vortex if lvortex), 0 MMS; t0 = start time */
 void orc_initialize(orc_t *o, int mode, double t0);
-/* integrate nsteps from t1; norms: nsteps*4 (may be NULL); verr: nsteps*13 (may be NULL) */
+/* integrate nsteps from t1; norms: nsteps*4 (may be NULL); verr: nsteps*15 (may be NULL) */
 void orc_time_integration(orc_t *o, double t1, int nsteps, double *norms, double *verr);
 /* per-cell work of later calls on n threads (OpenMP; 1 = serial, the
  * default); the results are bit-identical to the serial ones */
 int  orc_set_threads(orc_t *o, int n);
-void orc_vortex_error(const orc_t *o, double t, double *out13);
+/* the 13 norms of src/mms.f90:351-361, then x, y of the cell with the
+ * largest density error (src/mms.f90:363) */
+void orc_vortex_error(const orc_t *o, double t, double *out15);
 /* wall forces over slip/solid walls: cl, cd, cl1, cd1 (src/io.f90:340-449) */
 void orc_wall_forces(const orc_t *o, double *out4);
 int orc_wall_cp(const orc_t *o, double *out, double *un3);

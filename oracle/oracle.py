@@ -259,7 +259,7 @@ class Oracle:
 
     def time_integration(self, t1, nsteps, want_verr=False):
         norms = np.zeros((nsteps, 4))
-        verr = np.zeros((nsteps, 13)) if want_verr else None
+        verr = np.zeros((nsteps, 15)) if want_verr else None
         self.L.orc_time_integration(C.c_void_p(self.h), t1, nsteps, norms.ctypes.data,
                                     verr.ctypes.data if want_verr else None)
         return (norms, verr) if want_verr else norms
@@ -283,7 +283,7 @@ class Oracle:
         return out
 
     def vortex_error(self, t):
-        out = np.zeros(13)
+        out = np.zeros(15)
         self.L.orc_vortex_error(C.c_void_p(self.h), t, out)
         return out
 

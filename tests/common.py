@@ -18,6 +18,18 @@ def rel_err(a, b, floor=1e-300):
     return np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))
 
 
+def verr_err(got, ref, min_same_location=0.9):
+    """Largest relative difference of the 13 error norms of vortex-error
+    records (one record or one per step), after checking that the location
+    of the largest density error (the last two values) is the same cell in at
+    least ``min_same_location`` of the records (a near tie between two cells
+    may resolve differently under rounding)."""
+    got, ref = np.atleast_2d(got), np.atleast_2d(ref)
+    same = np.all(got[:, 13:15] == ref[:, 13:15], axis=1)
+    assert np.mean(same) >= min_same_location, (got[~same, 13:15], ref[~same, 13:15])
+    return rel_err(got[:, :13], ref[:, :13])
+
+
 def scaled_err(a, b):
     """max |a-b| / max|b| per column (for quantities that cross zero)."""
     a = np.asarray(a).reshape(b.shape[0], -1) if a.ndim > 1 else a[:, None]

@@ -3,7 +3,9 @@
 ! Reads mesh.grid / mesh.bc (the reference's text format), flags.nml and
 ! cvar0.bin from the working directory, then calls the drop-in procedures in
 ! the reference's order and writes resid.bin, cvar.bin and dp.bin (raw
-! little-endian doubles).
+! little-endian doubles) and the phase timers to cput.txt; the drop-in itself
+! writes log_res.plt and, for the vortex, log_vortex_err.plt and
+! log_vortex_err_xy.plt.
 !------------------------------------------------------------------------------!
 subroutine fvs2d_fortran_driver(nsteps) bind(C, name="fvs2d_fortran_driver")
   use iso_c_binding
@@ -90,6 +92,13 @@ subroutine fvs2d_fortran_driver(nsteps) bind(C, name="fvs2d_fortran_driver")
   write(16) cvar
   close(16)
   close(iunit_res)
+  if (lvortex) then
+    close(iunit_vortex)
+    close(701)
+  endif
+  open(18, file='cput.txt', status='replace')
+  write(18,*) cput_grad, cput_lim, cput_flux, cput_rk
+  close(18)
 
   call cvar2pvar
   allocate(dp(ncells, 2))

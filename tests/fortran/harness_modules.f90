@@ -8,8 +8,9 @@
 module mainparam
   implicit none
   integer, parameter :: nvar = 4, grid_dim = 2
-  integer, parameter :: iunit_res = 907
+  integer, parameter :: iunit_res = 907, iunit_vortex = 903
   character(len=15) :: file_res = 'log_res.plt'
+  real, save :: cput_grad = 0.d0, cput_lim = 0.d0, cput_flux = 0.d0, cput_rk = 0.d0
 end module mainparam
 
 module input

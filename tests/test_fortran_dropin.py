@@ -3,15 +3,28 @@ drives its own: data_solution_init, gradient_init, runge_kutta_init,
 compute_residual, time_integration, gradient_cellcntr_1var.  The Fortran test program is a
 shared library (tests/fortran, built by __graft_entry__.build) called in this
 process; its results must equal the Python binding's bit for bit (same library,
-same options), and the Python binding is checked against the oracle elsewhere."""
+same options), and the Python binding is checked against the oracle elsewhere.
+The log files the drop-in writes (log_res.plt, log_vortex_err.plt,
+log_vortex_err_xy.plt) carry the reference's headers and rows, the vortex rows
+checked against the oracle's error_isentropic_vortex; the phase timers
+(cput_*) are filled.
+
+Where the reference sources are present (this build container), the drop-in
+also compiles against the reference's own mainparam and data_grid modules
+(src/mainparam.f90, src/data_grid.f90, compiled where they lie into a
+temporary directory); its input module needs MPI, so the test harness's
+reduced input module stands beside them."""
 import ctypes as C
 import os
+import re
+import subprocess
 
 import numpy as np
 import pytest
 
 from fvs2d_amd import meshgen
 from fvs2d_amd.solver import Solver
+from oracle.oracle import Oracle
 from tests.common import vortex_scheme
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -40,7 +53,7 @@ def test_fortran_dropin_matches_c_abi(tmp_path, limiter):
     py.initialize_solution(1, 0.0)
     cv0 = py.cvar
     r_py, _ = py.compute_residual(0.0)
-    py.time_integration(0.0, nsteps)
+    py_norms = py.time_integration(0.0, nsteps, vortex_error=True)
     cv_py = py.cvar
     dp_py = py.gradient_cellcntr_1var(py.cvar2pvar()[:, 3])
 
@@ -65,3 +78,50 @@ def test_fortran_dropin_matches_c_abi(tmp_path, limiter):
     assert np.array_equal(dp_f, dp_py)
     log = (tmp_path / "log_res.plt").read_text().splitlines()
     assert len(log) == 1 + nsteps
+    assert log[0] == RES_HEADER
+    norms = py_norms[0]
+    for k, line in enumerate(log[1:]):
+        f = line.split()
+        assert int(f[0]) == k + 1
+        assert np.allclose([float(v) for v in f[1:]], norms[k], rtol=1e-7, atol=0)
+    # error_isentropic_vortex's files against the oracle's record
+    orc = Oracle(m, **sch.oracle_kwargs())
+    orc.cvar = cv0
+    _, v_ref = orc.time_integration(0.0, nsteps, want_verr=True)
+    vlog = (tmp_path / "log_vortex_err.plt").read_text().splitlines()
+    assert vlog[0] == 'variables = "t", ' and vlog[5] == '"Q<sub>L2</sub>"'
+    rows = np.array([[float(v) for v in ln.split()] for ln in vlog[6:]])
+    assert rows.shape == (nsteps, 14)
+    assert np.allclose(rows[:, 0], dt * np.arange(1, nsteps + 1), rtol=1e-7)
+    assert np.allclose(rows[:, 1:], v_ref[:, :13], rtol=1e-7, atol=0)
+    xy = (tmp_path / "log_vortex_err_xy.plt").read_text().splitlines()
+    assert xy[0] == 'variables = "t", "x" "y"'
+    xyr = np.array([[float(v) for v in ln.split()] for ln in xy[1:]])
+    assert xyr.shape == (nsteps, 3)
+    assert np.mean(np.all(np.isclose(xyr[:, 1:], v_ref[:, 13:15], rtol=1e-12, atol=1e-12), axis=1)) >= 0.9
+    cput = [float(v) for v in (tmp_path / "cput.txt").read_text().split()]
+    assert cput[0] > 0 and cput[1] == 0 and cput[2] > 0 and cput[3] > 0 and all(np.isfinite(cput))
+
+
+RES_HEADER = ('variables = "iteration" "|<greek>\\r</greek>|<sub>2</sub>",  "|<greek>\\r</greek>u|<sub>2</sub>", '
+              '"|<greek>\\r</greek>v|<sub>2</sub>", "|<greek>\\r</greek>E|<sub>2</sub>"')
+
+REF_SRC = "/root/reference/src"
+FLANG = "/opt/rocm/bin/amdflang"
+
+
+def test_dropin_compiles_against_reference_modules(tmp_path):
+    ref = [os.path.join(REF_SRC, n) for n in ("mainparam.f90", "data_grid.f90")]
+    if not all(os.path.exists(p) for p in ref) or not os.path.exists(FLANG):
+        pytest.skip("reference sources or amdflang not present")
+    harness = open(os.path.join(ROOT, "tests", "fortran", "harness_modules.f90")).read()
+    inp = re.search(r"^module input\b.*?^end module input", harness, re.S | re.M).group(0)
+    (tmp_path / "input_harness.f90").write_text(inp + "\n")
+    srcs = [ref[0], str(tmp_path / "input_harness.f90"), ref[1],
+            os.path.join(ROOT, "fvs2d_amd", "fortran", "fvs2d_capi.f90"),
+            os.path.join(ROOT, "fvs2d_amd", "fortran", "fvs2d_dropin.f90")]
+    for s in srcs:
+        cmd = [FLANG, "-O0", "-fPIC", "-fdefault-real-8", "-fdefault-double-8", "-module-dir", str(tmp_path), "-I",
+               str(tmp_path), "-c", s, "-o", str(tmp_path / (os.path.basename(s) + ".o"))]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        assert r.returncode == 0, r.stdout + r.stderr

@@ -466,6 +466,35 @@ def test_isentropic_vortex_closed_form():
         assert math.isclose(p, r ** G, rel_tol=1e-12)
 
 
+def test_vortex_error_record_brute_force():
+    """error_isentropic_vortex (src/mms.f90:271-365) restated in numpy from the
+    closed-form vortex at the interior cell centres: max, mean and RMS of
+    |U - U_exact| per conservative variable, the total RMS, and the centre of
+    the first cell holding the largest density error (maxloc)."""
+    m = meshgen.jittered_square(12, seed=3)
+    o = Oracle(m, **vortex_scheme(vortex_x0=0.4, vortex_K=4.0).oracle_kwargs())
+    o.initialize(1, 0.0)
+    rng = np.random.default_rng(21)
+    o.cvar = o.cvar * (1.0 + 1e-3 * rng.standard_normal((o.ncells, 4)))
+    t = 0.3
+    got = o.vortex_error(t)
+    cl, ci = o.cells(), o.lists()["cell_intr"]
+    d = np.zeros((ci.size, 4))
+    cv = o.cvar
+    for k, ic in enumerate(ci):
+        r, u, v, p = o.vortex_state(t, cl["x"][ic], cl["y"][ic])
+        ex = np.array([r, r * u, r * v, p / (G - 1) + 0.5 * r * (u * u + v * v)])
+        d[k] = np.abs(cv[ic] - ex)
+    n = ci.size
+    ref = np.concatenate([np.stack([d.max(0), d.sum(0) / n, np.sqrt((d ** 2).sum(0) / n)], 1).ravel(),
+                          [np.sqrt((d ** 2).sum() / n)]])
+    assert np.allclose(got[:13], ref, rtol=1e-13, atol=0)
+    erho = np.zeros(o.ncells)
+    erho[ci] = d[:, 0]
+    imax = int(np.argmax(erho))
+    assert got[13] == cl["x"][imax] and got[14] == cl["y"][imax]
+
+
 def test_wall_forces_linear_pressure():
     """p = a + b x + c y with exact (LSQ) gradients: the wall pressure is exact
     at each face midpoint, and the midpoint rule integrates a linear field
