@@ -26,7 +26,7 @@ EXPORTS = (
     "fvs_limiter", "fvs_compute_residual", "fvs_roe_flux_batch", "fvs_local_dt", "fvs_time_integration",
     "fvs_vortex_error", "fvs_wall_forces", "fvs_set_timestep", "fvs_kernel_times",
     "fvs_autotune", "fvs_freeze_limiter", "fvs_batch_freeze_limiter", "fvs_wall_cp", "fvs_last_elapsed_ms",
-    "fvs_set_phase_timers", "fvs_phase_times", "fvs_set_persistent",
+    "fvs_set_phase_timers", "fvs_phase_times", "fvs_set_persistent", "fvs_set_launch_choices",
 )
 
 
@@ -68,6 +68,7 @@ def load():
     L.fvs_set_graph.argtypes = [V, I]
     L.fvs_set_phase_timers.argtypes = [V, I]
     L.fvs_set_persistent.argtypes = [V, I]
+    L.fvs_set_launch_choices.argtypes = [V, V]
     L.fvs_phase_times.argtypes = [V, V]
     L.fvs_mesh_create.argtypes = [C.POINTER(V), I, V, I, I, V, V, I, V, V, V, I, I, D, I]
     for n in ("fvs_destroy", "fvs_mesh_destroy"):

@@ -286,7 +286,9 @@ __device__ __forceinline__ const unsigned *active_mask(const int *blist, int nbl
   return reinterpret_cast<const unsigned *>(blist + nbl);
 }
 
-template <int NT>
+// NT threads per workgroup, CB cells per block (CB <= NT); instance masks
+// exist for 256-cell blocks only
+template <int NT, int CB = kBlock>
 __device__ __forceinline__ void block_update_finish(int N, int lb, const Upd &A, const UpdIn &in,
                                                     const double *__restrict__ lF, const double *__restrict__ lS,
                                                     double *__restrict__ Ua, double *__restrict__ Ub,
@@ -294,14 +296,14 @@ __device__ __forceinline__ void block_update_finish(int N, int lb, const Upd &A,
                                                     double *__restrict__ dtl, double *__restrict__ part,
                                                     double *__restrict__ d2out, double *__restrict__ red,
                                                     const unsigned *__restrict__ amask) {
-  const int i = lb * kBlock + threadIdx.x;
+  const int i = lb * CB + threadIdx.x;
   double d2[4] = {0, 0, 0, 0};
   // amask (batches with masked instances): bit l of the block's eight words
   // is set when its cell l belongs to an active instance; a masked cell
   // keeps its state (the update is computed, nothing is stored) and
   // contributes zero increments
-  const bool on = !amask || threadIdx.x >= kBlock || ((amask[8 * lb + (threadIdx.x >> 5)] >> (threadIdx.x & 31)) & 1u);
-  if (threadIdx.x < kBlock && i < N) {
+  const bool on = !amask || threadIdx.x >= CB || ((amask[8 * lb + (threadIdx.x >> 5)] >> (threadIdx.x & 31)) & 1u);
+  if (threadIdx.x < CB && i < N) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0}, wacc = 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -385,7 +387,10 @@ __device__ __forceinline__ void block_update_finish(int N, int lb, const Upd &A,
 // from the staged copy and writes the new one to Ub (a buffer other than
 // Win).  Converted copies would need another 8 KB of LDS or 8 registers per
 // lane, a wavefront per SIMD less.
-__global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, int maxf, Upd A, Phys P, double t,
+// BS cells (and threads) per workgroup; the per-block face lists (boff,
+// lrec, lgeo, lgat) are those built for blocks of BS cells.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_flux_update(int N, int nb, int gw, int maxf, Upd A, Phys P, double t,
                                                         const int *__restrict__ boff, const int2 *__restrict__ lrec,
                                                         const double *__restrict__ lgeo, long L,
                                                         const short *__restrict__ lgat,
@@ -402,12 +407,12 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
   double *lF = lds;             // [maxf][4] flux*area (later the block reduction)
   double *lS = lds + 4 * maxf;  // [maxf]    wave speed*area, only when the stage sets local time steps
   const bool ws = A.steady && A.first;
-  __shared__ double sG[8 * kBlock];  // gradients of the block's own cells
-  __shared__ double sW[4 * kBlock];  // and their states (primitive; with cons conservative)
+  __shared__ double sG[8 * BS];  // gradients of the block's own cells
+  __shared__ double sW[4 * BS];  // and their states (primitive; with cons conservative)
   const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
   const int nbl = nb < 0 ? -nb : nb;
   if (lb >= nbl) return;  // the whole workgroup leaves before any barrier
-  const int i0 = lb * kBlock;
+  const int i0 = lb * BS;
   const int f0 = boff[lb], nf = boff[lb + 1] - f0;
   // the first face record is in flight while the block's own cells are staged
   int f = threadIdx.x;
@@ -426,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
     st4(sW, threadIdx.x, ww);
   }
   __syncthreads();
-  for (; f < nf; f += kBlock) {
+  for (; f < nf; f += BS) {
     if (f != int(threadIdx.x)) {
       c = lrec[f0 + f];
       for (int k = 0; k < 7; ++k) g[k] = lgeo[k * L + f0 + f];
@@ -435,14 +440,14 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
     if (c.y >= 0) {
       double wL[4], wR[4], gL[8], gR[8];
       const unsigned la = unsigned(c.x - i0), lc = unsigned(c.y - i0);
-      if (la < unsigned(kBlock)) {
+      if (la < unsigned(BS)) {
         ld4(sW, la, wL);
         ld8(sG, la, gL);
       } else {
         ld4(Win, c.x, wL);
         ld8(grad, c.x, gL);
       }
-      if (lc < unsigned(kBlock)) {
+      if (lc < unsigned(BS)) {
         ld4(sW, lc, wR);
         ld8(sG, lc, gR);
       } else {
@@ -475,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_update(int N, int nb, int gw, i
     }
   }
   __syncthreads();
-  block_update_finish<kBlock>(N, lb, A, in, lF, lS, Ua, Ub, Facc, Wout, dtl, part, d2out, lF,
+  block_update_finish<BS, BS>(N, lb, A, in, lF, lS, Ua, Ub, Facc, Wout, dtl, part, d2out, lF,
                               blist ? active_mask(blist, nbl) : nullptr);
 }
 

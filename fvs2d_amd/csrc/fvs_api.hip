@@ -47,6 +47,7 @@ int fail(int code, const std::string &msg) {
   } while (0)
 
 constexpr int kBlock = 256;
+constexpr int kSmallBlock = 128;  // the other block size the fused flux kernel can run with
 constexpr int kVerr = 15;  // vortex-error record: 13 norms + location of the largest density error
 constexpr int kSlice = 64;  // one wavefront
 
@@ -940,6 +941,15 @@ struct fvs_ctx {
   int maxf = 0;
   int *boff = nullptr, *bfaces = nullptr;
   short *lgat = nullptr;  // signed 1-based index into the block's face list
+  // the same per-block lists for blocks of kSmallBlock cells (single meshes),
+  // used by the fused flux kernel when small_blocks is set (autotuned)
+  bool have_small = false, small_blocks = false;
+  int maxf_s = 0;
+  long nlisted_s = 0;
+  int *boff_s = nullptr;
+  short *lgat_s = nullptr;
+  int2 *lrec_s = nullptr;
+  double *lgeo_s = nullptr;
   long nlisted = 0;
   // the listed faces' records in list order: cells (interior: c1, c2;
   // boundary: cell, -1 - (type | instance << 2)) and geometry, 7 planes
@@ -1013,6 +1023,7 @@ struct fvs_ctx {
   int *perr = nullptr;
   int ncu = 0;
   bool persist_ran = false;
+  size_t lds_max = 64 * 1024;  // LDS per workgroup the device allows
   // W does not hold the primitive state of U0 (the conservative-stage path
   // writes none); refreshed by the first caller that reads W
   bool w_stale = false;
@@ -1259,7 +1270,7 @@ int setup_stage(fvs_ctx &C, const std::vector<int> &boff, const std::vector<int>
   }
   max1 += max1 & 1;  // keep the LDS arrays after lP 16-byte aligned
   const int max2n = max2;
-  if (size_t(6) * max2n * sizeof(double) > 64 * 1024) return 0;
+  if (size_t(6) * max2n * sizeof(double) > C.lds_max) return 0;
   // k_stage's block reduction reuses the staged-state area: 4 x 512 doubles
   max2 = std::max(max2, kStageThreads);
   const size_t lds = (size_t(6) * max2 + size_t(9) * max1 + size_t(5) * C.maxf) * sizeof(double);
@@ -1277,7 +1288,9 @@ int setup_stage(fvs_ctx &C, const std::vector<int> &boff, const std::vector<int>
   C.nslid = long(slid.size());
   C.nlist2 = long(s2list.size());
   C.nslid_own = eo_own;
-  C.stage = lds <= 64 * 1024;
+  // the device's per-workgroup LDS limit (160 KiB on gfx950): blocks of
+  // stretched cells (O-grids) have larger halos
+  C.stage = lds <= C.lds_max;
   return 0;
 }
 
@@ -1415,15 +1428,19 @@ int setup_device(fvs_ctx &C) {
   // touching block b's cells, ascending (interior slots, then boundary slots),
   // and each cell's faces as signed local indices in the gather order
   std::vector<int> h_boff, h_bfaces;
-  // (the fused update gathers at most 4 faces per cell: triangles and quads)
-  if ((C.opt.kernel_path == 0 || C.opt.kernel_path == 2 || C.opt.kernel_path == 3) && gw <= 4) {
-    const int nbk = nblk(N);
-    std::vector<int> boff(nbk + 1, 0), bfaces, loc;
-    std::vector<short> lgat(size_t(gw) * N, 0);
-    int maxf = 0;
+  // per-block lists for blocks of bs cells: boff, bfaces, lgat and the
+  // largest list (maxf)
+  auto block_lists = [&](int bs, std::vector<int> &boff, std::vector<int> &bfaces, std::vector<short> &lgat,
+                         int &maxf) {
+    const int nbk = nblk(N, bs);
+    std::vector<int> loc;
+    boff.assign(nbk + 1, 0);
+    bfaces.clear();
+    lgat.assign(size_t(gw) * N, 0);
+    maxf = 0;
     for (int b = 0; b < nbk; ++b) {
       loc.clear();
-      const int i0 = b * kBlock, i1 = std::min(N, i0 + kBlock);
+      const int i0 = b * bs, i1 = std::min(N, i0 + bs);
       for (int i = i0; i < i1; ++i)
         for (int k = 0; k < gw; ++k) {
           const int sg = gat[size_t(k) * N + i];
@@ -1443,6 +1460,35 @@ int setup_device(fvs_ctx &C) {
       boff[b + 1] = int(bfaces.size());
       maxf = std::max(maxf, int(loc.size()));
     }
+  };
+  // the listed faces' records in list order: cells and 7 geometry planes
+  auto upload_records = [&](const std::vector<int> &bfaces, int2 *&dlrec, double *&dlgeo) -> int {
+    const size_t L = bfaces.size();
+    std::vector<int2> lrec(L);
+    std::vector<double> lgeo(7 * L);
+    for (size_t f = 0; f < L; ++f) {
+      const int sl = bfaces[f];
+      const double *g;
+      if (sl < Fi) {
+        lrec[f] = fic[sl];
+        g = &fig[8 * size_t(sl)];
+      } else {
+        const int bq = sl - Fi;
+        const int inst = C.ninst > 0 ? C.inst_of_real[P[fbc[bq]]] : 0;
+        lrec[f] = make_int2(fbc[bq], -1 - (fbt[bq] | (inst << 2)));
+        g = &fbg[8 * size_t(bq)];
+      }
+      for (int k = 0; k < 7; ++k) lgeo[size_t(k) * L + f] = g[k];
+    }
+    if (int rc = C.upload(dlrec, lrec)) return rc;
+    return C.upload(dlgeo, lgeo);
+  };
+  // (the fused update gathers at most 4 faces per cell: triangles and quads)
+  if ((C.opt.kernel_path == 0 || C.opt.kernel_path == 2 || C.opt.kernel_path == 3) && gw <= 4) {
+    std::vector<int> boff, bfaces;
+    std::vector<short> lgat;
+    int maxf = 0;
+    block_lists(kBlock, boff, bfaces, lgat, maxf);
     // LDS of k_flux_update: face fluxes (at least the 4 x 256 reduction) +
     // the block's gradients and states
     if (size_t(std::max(5 * maxf, 4 * kBlock)) * sizeof(double) + 12 * kBlock * sizeof(double) <= 64 * 1024) {
@@ -1451,28 +1497,22 @@ int setup_device(fvs_ctx &C) {
       if (int rc = C.upload(C.boff, boff)) return rc;
       if (int rc = C.upload(C.bfaces, bfaces)) return rc;
       if (int rc = C.upload(C.lgat, lgat)) return rc;
-      const size_t L = bfaces.size();
-      std::vector<int2> lrec(L);
-      std::vector<double> lgeo(7 * L);
-      for (size_t f = 0; f < L; ++f) {
-        const int sl = bfaces[f];
-        const double *g;
-        if (sl < Fi) {
-          lrec[f] = fic[sl];
-          g = &fig[8 * size_t(sl)];
-        } else {
-          const int bq = sl - Fi;
-          const int inst = C.ninst > 0 ? C.inst_of_real[P[fbc[bq]]] : 0;
-          lrec[f] = make_int2(fbc[bq], -1 - (fbt[bq] | (inst << 2)));
-          g = &fbg[8 * size_t(bq)];
-        }
-        for (int k = 0; k < 7; ++k) lgeo[size_t(k) * L + f] = g[k];
-      }
-      if (int rc = C.upload(C.lrec, lrec)) return rc;
-      if (int rc = C.upload(C.lgeo, lgeo)) return rc;
+      if (int rc = upload_records(bfaces, C.lrec, C.lgeo)) return rc;
       C.fused = true;
       h_boff.swap(boff);
       h_bfaces.swap(bfaces);
+    }
+    // the same lists for blocks of kSmallBlock cells, a launch shape the
+    // autotuner tries (single meshes of at least 64 blocks: batches keep the
+    // 256-cell blocks their instance masks are built for)
+    if (C.fused && C.ninst == 0 && C.opt.kernel_path != 2 && N >= 64 * kSmallBlock) {
+      block_lists(kSmallBlock, boff, bfaces, lgat, maxf);
+      C.maxf_s = maxf;
+      C.nlisted_s = long(bfaces.size());
+      if (int rc = C.upload(C.boff_s, boff)) return rc;
+      if (int rc = C.upload(C.lgat_s, lgat)) return rc;
+      if (int rc = upload_records(bfaces, C.lrec_s, C.lgeo_s)) return rc;
+      C.have_small = true;
     }
   }
   // limiter face offsets (xf - xc) per local edge, in the cell's own frame
@@ -1695,7 +1735,7 @@ int setup_device(fvs_ctx &C) {
   if (int rc = C.alloc(C.WSA, size_t(Fi + Fb))) return rc;
   if (int rc = C.alloc(C.rout, n4)) return rc;
   if (int rc = C.alloc(C.wsout, N)) return rc;
-  C.nparts = std::max(nblk(N), nblk(std::max(C.ncell_intr, 1)));
+  C.nparts = std::max(nblk(N, kSmallBlock), nblk(std::max(C.ncell_intr, 1)));
   if (int rc = C.alloc(C.part, 13 * size_t(C.nparts))) return rc;
   if (int rc = C.alloc(C.small, 64)) return rc;
   if (int rc = C.alloc(C.dctr, 1)) return rc;
@@ -1900,16 +1940,26 @@ int launch_flux_update(fvs_ctx &C, const Upd &A, double t, const double *uin = n
     if (int rc = launch_gradient(C, true, uin)) return rc;
   {
     ProfScope ps(C, KC_FUSED);
-    const int nb = nblk(C.N);
     // the wave-speed array only where the stage sets the local time step
     const int per = (A.steady && A.first) ? 5 : 4;
-    const size_t lds = size_t(std::max(per * C.maxf, 4 * kBlock)) * sizeof(double);
-    hipLaunchKernelGGL(k_flux_update, dim3(C.nact ? C.nact : xcd_grid(nb)), dim3(kBlock), lds, C.stream, C.N,
-                       xcd_arg(nb, C.xcd_fused), C.GW, C.maxf, A, C.P, t, C.boff, C.lrec, C.lgeo, C.nlisted,
-                       C.lgat, cons ? uin : C.W, C.grad, C.phi, C.vol, C.U0, cons ? uout : C.Ub, C.Facc,
-                       cons ? nullptr : C.Walt, C.dtl, C.part, C.nact ? C.blist : nullptr,
-                       C.ninst > 0 ? C.dpinf_inst : nullptr, A.last ? C.d2buf : nullptr,
-                       (C.opt.limiter == 0 && C.opt.recon != 1) ? 1 : 0, cons ? 1 : 0);
+    const int phi_one = (C.opt.limiter == 0 && C.opt.recon != 1) ? 1 : 0;
+    if (C.small_blocks) {
+      const int nb = nblk(C.N, kSmallBlock);
+      const size_t lds = size_t(std::max(per * C.maxf_s, 4 * kSmallBlock)) * sizeof(double);
+      hipLaunchKernelGGL(k_flux_update<kSmallBlock>, dim3(xcd_grid(nb)), dim3(kSmallBlock), lds, C.stream, C.N,
+                         xcd_arg(nb, C.xcd_fused), C.GW, C.maxf_s, A, C.P, t, C.boff_s, C.lrec_s, C.lgeo_s,
+                         C.nlisted_s, C.lgat_s, cons ? uin : C.W, C.grad, C.phi, C.vol, C.U0, cons ? uout : C.Ub,
+                         C.Facc, cons ? nullptr : C.Walt, C.dtl, C.part, nullptr, nullptr,
+                         A.last ? C.d2buf : nullptr, phi_one, cons ? 1 : 0);
+    } else {
+      const int nb = nblk(C.N);
+      const size_t lds = size_t(std::max(per * C.maxf, 4 * kBlock)) * sizeof(double);
+      hipLaunchKernelGGL(k_flux_update<kBlock>, dim3(C.nact ? C.nact : xcd_grid(nb)), dim3(kBlock), lds, C.stream,
+                         C.N, xcd_arg(nb, C.xcd_fused), C.GW, C.maxf, A, C.P, t, C.boff, C.lrec, C.lgeo, C.nlisted,
+                         C.lgat, cons ? uin : C.W, C.grad, C.phi, C.vol, C.U0, cons ? uout : C.Ub, C.Facc,
+                         cons ? nullptr : C.Walt, C.dtl, C.part, C.nact ? C.blist : nullptr,
+                         C.ninst > 0 ? C.dpinf_inst : nullptr, A.last ? C.d2buf : nullptr, phi_one, cons ? 1 : 0);
+    }
   }
   HIPC(hipGetLastError());
   if (!cons) std::swap(C.W, C.Walt);
@@ -2122,6 +2172,10 @@ static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, in
   int rc0 = hipGetDevice(&C->device) == hipSuccess ? 0 : fail(FVS_EHIP, "hipGetDevice failed");
   if (!rc0 && hipDeviceGetAttribute(&C->ncu, hipDeviceAttributeMultiprocessorCount, C->device) != hipSuccess)
     rc0 = fail(FVS_EHIP, "hipDeviceGetAttribute failed");
+  int lmax = 0;
+  if (!rc0 && hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, C->device) != hipSuccess)
+    rc0 = fail(FVS_EHIP, "hipDeviceGetAttribute failed");
+  if (!rc0 && lmax > 0) C->lds_max = size_t(lmax);
   if (rc0) {
     std::string e = g_err;
     delete C;
@@ -2529,7 +2583,9 @@ int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {
   const bool inst = dinst && C.ninst > 0;
   if (!C.nact) {
     ProfScope ps(C, KC_NORM);
-    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kNormThreads), 0, C.stream, nblk(C.N), C.N, C.part, dn, C.dctr,
+    // one partial sum per launch block of the last stage
+    const int nparts = (C.fused && !C.stage && C.small_blocks) ? nblk(C.N, kSmallBlock) : nblk(C.N);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kNormThreads), 0, C.stream, nparts, C.N, C.part, dn, C.dctr,
                        inst ? 0 : 1);
   }
   if (inst)
@@ -2956,14 +3012,15 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   // grad once (phi unless there is no limiter), the cell-side traffic of the
   // update kernel without the face-flux buffer
   const bool phi1 = C->opt.limiter == 0 && C->opt.recon != 1;
-  double bfu = double(C->nlisted) * (8 + 56) + N * (32 + 64 + (phi1 ? 0 : 8)) + N * (C->GW * 2.0 + 8 + 32 + state);
+  const double nlisted = double(C->small_blocks ? C->nlisted_s : C->nlisted);
+  double bfu = nlisted * (8 + 56) + N * (32 + 64 + (phi1 ? 0 : 8)) + N * (C->GW * 2.0 + 8 + 32 + state);
   if (cons_stages(*C)) {
     // conservative stages: the stage input (read once, also the update's
     // stage state; on the first stage also its step-start state), the
     // step-start state on the other stages, the new state; no W write
     double st3 = (32 + 64 + 64) / 3.0;
     if (C->opt.steady) st3 += (8.0 + 8.0 * (nst - 1)) / nst;
-    bfu = double(C->nlisted) * (8 + 56) + N * (32 + 64 + (phi1 ? 0 : 8)) + N * (C->GW * 2.0 + 8 + st3);
+    bfu = nlisted * (8 + 56) + N * (32 + 64 + (phi1 ? 0 : 8)) + N * (C->GW * 2.0 + 8 + st3);
   }
   // whole stage: own states, centres and G^-1 once, the per-block cell lists
   // and local stencils, face records and block lists once, limiter inputs,
@@ -3027,13 +3084,13 @@ int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) {
     ms = C->last_ms / nsteps;
     return 0;
   };
-  bool *knobs[5] = {&C->xcd_grad, &C->xcd_flux, &C->xcd_upd, &C->xcd_fused, &C->use_halo};
-  const bool used[5] = {C->opt.recon != 1, !C->fused, !C->fused, C->fused && !C->stage,
-                        C->halo && !C->stage && C->opt.recon != 1};
+  bool *knobs[6] = {&C->xcd_grad, &C->xcd_flux, &C->xcd_upd, &C->xcd_fused, &C->use_halo, &C->small_blocks};
+  const bool used[6] = {C->opt.recon != 1, !C->fused, !C->fused, C->fused && !C->stage,
+                        C->halo && !C->stage && C->opt.recon != 1, C->have_small && C->fused && !C->stage};
   double best = 0.0;
   if (!rc) rc = trial(best);  // warm-up
   if (!rc) rc = trial(best);
-  for (int k = 0; k < 5 && !rc; ++k) {
+  for (int k = 0; k < 6 && !rc; ++k) {
     if (!used[k]) continue;
     *knobs[k] = !*knobs[k];
     double ms = 0.0;
@@ -3051,8 +3108,25 @@ int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) {
   if (choice4) {
     for (int k = 0; k < 4; ++k) choice4[k] = *knobs[k] ? 1 : 0;
     if (C->halo && C->use_halo) choice4[0] |= 2;
+    if (C->small_blocks) choice4[3] |= 4;
   }
   if (best_ms) *best_ms = best;
+  return 0;
+}
+
+int fvs_set_launch_choices(fvs_ctx *C, const int *choice4) {
+  if (!C || !choice4) return fail(FVS_EINVAL, "null argument");
+  const bool halo = (choice4[0] & 2) != 0, small = (choice4[3] & 4) != 0;
+  if (halo && !C->halo) return fail(FVS_EINVAL, "launch choices: no staged-halo gradient data for this context");
+  if (small && !(C->have_small && C->fused && !C->stage))
+    return fail(FVS_EINVAL, "launch choices: no 128-cell block lists for this context");
+  C->xcd_grad = (choice4[0] & 1) != 0;
+  C->xcd_flux = (choice4[1] & 1) != 0;
+  C->xcd_upd = (choice4[2] & 1) != 0;
+  C->xcd_fused = (choice4[3] & 1) != 0;
+  C->use_halo = halo;
+  C->small_blocks = small;
+  C->launch_gen++;
   return 0;
 }
 

@@ -434,6 +434,12 @@ class Solver:
         names = ("gradient", "flux", "update", "fused")
         return {n: int(c) for n, c in zip(names, ch)}, float(best[0])
 
+    def set_launch_choices(self, choices: dict):
+        """Apply launch choices in the encoding autotune() returns
+        (fvs_set_launch_choices)."""
+        ch = np.array([int(choices.get(n, 0)) for n in ("gradient", "flux", "update", "fused")], np.int32)
+        check(self._L.fvs_set_launch_choices(self._h, ptr(ch)))
+
     @property
     def last_elapsed_ms(self) -> float:
         return float(self._L.fvs_last_elapsed_ms(self._h))

@@ -235,13 +235,20 @@ int fvs_freeze_limiter(fvs_ctx *ctx, int frozen);
 int fvs_batch_freeze_limiter(fvs_ctx *ctx, const int *frozen);
 /* Launch autotuner, no reference counterpart: times nsteps steps per trial
  * and keeps, per kernel class [gradient, face flux, gather+update, fused
- * flux+update], the faster of plain and XCD-aware workgroup numbering.  The
- * state is restored afterwards.  choice4 (may be NULL) receives the choices
- * (1 = XCD-aware; for the gradient, bit 1 set = the LSQ gradient, or a
- * limited Green-Gauss gradient's limiter, reads its stencil from a per-block
- * halo staged in LDS, the other candidate it tries), best_ms (may be NULL) the
- * best time per step. */
+ * flux+update], the faster of plain and XCD-aware workgroup numbering, and
+ * the faster of 256 and 128 cells per workgroup for the fused flux + update
+ * kernel (single meshes of at least 8,192 cells).  The state is restored
+ * afterwards.  choice4 (may be NULL) receives the choices (bit 0 set =
+ * XCD-aware; for the gradient, bit 1 set = the LSQ gradient, or a limited
+ * Green-Gauss gradient's limiter, reads its stencil from a per-block halo
+ * staged in LDS, the other candidate it tries; for the fused kernel, bit 2
+ * set = 128-cell workgroups), best_ms (may be NULL) the best time per step. */
 int fvs_autotune(fvs_ctx *ctx, int nsteps, int *choice4, double *best_ms);
+/* Applies launch choices in fvs_autotune's choice4 encoding (for instance
+ * choices an earlier fvs_autotune returned), without timing anything; a
+ * choice this context has no data for fails with FVS_EINVAL.  No reference
+ * counterpart. */
+int fvs_set_launch_choices(fvs_ctx *ctx, const int *choice4);
 /* wall-clock of the last fvs_time_integration call's device work (ms) */
 double fvs_last_elapsed_ms(const fvs_ctx *ctx);
 
