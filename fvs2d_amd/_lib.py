@@ -24,7 +24,7 @@ EXPORTS = (
     "fvs_ctx_mesh", "fvs_mesh_sizes", "fvs_mesh_topology", "fvs_mesh_lists", "fvs_mesh_lsq_nnz", "fvs_mesh_lsq",
     "fvs_set_state", "fvs_get_state", "fvs_initialize", "fvs_primitives", "fvs_gradient", "fvs_gradient_1var",
     "fvs_limiter", "fvs_compute_residual", "fvs_roe_flux_batch", "fvs_local_dt", "fvs_time_integration",
-    "fvs_vortex_error", "fvs_wall_forces", "fvs_set_timestep", "fvs_kernel_times",
+    "fvs_time_integration_noalloc", "fvs_vortex_error", "fvs_wall_forces", "fvs_set_timestep", "fvs_kernel_times",
     "fvs_autotune", "fvs_freeze_limiter", "fvs_batch_freeze_limiter", "fvs_wall_cp", "fvs_last_elapsed_ms",
     "fvs_set_phase_timers", "fvs_phase_times", "fvs_set_persistent", "fvs_set_launch_choices",
 )
@@ -92,6 +92,7 @@ def load():
     L.fvs_roe_flux_batch.argtypes = [I, D, V, V, V, V, V]
     L.fvs_local_dt.argtypes = [V, V]
     L.fvs_time_integration.argtypes = [V, D, I, V, V]
+    L.fvs_time_integration_noalloc.argtypes = [V, D, I, V, V]
     L.fvs_vortex_error.argtypes = [V, D, V]
     L.fvs_wall_forces.argtypes = [V, V]
     L.fvs_set_timestep.argtypes = [V, D, D]

@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <tuple>
 #include <string>
@@ -892,6 +893,36 @@ __global__ void k_fill(long n, double v, double *__restrict__ p) {
 
 }  // namespace
 
+// Staging of the per-step outputs of fvs_time_integration_noalloc (4 residual
+// norms and 15 vortex-error values per step, at most kNoAllocMaxSteps steps): a
+// file-scope device array, so that entry point allocates nothing.  Every GPU
+// holds its own copy at its own address until the process ends; host code
+// uses the address hipGetSymbolAddress gives for the current device
+// (scratch_address), never the array's name.
+constexpr int kNoAllocMaxSteps = 4096;
+constexpr size_t kScratchDoubles = size_t(kNoAllocMaxSteps) * (4 + kVerr);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+double *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// device address of `scratch` on device dev, which must be the current device
+int scratch_address(int dev, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxDevices) return fail(FVS_EHIP, "device index out of range");
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(FVS_EHIP, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
 struct fvs_mesh {
   MeshPrep mp;
 };
@@ -1030,8 +1061,9 @@ struct fvs_ctx {
   int nparts = 0;
   bool have_state = false;
   double last_ms = 0.0;
-  // the device the context was created on
+  // the device the context was created on, and the address of `scratch` there
   int device = 0;
+  double *scratch_dev = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // per-launch event pairs while profiling (kernel class, start, stop)
   bool profiling = false;
@@ -2176,6 +2208,7 @@ static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, in
   if (!rc0 && hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, C->device) != hipSuccess)
     rc0 = fail(FVS_EHIP, "hipDeviceGetAttribute failed");
   if (!rc0 && lmax > 0) C->lds_max = size_t(lmax);
+  if (!rc0) rc0 = scratch_address(C->device, &C->scratch_dev);
   if (rc0) {
     std::string e = g_err;
     delete C;
@@ -2785,6 +2818,34 @@ int fvs_time_integration(fvs_ctx *C, double t1, int nsteps, double *norms, doubl
     return integrate(C, t1, nsteps, nullptr, verr, nullptr);
   }
   return integrate(C, t1, nsteps, norms, verr, nullptr);
+}
+
+int fvs_time_integration_noalloc(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr) {
+  if (int rc = need_state(C)) return rc;
+  if (nsteps < 0) return fail(FVS_EINVAL, "nsteps < 0");
+  if (nsteps > kNoAllocMaxSteps)
+    return fail(FVS_EINVAL, "fvs_time_integration_noalloc: nsteps exceeds the " + std::to_string(kNoAllocMaxSteps) +
+                                " steps its device staging holds; call it in chunks");
+  if (nsteps == 0) return 0;
+  if (!C->scratch_dev) return fail(FVS_ESTATE, "no device staging address for this context");
+  if (C->nact && norms) {
+    std::fill(norms, norms + 4 * size_t(nsteps), std::nan(""));
+    norms = nullptr;
+  }
+  // staging in the device's `scratch`: norms of step s at [4 s, 4 s + 4),
+  // vortex errors after all the norms; every value is written by the steps
+  // before it is copied out, so nothing needs clearing between calls
+  double *dn = C->scratch_dev, *dv = verr ? C->scratch_dev + 4 * size_t(nsteps) : nullptr;
+  HIPC(hipEventRecord(C->ev0, C->stream));
+  if (int rc = run_steps(*C, t1, nsteps, dn, dv, nullptr, /*graph_ok=*/false)) return rc;
+  HIPC(hipEventRecord(C->ev1, C->stream));
+  if (norms) HIPC(hipMemcpyAsync(norms, dn, size_t(nsteps) * 4 * 8, hipMemcpyDeviceToHost, C->stream));
+  if (dv) HIPC(hipMemcpyAsync(verr, dv, size_t(nsteps) * kVerr * 8, hipMemcpyDeviceToHost, C->stream));
+  HIPC(hipStreamSynchronize(C->stream));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, C->ev0, C->ev1));
+  C->last_ms = ms;
+  return 0;
 }
 
 int fvs_set_phase_timers(fvs_ctx *C, int on) {

@@ -200,6 +200,17 @@ int fvs_local_dt(fvs_ctx *ctx, double *dt);
  * the cell with the largest density error (log_vortex_err_xy.plt,
  * src/mms.f90:363). */
 int fvs_time_integration(fvs_ctx *ctx, double t1, int nsteps, double *norms, double *verr);
+/* No counterpart in the reference: fvs_time_integration for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same steps and outputs; the
+ * per-step norms and vortex errors are staged in a fixed device array of the
+ * library (one per GPU, held until the process ends) instead of buffers the
+ * context grows, and steps are launched on the stream (never from a HIP
+ * graph), so the call allocates nothing.  nsteps is at most 4096 (call it in
+ * chunks); a larger nsteps fails with FVS_EINVAL.  Every context on one GPU
+ * shares that staging: calls that overlap on one GPU (several threads or
+ * contexts at once) race, and the caller must serialise them. */
+int fvs_time_integration_noalloc(fvs_ctx *ctx, double t1, int nsteps, double *norms, double *verr);
 /* error_isentropic_vortex (src/mms.f90:271-365) on the current state: out15
  * as one step's verr record of fvs_time_integration */
 int fvs_vortex_error(fvs_ctx *ctx, double t, double *out15);
