@@ -27,6 +27,7 @@ EXPORTS = (
     "fvs_time_integration_noalloc", "fvs_vortex_error", "fvs_wall_forces", "fvs_set_timestep", "fvs_kernel_times",
     "fvs_autotune", "fvs_freeze_limiter", "fvs_batch_freeze_limiter", "fvs_wall_cp", "fvs_last_elapsed_ms",
     "fvs_set_phase_timers", "fvs_phase_times", "fvs_set_persistent", "fvs_set_launch_choices",
+    "fvs_mesh_create_batch",
 )
 
 
@@ -71,6 +72,7 @@ def load():
     L.fvs_set_launch_choices.argtypes = [V, V]
     L.fvs_phase_times.argtypes = [V, V]
     L.fvs_mesh_create.argtypes = [C.POINTER(V), I, V, I, I, V, V, I, V, V, V, I, I, D, I]
+    L.fvs_mesh_create_batch.argtypes = [C.POINTER(V), I, V, I, V, I, I, V, I, V, V, V, I, I, D, I, I]
     for n in ("fvs_destroy", "fvs_mesh_destroy"):
         getattr(L, n).argtypes = [V]
     L.fvs_ctx_mesh.restype = V

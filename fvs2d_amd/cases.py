@@ -103,23 +103,24 @@ def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, fr
     _, sch = naca(16, 8, cfl=cfl)
     # a drawn grid that the mesh preparation rejects (the reference's LSQ
     # exactness check, src/gradient_lsq.f90:490-529, fails on the far-field
-    # slivers of a few thin 512x256 O-grids) is left out of the batch and listed
-    rejected, keep = [], []
-    for k, m in enumerate(meshes):
-        try:
-            HostMesh(m, grad_method=sch.grad_method, lsq_nn=sch.lsq_nn, lsq_p=sch.lsq_p, limiter=sch.limiter)
-            keep.append(k)
-        except FvsError as e:
-            rejected.append(dict(inst[k], index=k, reason=str(e)))
-    if rejected:
-        inst = [inst[k] for k in keep]
-        meshes = [meshes[k] for k in keep]
+    # slivers of a few thin 512x256 O-grids) is left out of the batch and
+    # listed; the batch preparation names every such instance at once
+    rejected = []
+    t0 = time.perf_counter()
+    try:
+        b = BatchSolver(meshes, [(d["mach"], d["aoa"]) for d in inst], sch)
+    except FvsError as e:
+        bad = set(HostMesh.failed_instances(e))
+        if not bad:
+            raise
+        rejected = [dict(inst[k], index=k, reason=str(e)) for k in sorted(bad)]
+        inst = [d for k, d in enumerate(inst) if k not in bad]
+        meshes = [m for k, m in enumerate(meshes) if k not in bad]
         ninst = len(inst)
         print(f"batch: {len(rejected)} drawn grid(s) rejected by the mesh preparation", file=sys.stderr, flush=True)
+        b = BatchSolver(meshes, [(d["mach"], d["aoa"]) for d in inst], sch)
     stall_of = lambda d: stall_rule_for(d, freeze_stall, stall_scale)  # noqa: E731
     batch_stall = None if freeze_stall is None else ([stall_of(d)[0] for d in inst], freeze_stall[1])
-    t0 = time.perf_counter()
-    b = BatchSolver(meshes, [(d["mach"], d["aoa"]) for d in inst], sch)
     b.set_graph(graph)
     b.initialize_freestream()
     setup = time.perf_counter() - t0

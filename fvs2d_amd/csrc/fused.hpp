@@ -159,7 +159,10 @@ __global__ __launch_bounds__(kBlock) void k_grad_lsqc(int N, int nb, const doubl
   }
 }
 
-// interior face flux with L = c1, R = c2 (shared by both flux kernels)
+// interior face flux with L = c1, R = c2 (shared by every flux kernel).
+// The reconstruction (src/residual.f90:87-92) is written with explicit FMAs
+// so that it rounds the same in every kernel it is inlined into, whatever
+// the compiler's contraction choices there.
 __device__ __forceinline__ void interior_flux_v(const Phys &P, const double g[8], const double wL[4],
                                                 const double wR[4], const double gL[8], const double gR[8],
                                                 double phL, double phR, double fa[4], double &wsa) {
@@ -167,10 +170,10 @@ __device__ __forceinline__ void interior_flux_v(const Phys &P, const double g[8]
   double pL[4], pR[4];
   for (int v = 0; v < 4; ++v) {
     const double gC = wR[v] - wL[v];
-    const double dL = g[3] * gL[v] + g[4] * gL[4 + v];
-    const double dR = g[5] * gR[v] + g[6] * gR[4 + v];
-    pL[v] = wL[v] + phL * (k2 * gC + k1 * dL);
-    pR[v] = wR[v] + phR * (-(k2 * gC) + k1 * dR);
+    const double dL = fma(g[3], gL[v], g[4] * gL[4 + v]);
+    const double dR = fma(g[5], gR[v], g[6] * gR[4 + v]);
+    pL[v] = fma(phL, fma(k2, gC, k1 * dL), wL[v]);
+    pR[v] = fma(phR, fma(-k2, gC, k1 * dR), wR[v]);
   }
   double fl[4], ws;
   roe_flux(P.gamma, pL, pR, g[0], g[1], fl, ws);
@@ -194,7 +197,7 @@ __device__ __forceinline__ void boundary_flux_v(const Phys &P, double t, int ty,
                                                 const double gr[8], double ph, double fa[4], double &wsa,
                                                 const double *__restrict__ pinf) {
   double pL[4], pR[4];
-  for (int v = 0; v < 4; ++v) pL[v] = w[v] + ph * (g[3] * gr[v] + g[4] * gr[4 + v]);
+  for (int v = 0; v < 4; ++v) pL[v] = fma(ph, fma(g[3], gr[v], g[4] * gr[4 + v]), w[v]);
   const double nx = g[0], ny = g[1];
   if (ty == FVS_BC_FREESTREAM) {
     for (int v = 0; v < 4; ++v) pR[v] = pinf[v];

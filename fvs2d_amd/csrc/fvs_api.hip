@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <numeric>
 #include <tuple>
 #include <string>
@@ -513,10 +514,10 @@ __global__ __launch_bounds__(kBlock) void k_flux_int(int nbx, int Fi, Phys P, co
   double pL[4], pR[4];
   for (int v = 0; v < 4; ++v) {
     const double gC = wR[v] - wL[v];
-    const double dL = g[3] * gL[v] + g[4] * gL[4 + v];
-    const double dR = g[5] * gR[v] + g[6] * gR[4 + v];
-    pL[v] = wL[v] + phL * (k2 * gC + k1 * dL);
-    pR[v] = wR[v] + phR * (-(k2 * gC) + k1 * dR);
+    const double dL = fma(g[3], gL[v], g[4] * gL[4 + v]);  // the FMAs of interior_flux_v
+    const double dR = fma(g[5], gR[v], g[6] * gR[4 + v]);
+    pL[v] = fma(phL, fma(k2, gC, k1 * dL), wL[v]);
+    pR[v] = fma(phR, fma(-k2, gC, k1 * dR), wR[v]);
   }
   double fl[4], ws;
   roe_flux(P.gamma, pL, pR, g[0], g[1], fl, ws);
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_bnd(int Fb, int Fi, Phys P, dou
   ld8(grad, c, gr);
   const double ph = phi[c];
   double pL[4], pR[4];
-  for (int v = 0; v < 4; ++v) pL[v] = w[v] + ph * (g[3] * gr[v] + g[4] * gr[4 + v]);
+  for (int v = 0; v < 4; ++v) pL[v] = fma(ph, fma(g[3], gr[v], g[4] * gr[4 + v]), w[v]);
   const int ty = btype[b];
   const double nx = g[0], ny = g[1];
   if (ty == FVS_BC_FREESTREAM) {
@@ -2136,6 +2137,13 @@ namespace fvs {
 int set_error(int code, const std::string &msg) { return fail(code, msg); }
 }  // namespace fvs
 
+// worker threads for host-side preparation: the cores this process may use,
+// at most 16
+static int host_threads() {
+  const unsigned n = std::thread::hardware_concurrency();
+  return int(std::max(1u, std::min(16u, n)));
+}
+
 static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
                        const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types,
                        const int *bnd_cells, const fvs_options *opt, int ninst, const int *cell_offsets,
@@ -2175,8 +2183,13 @@ static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, in
   fl.lsq_nn = opt->lsq_nn;
   fl.lsq_p = opt->lsq_p;
   fl.limiter = opt->limiter;
-  std::string err = build_mesh(C->mp, nnodes, xy, ntri, nquad, cell_nodes, node_canon, nb, bnd_ncells, bnd_types,
-                               bnd_cells, fl);
+  // a batch of cells of one kind is prepared instance by instance on the
+  // host's cores
+  std::string err = (ninst > 0 && (ntri == 0 || nquad == 0))
+                        ? build_mesh_batch(C->mp, ninst, cell_offsets, nnodes, xy, ntri, nquad, cell_nodes, nb,
+                                           bnd_ncells, bnd_types, bnd_cells, fl, host_threads())
+                        : build_mesh(C->mp, nnodes, xy, ntri, nquad, cell_nodes, node_canon, nb, bnd_ncells,
+                                     bnd_types, bnd_cells, fl);
   if (!err.empty()) {
     delete C;
     return fail(FVS_EINVAL, err);
@@ -2269,6 +2282,34 @@ int fvs_mesh_create(fvs_mesh **out, int nnodes, const double *xy, int ntri, int 
   auto *M = new fvs_mesh();
   std::string err = build_mesh(M->mp, nnodes, xy, ntri, nquad, cell_nodes, node_canon, nb, bnd_ncells, bnd_types,
                                bnd_cells, fl);
+  if (!err.empty()) {
+    delete M;
+    return fail(FVS_EINVAL, err);
+  }
+  *out = M;
+  return 0;
+}
+
+int fvs_mesh_create_batch(fvs_mesh **out, int ninst, const int *cell_offsets, int nnodes, const double *xy,
+                          int ntri, int nquad, const int *cell_nodes, int nb, const int *bnd_ncells,
+                          const int *bnd_types, const int *bnd_cells, int grad_method, int lsq_nn, double lsq_p,
+                          int limiter, int nthreads) {
+  if (!out || !xy || !cell_nodes || !cell_offsets || (nb > 0 && (!bnd_ncells || !bnd_types || !bnd_cells)))
+    return fail(FVS_EINVAL, "fvs_mesh_create_batch: null argument");
+  *out = nullptr;
+  if (ninst < 1 || cell_offsets[0] != 0 || cell_offsets[ninst] != ntri + nquad)
+    return fail(FVS_EINVAL, "fvs_mesh_create_batch: cell_offsets must run from 0 to ncells");
+  for (int k = 0; k < ninst; ++k)
+    if (cell_offsets[k + 1] <= cell_offsets[k])
+      return fail(FVS_EINVAL, "fvs_mesh_create_batch: cell_offsets must increase (every instance has cells)");
+  SchemeFlags fl;
+  fl.grad_method = grad_method;
+  fl.lsq_nn = lsq_nn;
+  fl.lsq_p = lsq_p;
+  fl.limiter = limiter;
+  auto *M = new fvs_mesh();
+  std::string err = build_mesh_batch(M->mp, ninst, cell_offsets, nnodes, xy, ntri, nquad, cell_nodes, nb, bnd_ncells,
+                                     bnd_types, bnd_cells, fl, nthreads > 0 ? nthreads : host_threads());
   if (!err.empty()) {
     delete M;
     return fail(FVS_EINVAL, err);

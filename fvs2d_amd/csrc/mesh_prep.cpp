@@ -14,9 +14,12 @@
 #include "mesh_prep.hpp"
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <thread>
 #include <unordered_map>
 
 namespace fvs {
@@ -98,32 +101,35 @@ std::string topology(MeshPrep &m) {
     for (int c = 0; c < N; ++c)
       for (int k = 0; k < m.nvrt[c]; ++k) m.n2c[fill[m.cnodec[4 * c + k]]++] = c;
   }
-  // neighbours across local edges from a directed-edge hash
-  std::unordered_map<uint64_t, int64_t> dir;
-  dir.reserve(size_t(N) * 4);
+  // neighbours across local edges: every directed edge (a, b) of every
+  // cell, sorted by its node pair; the neighbour across (a, b) owns (b, a)
   auto key = [](int a, int b) { return (uint64_t(uint32_t(a)) << 32) | uint32_t(b); };
+  std::vector<std::pair<uint64_t, int64_t>> dir;
+  dir.reserve(size_t(N) * 4);
   for (int c = 0; c < N; ++c) {
     int nv = m.nvrt[c];
-    for (int e = 0; e < nv; ++e) {
-      int a = m.cnodec[4 * c + e], b = m.cnodec[4 * c + (e + 1) % nv];
-      if (!dir.emplace(key(a, b), int64_t(c) * 4 + e).second)
-        return fmt("directed edge (%ld,%ld) appears twice: mesh is not a consistently oriented manifold", a, b);
-    }
+    for (int e = 0; e < nv; ++e)
+      dir.emplace_back(key(m.cnodec[4 * c + e], m.cnodec[4 * c + (e + 1) % nv]), int64_t(c) * 4 + e);
   }
+  std::sort(dir.begin(), dir.end());
+  for (size_t q = 1; q < dir.size(); ++q)
+    if (dir[q].first == dir[q - 1].first)
+      return fmt("directed edge (%ld,%ld) appears twice: mesh is not a consistently oriented manifold",
+                 long(dir[q].first >> 32), long(dir[q].first & 0xffffffffu));
   m.nghbre.assign(4 * N, -1);
   std::vector<int> nb_local(4 * N, -1);  // local edge index in the neighbour
   for (int c = 0; c < N; ++c) {
     int nv = m.nvrt[c];
     for (int e = 0; e < nv; ++e) {
-      int a = m.cnodec[4 * c + e], b = m.cnodec[4 * c + (e + 1) % nv];
-      auto it = dir.find(key(b, a));
-      if (it != dir.end()) {
+      const uint64_t rk = key(m.cnodec[4 * c + (e + 1) % nv], m.cnodec[4 * c + e]);
+      auto it = std::lower_bound(dir.begin(), dir.end(), std::make_pair(rk, int64_t(-1)));
+      if (it != dir.end() && it->first == rk) {
         m.nghbre[4 * c + e] = int(it->second / 4);
         nb_local[4 * c + e] = int(it->second % 4);
       }
     }
   }
-  dir.clear();
+  std::vector<std::pair<uint64_t, int64_t>>().swap(dir);
   // edge numbering: src/grid_procs.f90:401-624
   int ne = 0;
   m.cedge.assign(4 * N, -1);
@@ -313,8 +319,9 @@ std::string lsq_stencil(MeshPrep &m, bool nn, double p) {
   } else {
     // setup_nn (src/gradient_lsq.f90:227-271): cells sharing a vertex,
     // ascending and unique
-    std::vector<std::vector<int>> st(N);
     std::vector<int> tmp;
+    m.lsq_cell.clear();
+    m.lsq_cell.reserve(size_t(N) * 13);
     for (int c = 0; c < N; ++c) {
       tmp.clear();
       for (int k = 0; k < m.nvrt[c]; ++k) {
@@ -324,11 +331,9 @@ std::string lsq_stencil(MeshPrep &m, bool nn, double p) {
       }
       std::sort(tmp.begin(), tmp.end());
       tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-      st[c] = tmp;
+      m.lsq_cell.insert(m.lsq_cell.end(), tmp.begin(), tmp.end());
       m.lsq_ptr[c + 1] = m.lsq_ptr[c] + int(tmp.size());
     }
-    m.lsq_cell.resize(m.lsq_ptr[N]);
-    for (int c = 0; c < N; ++c) std::copy(st[c].begin(), st[c].end(), m.lsq_cell.begin() + m.lsq_ptr[c]);
   }
   // weights w = 1/d^p and coef = G^-1 D^T, D rows w*(dx,dy):
   // src/gradient_lsq.f90:137-201 and 277-348 (same arithmetic in both)
@@ -521,6 +526,238 @@ std::string build_mesh(MeshPrep &m, int nnodes, const double *xy, int ntri, int 
   }
   if (flags.grad_method == 1) ggcb_setup(m);
   if (flags.grad_method == 2) ggnb_setup(m);
+  return "";
+}
+
+namespace {
+
+// append src to dst, adding `add` to every entry that is >= 0 (ids; -1 stays)
+void append_ids(std::vector<int> &dst, const std::vector<int> &src, int add) {
+  const size_t o = dst.size();
+  dst.resize(o + src.size());
+  for (size_t k = 0; k < src.size(); ++k) dst[o + k] = src[k] >= 0 ? src[k] + add : src[k];
+}
+template <class T>
+void append(std::vector<T> &dst, const std::vector<T> &src) {
+  dst.insert(dst.end(), src.begin(), src.end());
+}
+// append a CSR pointer array (src[0] = 0) continuing dst's last value
+void append_ptr(std::vector<int> &dst, const std::vector<int> &src) {
+  if (src.empty()) return;
+  const int base = dst.empty() ? 0 : dst.back();
+  if (dst.empty()) dst.push_back(0);
+  for (size_t k = 1; k < src.size(); ++k) dst.push_back(src[k] + base);
+}
+
+}  // namespace
+
+std::string build_mesh_batch(MeshPrep &m, int ninst, const int *cell_offsets, int nnodes, const double *xy, int ntri,
+                             int nquad, const int *cell_nodes, int nb, const int *bnd_ncells, const int *bnd_types,
+                             const int *bnd_cells, const SchemeFlags &flags, int nthreads) {
+  const int N = ntri + nquad;
+  if (ninst <= 0 || N <= 0 || nnodes <= 0) return "empty batch";
+  if (ntri > 0 && nquad > 0) return "per-instance preparation needs cells of one kind";
+  if (nb < 0) return "negative boundary count";
+  const int nv = ntri > 0 ? 3 : 4;
+  // node range of every instance; instances own consecutive ranges
+  std::vector<int> nlo(ninst), nhi(ninst);
+  for (int k = 0; k < ninst; ++k) {
+    int lo = nnodes, hi = -1;
+    for (long q = long(cell_offsets[k]) * nv; q < long(cell_offsets[k + 1]) * nv; ++q) {
+      const int v = cell_nodes[q];
+      if (v < 0 || v >= nnodes) return fmt("cell %ld references node %ld outside the mesh", q / nv, v);
+      lo = std::min(lo, v);
+      hi = std::max(hi, v);
+    }
+    nlo[k] = lo;
+    nhi[k] = hi;
+    if (lo != (k == 0 ? 0 : nhi[k - 1] + 1)) return "instances do not own consecutive node ranges";
+  }
+  if (nhi[ninst - 1] != nnodes - 1) return "instances do not own consecutive node ranges";
+  // every boundary's cells, split by instance: a boundary may span several
+  // instances (same-named boundaries concatenated), its cells listed in
+  // instance order; instance k gets the boundaries it has cells on, in order
+  std::vector<int> bptr(nb + 1, 0);
+  for (int b = 0; b < nb; ++b) {
+    if (bnd_ncells[b] < 0) return "negative boundary cell count";
+    if (bnd_types[b] < 0 || bnd_types[b] > 2)
+      return fmt("boundary %ld: type %ld not implemented (freestream, slip_wall, dirichlet)", b + 1, bnd_types[b]);
+    bptr[b + 1] = bptr[b] + bnd_ncells[b];
+  }
+  // sub[b]: for each instance with cells on boundary b, (instance, first,
+  // end) positions in the boundary's list
+  std::vector<std::vector<std::array<int, 3>>> sub(nb);
+  std::vector<std::vector<int>> inst_b(ninst);  // boundaries of instance k
+  for (int b = 0; b < nb; ++b) {
+    int prev = -1;
+    for (int i = bptr[b]; i < bptr[b + 1]; ++i) {
+      const int c = bnd_cells[i];
+      if (c < 0 || c >= N) return fmt("boundary %ld lists cell %ld outside the mesh", b + 1, c);
+      const int k = int(std::upper_bound(cell_offsets, cell_offsets + ninst + 1, c) - cell_offsets) - 1;
+      if (k < prev) return fmt("boundary %ld does not list its cells instance by instance", b + 1, 0);
+      if (k != prev) {
+        sub[b].push_back({k, i, i});
+        inst_b[k].push_back(b);
+        prev = k;
+      }
+      sub[b].back()[2] = i + 1;
+    }
+  }
+  // every instance on its own, on nthreads workers
+  std::vector<MeshPrep> part(ninst);
+  std::vector<std::string> errs(ninst);
+  std::atomic<int> next(0);
+  auto work = [&]() {
+    for (int k = next++; k < ninst; k = next++) {
+      const int c0 = cell_offsets[k], nc = cell_offsets[k + 1] - c0, v0 = nlo[k], nn = nhi[k] - v0 + 1;
+      std::vector<int> cn(size_t(nc) * nv);
+      for (size_t q = 0; q < cn.size(); ++q) cn[q] = cell_nodes[size_t(c0) * nv + q] - v0;
+      const int nbk = int(inst_b[k].size());
+      std::vector<int> bn(nbk), bt(nbk), bc;
+      for (int q = 0; q < nbk; ++q) {
+        const int b = inst_b[k][q];
+        for (const auto &r : sub[b])
+          if (r[0] == k) {
+            bn[q] = r[2] - r[1];
+            for (int i = r[1]; i < r[2]; ++i) bc.push_back(bnd_cells[i] - c0);
+          }
+        bt[q] = bnd_types[b];
+      }
+      errs[k] = build_mesh(part[k], nn, xy + 2 * size_t(v0), ntri > 0 ? nc : 0, ntri > 0 ? 0 : nc, cn.data(),
+                           nullptr, nbk, bn.data(), bt.data(), bc.data(), flags);
+    }
+  };
+  const int nt = std::max(1, std::min(nthreads, ninst));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  // every failing instance is named (callers may drop them all at once)
+  {
+    std::string bad, first;
+    for (int k = 0; k < ninst; ++k)
+      if (!errs[k].empty()) {
+        bad += (bad.empty() ? "" : ",") + std::to_string(k);
+        if (first.empty()) first = errs[k];
+      }
+    if (!bad.empty()) return "instances [" + bad + "] failed; the first: " + first;
+  }
+  // merge in instance order: cells, nodes and edges of instance k follow
+  // those of instance k - 1, the numbering a whole-concatenation build gives
+  m = MeshPrep();
+  m.nnodes = nnodes;
+  m.ntri = ntri;
+  m.nquad = nquad;
+  m.ncells = N;
+  m.nb = nb;
+  std::vector<int> cofs(ninst + 1, 0), vofs(ninst + 1, 0), eofs(ninst + 1, 0);
+  for (int k = 0; k < ninst; ++k) {
+    cofs[k + 1] = cofs[k] + part[k].ncells;
+    vofs[k + 1] = vofs[k] + part[k].nnodes;
+    eofs[k + 1] = eofs[k] + part[k].nedges;
+  }
+  // boundaries: each one's cells and edges, instance by instance (the order
+  // a whole-concatenation build walks its listed cells in)
+  m.bnd_type.assign(bnd_types, bnd_types + nb);
+  m.bnd_ncells.assign(nb, 0);
+  m.bnd_nedges.assign(nb, 0);
+  m.bnd_cptr.assign(1, 0);
+  m.bnd_eptr.assign(1, 0);
+  for (int b = 0; b < nb; ++b) {
+    for (const auto &r : sub[b]) {
+      const int k = r[0];
+      const MeshPrep &p = part[k];
+      const int q = int(std::find(inst_b[k].begin(), inst_b[k].end(), b) - inst_b[k].begin());
+      for (int i = p.bnd_cptr[q]; i < p.bnd_cptr[q + 1]; ++i) m.bnd_cells.push_back(p.bnd_cells[i] + cofs[k]);
+      for (int i = p.bnd_eptr[q]; i < p.bnd_eptr[q + 1]; ++i) m.bnd_edges.push_back(p.bnd_edges[i] + eofs[k]);
+      m.bnd_ncells[b] += p.bnd_ncells[q];
+      m.bnd_nedges[b] += p.bnd_nedges[q];
+    }
+    m.bnd_cptr.push_back(int(m.bnd_cells.size()));
+    m.bnd_eptr.push_back(int(m.bnd_edges.size()));
+  }
+  // every merged array sized once (the parts are released as they are merged)
+  {
+    size_t tot[12] = {0};
+    for (const MeshPrep &p : part) {
+      tot[0] += p.nxy.size();
+      tot[1] += p.canon.size();
+      tot[2] += p.nvrt.size();
+      tot[3] += p.cnode.size();
+      tot[4] += p.n2c.size();
+      tot[5] += p.en1.size();
+      tot[6] += p.edge_intr.size();
+      tot[7] += p.cell_intr.size();
+      tot[8] += p.lsq_cell.size();
+      tot[9] += p.ggcb_ptr.size();
+      tot[10] += p.ggnb_ncoef.size();
+      tot[11] += p.ggnb_edg.size();
+    }
+    m.nxy.reserve(tot[0]);
+    m.canon.reserve(tot[1]);
+    for (auto *v : {&m.nvrt}) v->reserve(tot[2]);
+    for (auto *v : {&m.cnode, &m.cnodec, &m.nghbre, &m.cedge}) v->reserve(tot[3]);
+    for (auto *v : {&m.cx, &m.cy, &m.vol}) v->reserve(tot[2]);
+    m.nrmlsign.reserve(tot[3]);
+    m.n2c_ptr.reserve(size_t(nnodes) + 1);
+    m.n2c.reserve(tot[4]);
+    for (auto *v : {&m.en1, &m.en2, &m.ec1, &m.ec2}) v->reserve(tot[5]);
+    for (auto *v : {&m.ex, &m.ey, &m.ea, &m.enx, &m.eny, &m.esx, &m.esy}) v->reserve(tot[5]);
+    m.edge_intr.reserve(tot[6]);
+    m.cell_intr.reserve(tot[7]);
+    m.lsq_ptr.reserve(size_t(N) + 1);
+    m.lsq_cell.reserve(tot[8]);
+    m.lsq_w.reserve(tot[8]);
+    m.lsq_coef.reserve(2 * tot[8]);
+    m.ggcb_ptr.reserve(tot[9]);
+    m.ggcb_coef0.reserve(tot[9] / 2);
+    m.ggcb_coefnb.reserve(2 * tot[9]);
+    m.ggnb_ncoef.reserve(tot[10]);
+    m.ggnb_edg.reserve(tot[11]);
+  }
+  for (int k = 0; k < ninst; ++k) {
+    MeshPrep &p = part[k];
+    const int co = cofs[k], vo = vofs[k], eo = eofs[k];
+    append(m.nxy, p.nxy);
+    append_ids(m.canon, p.canon, vo);
+    append(m.nvrt, p.nvrt);
+    append_ids(m.cnode, p.cnode, vo);
+    append_ids(m.cnodec, p.cnodec, vo);
+    append_ids(m.nghbre, p.nghbre, co);
+    append_ids(m.cedge, p.cedge, eo);
+    append(m.cx, p.cx);
+    append(m.cy, p.cy);
+    append(m.vol, p.vol);
+    append(m.nrmlsign, p.nrmlsign);
+    append_ptr(m.n2c_ptr, p.n2c_ptr);
+    append_ids(m.n2c, p.n2c, co);
+    append_ids(m.en1, p.en1, vo);
+    append_ids(m.en2, p.en2, vo);
+    append_ids(m.ec1, p.ec1, co);
+    append_ids(m.ec2, p.ec2, co);
+    append(m.ex, p.ex);
+    append(m.ey, p.ey);
+    append(m.ea, p.ea);
+    append(m.enx, p.enx);
+    append(m.eny, p.eny);
+    append(m.esx, p.esx);
+    append(m.esy, p.esy);
+    append_ids(m.edge_intr, p.edge_intr, eo);
+    append_ids(m.cell_intr, p.cell_intr, co);
+    m.ncells_bndr += p.ncells_bndr;
+    m.nedges_bndr += p.nedges_bndr;
+    append_ptr(m.lsq_ptr, p.lsq_ptr);
+    append_ids(m.lsq_cell, p.lsq_cell, co);
+    append(m.lsq_w, p.lsq_w);
+    append(m.lsq_coef, p.lsq_coef);
+    append_ids(m.ggcb_ptr, p.ggcb_ptr, co);
+    append(m.ggcb_coef0, p.ggcb_coef0);
+    append(m.ggcb_coefnb, p.ggcb_coefnb);
+    append(m.ggnb_ncoef, p.ggnb_ncoef);
+    append(m.ggnb_edg, p.ggnb_edg);
+    p = MeshPrep();  // release as we go
+  }
+  m.nedges = eofs[ninst];
   return "";
 }
 

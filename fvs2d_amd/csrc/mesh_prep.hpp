@@ -52,6 +52,17 @@ std::string build_mesh(MeshPrep &m, int nnodes, const double *xy, int ntri, int 
                        const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types,
                        const int *bnd_cells, const SchemeFlags &flags);
 
+// The same for a batch of ninst independent instances concatenated in
+// instance order (instance k owns cells [cell_offsets[k], cell_offsets[k+1]),
+// a consecutive node range and consecutive boundaries, cells of one kind):
+// each instance is prepared on its own, on up to nthreads threads, and the
+// results are merged with node, cell and edge offsets, which is the numbering
+// the whole-concatenation build gives (stencil searches stay inside an
+// instance).  Returns "" or an error message.
+std::string build_mesh_batch(MeshPrep &m, int ninst, const int *cell_offsets, int nnodes, const double *xy, int ntri,
+                             int nquad, const int *cell_nodes, int nb, const int *bnd_ncells, const int *bnd_types,
+                             const int *bnd_cells, const SchemeFlags &flags, int nthreads);
+
 // Frame helpers for periodic meshes (no reference counterpart): the centroid
 // of edge `je`'s cell `c` expressed in the edge's c1 frame.
 inline void side_centroid(const MeshPrep &m, int je, int c, double &x, double &y) {

@@ -157,6 +157,33 @@ class HostMesh:
         self._L = L
         self._load_sizes()
 
+    @staticmethod
+    def failed_instances(err: Exception):
+        """Indices of the instances a batch preparation rejected, from its
+        error message (empty when the error names none)."""
+        import re
+        m = re.search(r"instances \[([0-9,]+)\] failed", str(err))
+        return [int(v) for v in m.group(1).split(",")] if m else []
+
+    @classmethod
+    def batch(cls, meshes, grad_method=3, lsq_nn=1, lsq_p=0.0, limiter=0, nthreads=0):
+        """Host preparation of a batch of meshes, instance by instance on
+        nthreads threads (fvs_mesh_create_batch), numbered as the
+        concatenation of the meshes (meshgen.concat)."""
+        from .meshgen import concat
+        mesh, coff, _ = concat(meshes)
+        L = _lib.load()
+        xy, cn, nb, bt, bc, _ = _mesh_arrays(mesh)
+        off = np.ascontiguousarray(coff, np.int32)
+        h = C.c_void_p()
+        check(L.fvs_mesh_create_batch(C.byref(h), len(meshes), ptr(off), mesh.nnodes, ptr(xy), mesh.ntri, mesh.nquad,
+                                      ptr(cn), len(nb), ptr(nb), ptr(bt), ptr(bc), grad_method, lsq_nn, lsq_p,
+                                      limiter, nthreads))
+        obj = cls.__new__(cls)
+        obj._h, obj._owned, obj._L = h, True, L
+        obj._load_sizes()
+        return obj
+
     @classmethod
     def _view(cls, L, h):
         obj = cls.__new__(cls)
@@ -526,7 +553,8 @@ class BatchSolver(Solver):
     def steady_solve_batch(self, max_steps: int, drop: float = 1e-8, chunk: int = 200,
                            freeze_after: Optional[int] = None, freeze_drop: Optional[float] = None,
                            r_ref=None, diverge: Optional[float] = None, on_diverge=None,
-                           freeze_stall: Optional[Tuple[int, float]] = None, progress=None):
+                           freeze_stall: Optional[Tuple[int, float]] = None, progress=None,
+                           max_active: Optional[int] = None):
         """Pseudo-time stepping of all instances, each until its density
         residual norm is at most ``drop * r_ref[k]`` (``r_ref`` defaults to
         each instance's first-step norm); converged instances are masked out
@@ -541,15 +569,30 @@ class BatchSolver(Solver):
         on, otherwise it is masked out.  Returns per-instance iterations (-1
         when not converged), steps run, divergence and restart flags, and the
         step at which each limiter was frozen (-1: never).  ``progress(steps,
-        active)`` (optional) is called after every chunk."""
+        active)`` (optional) is called after every chunk.
+
+        With ``max_active`` at most that many instances step at once: the
+        others wait, in index order, with their initial state (masked blocks
+        are not launched, so they cost nothing), and one starts at each chunk
+        boundary for each instance that has finished (converged or given up).
+        An instance's iterations, freezing and divergence count from its
+        start; ``max_steps`` bounds each instance's own steps."""
         n = self.ninst
+        cap = n if max_active is None else max(1, min(int(max_active), n))
         if freeze_stall is not None:
             win = np.broadcast_to(np.asarray(freeze_stall[0], np.int64), (n,))
 
             def stall_rule(k):
                 return int(win[k]), freeze_stall[1]
-        self.set_active(np.ones(n, np.int32))
+        act0 = np.zeros(n, np.int32)
+        act0[:cap] = 1
+        self.set_active(act0)
+        queued = list(range(cap, n))
+        started = np.full(n, -1, np.int64)
+        started[:cap] = 0
         iters = np.full(n, -1, np.int64)
+        if r_ref is None and cap < n:
+            raise ValueError("max_active below the instance count needs r_ref (each instance's reference norm)")
         r_ref = None if r_ref is None else np.asarray(r_ref, np.float64)
         done = 0
         since = np.zeros(n, np.int64)
@@ -559,8 +602,22 @@ class BatchSolver(Solver):
         restarted = np.zeros(n, bool)
         ends = [[] for _ in range(n)]
         self.freeze_limiter_batch(frozen)
-        while done < max_steps and np.any(self.active):
-            m = min(chunk, max_steps - done)
+        while np.any(self.active):
+            m = chunk
+            own = [max_steps - (done - started[k]) for k in np.nonzero(self.active)[0]]
+            if min(own) <= 0:
+                for k in np.nonzero(self.active)[0]:
+                    if done - started[k] >= max_steps:
+                        self.active[k] = 0
+                while queued and int(np.sum(self.active)) < cap:
+                    k = queued.pop(0)
+                    self.active[k] = 1
+                    started[k] = done
+                if not np.any(self.active):
+                    break
+                self.set_active(self.active)
+                continue
+            m = min(m, min(own))
             if freeze_after is not None:
                 left = [freeze_after - since[k] for k in np.nonzero(self.active & (frozen == 0))[0]]
                 left = [v for v in left if v > 0]
@@ -584,7 +641,7 @@ class BatchSolver(Solver):
                     continue
                 hit = np.nonzero(nr[:, k] <= drop * r_ref[k])[0]
                 if hit.size:
-                    iters[k] = done + int(hit[0]) + 1
+                    iters[k] = done - started[k] + int(hit[0]) + 1
                     self.active[k] = 0
                     continue
                 since[k] += m
@@ -593,7 +650,7 @@ class BatchSolver(Solver):
                                       or (freeze_after is not None and since[k] >= freeze_after)
                                       or (freeze_stall is not None and _stalled(ends[k], stall_rule(k)))):
                     frozen[k] = 1
-                    frozen_at[k] = done + m
+                    frozen_at[k] = done - started[k] + m
             for k in reset:
                 since[k] = 0
                 frozen[k] = 0
@@ -601,6 +658,10 @@ class BatchSolver(Solver):
                 ends[k] = []
             done += m
             self.freeze_limiter_batch(frozen)
+            while queued and int(np.sum(self.active)) < cap:
+                k = queued.pop(0)
+                self.active[k] = 1
+                started[k] = done
             if np.any(self.active):
                 self.set_active(self.active)
             if progress is not None:

@@ -146,6 +146,18 @@ typedef struct fvs_mesh fvs_mesh;
 int fvs_mesh_create(fvs_mesh **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
                     const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types,
                     const int *bnd_cells, int grad_method, int lsq_nn, double lsq_p, int limiter);
+/* The same for a batch of ninst instances concatenated as fvs_create_batch
+ * takes them (cells of one kind; each instance owning a consecutive node range
+ * and consecutive boundaries): every instance is prepared on its own, on
+ * nthreads host threads (0: the cores available, at most 16), and the results
+ * merged -- the numbering the preparation of the whole concatenation gives,
+ * with stencil searches kept inside each instance.  fvs_create_batch prepares
+ * its meshes this way.  No reference counterpart (the reference prepares one
+ * mesh per run, src/grid_procs.f90:170-794). */
+int fvs_mesh_create_batch(fvs_mesh **out, int ninst, const int *cell_offsets, int nnodes, const double *xy,
+                          int ntri, int nquad, const int *cell_nodes, int nb, const int *bnd_ncells,
+                          const int *bnd_types, const int *bnd_cells, int grad_method, int lsq_nn, double lsq_p,
+                          int limiter, int nthreads);
 int fvs_mesh_destroy(fvs_mesh *mesh);
 /* the mesh a solver context was built on (owned by the context) */
 const fvs_mesh *fvs_ctx_mesh(const fvs_ctx *ctx);

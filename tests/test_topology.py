@@ -128,3 +128,38 @@ def test_bad_meshes_rejected():
     bad_type = meshgen.Mesh(m.xy, m.ntri, 0, m.cell_nodes, [("solid_wall", m.boundaries[0][1])])
     with pytest.raises(FvsError, match="not implemented"):
         HostMesh(bad_type)
+
+
+@pytest.mark.parametrize("gm,nn,lim", [(3, 1, 1), (1, 1, 1), (2, 1, 0)])
+def test_batch_preparation_equals_whole_concatenation(gm, nn, lim):
+    """Preparing a batch instance by instance on several threads
+    (fvs_mesh_create_batch) gives the topology, lists and stencils of
+    preparing the whole concatenation at once, bit for bit."""
+    from fvs2d_amd.solver import HostMesh
+    meshes = [meshgen.naca_ogrid(32, 16, thickness=0.12), meshgen.jittered_square(6, seed=3),
+              meshgen.naca_ogrid(40, 20, thickness=0.1), meshgen.naca_ogrid(24, 12, thickness=0.15)]
+    big, off, _ = meshgen.concat(meshes)
+    whole = HostMesh(big, grad_method=gm, lsq_nn=nn, limiter=lim)
+    part = HostMesh.batch(meshes, grad_method=gm, lsq_nn=nn, limiter=lim, nthreads=3)
+    for get in ("edges", "cells", "lists", "lsq"):
+        if get == "lsq" and gm != 3 and lim == 0:
+            continue
+        a, b = getattr(whole, get)(), getattr(part, get)()
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (get, k)
+
+
+def test_batch_preparation_names_every_failing_instance():
+    """A batch whose instances 1 and 3 each hold a clockwise cell fails with
+    an error naming both instances, so a caller can drop them in one pass."""
+    from fvs2d_amd.solver import HostMesh
+    from fvs2d_amd._lib import FvsError
+    meshes = [meshgen.naca_ogrid(16, 8, thickness=0.12) for _ in range(4)]
+    for k in (1, 3):
+        cn = meshes[k].cell_nodes.copy()
+        cn[15], cn[16] = cn[16], cn[15]          # cell 5 runs clockwise
+        meshes[k].cell_nodes = cn
+    with pytest.raises(FvsError, match=r"instances \[1,3\] failed") as e:
+        HostMesh.batch(meshes, nthreads=2)
+    assert HostMesh.failed_instances(e.value) == [1, 3]
+    HostMesh.batch([meshes[0], meshes[2]], nthreads=2)
