@@ -161,14 +161,14 @@ def mlp_bench(steps: int):
             "roofline": roof}
 
 
-def steady_bench(n_train: int, n_test: int, n_seq: int, train_res=(64, 32), test_res=(128, 64)):
+def steady_bench(n_train: int, n_test: int, n_seq: int, train_res=(64, 32), test_res=(128, 64), max_active=None):
     """Warm versus cold steady solves/hr (see the module docstring)."""
     from fvs2d_amd import warmstart
     out = warmstart.run(n_train, n_test, train_res, seed=0, steps=4000, batch=65536, lr=1e-3, depth=5, cfl=1.25,
                         max_steps=80000, drop=1e-8, chunk=250, freeze_after=30000, freeze_drop=None,
                         freeze_stall=(2000, 0.1), diverge=1e3, fallback=1.0, group=256, n_seq=n_seq,
                         log=lambda s: print(f"[bench] steady: {s}", file=sys.stderr, flush=True),
-                        test_res=test_res, stall_scale=True)
+                        test_res=test_res, stall_scale=True, max_active=max_active)
     keep = ("n_train", "n_test", "train_converged", "cold", "warm", "plain_cold", "iteration_ratio_median",
             "cl_abs_diff_p50", "cd_abs_diff_p50", "forces_within_1e5", "warm_batched_vs_plain_cold")
     d = {k: out[k] for k in keep if k in out}
@@ -177,7 +177,10 @@ def steady_bench(n_train: int, n_test: int, n_seq: int, train_res=(64, 32), test
                      f"quads -> {2 * te[0] * te[1]} triangles each (the smallest resolution of the batch workload), "
                      "M U[0.5,0.85], AoA U[-2,4] deg, t U[0.10,0.15], Venkatakrishnan K=5 (frozen on stall), "
                      "SSP-RK(4,2) CFL 1.25 (reference convention), to 1e-8 x the cold first-step density residual; "
-                     f"warm start trained on {n_train} seeded solves of the same generator at {tr[0]}x{tr[1]}")
+                     f"warm start trained on {n_train} seeded solves of the same generator at {tr[0]}x{tr[1]}; "
+                     + (f"at most {max_active} instances stepping at once, the next one starting as one finishes"
+                        if max_active else "all instances stepping from the start"))
+    d["max_active"] = max_active
     d["warm_vs_cold_batched"] = round(out["warm"]["solves_per_hour"] / out["cold"]["solves_per_hour"], 3)
     return d
 
@@ -198,6 +201,8 @@ def main():
     ap.add_argument("--steady-train", type=int, default=256)
     ap.add_argument("--steady-test", type=int, default=64)
     ap.add_argument("--steady-seq", type=int, default=8)
+    ap.add_argument("--steady-active", type=int, default=32,
+                    help="instances stepping at once in the batched steady solves (0 = all)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only stream this many MiB through a plain read-once/write-once kernel (x1.0 "
                          "multiply), 3 times, and print its rate: the calibration of the HBM byte counters")
@@ -301,7 +306,8 @@ def main():
               file=sys.stderr, flush=True)
     steady = None
     if args.steady:
-        steady = steady_bench(args.steady_train, args.steady_test, args.steady_seq)
+        steady = steady_bench(args.steady_train, args.steady_test, args.steady_seq,
+                              max_active=args.steady_active or None)
 
     traffic = measured_traffic(args.n, args.reorder, args.path, dom)
     out = {

@@ -111,25 +111,27 @@ class Batch:
         self.solver.cvar = cv
         return True
 
-    def solve(self, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge):
+    def solve(self, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge, max_active=None):
         return self.solver.steady_solve_batch(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after,
                                               freeze_drop=freeze_drop, r_ref=self.r_ref, diverge=diverge,
-                                              on_diverge=self.reset, freeze_stall=freeze_stall)
+                                              on_diverge=self.reset, freeze_stall=freeze_stall,
+                                              max_active=max_active)
 
 
-def cold_solve(conds, cfl, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge, limiter=1):
+def cold_solve(conds, cfl, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge, limiter=1,
+               max_active=None):
     t0 = time.perf_counter()
     b = Batch(conds, cfl, limiter=limiter)
     setup = time.perf_counter() - t0
     b.solver.initialize_freestream()
     t0 = time.perf_counter()
-    res = b.solve(max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge)
+    res = b.solve(max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge, max_active)
     sec = time.perf_counter() - t0
     return b, res, sec, setup
 
 
 def warm_solve(b: Batch, net: WarmStartNet, scale, cfl, max_steps, drop, chunk, freeze_after, freeze_drop,
-               freeze_stall, diverge, fallback: float, limiter=1):
+               freeze_stall, diverge, fallback: float, limiter=1, max_active=None):
     """Warm start of the instances of b (its solver is reused).  Returns the
     solve result with one extra step counted for the first-step check (and
     res["first_ratio"], that step's norm over the cold first-step norm), the
@@ -158,7 +160,7 @@ def warm_solve(b: Batch, net: WarmStartNet, scale, cfl, max_steps, drop, chunk, 
         for k in np.nonzero(fell)[0]:
             cv[b.off[k]:b.off[k + 1]] = freestream(b.conds[k]["mach"], b.conds[k]["aoa"])
         s.cvar = cv
-    res = b.solve(max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge)
+    res = b.solve(max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge, max_active)
     sec = time.perf_counter() - t0
     res["first_ratio"] = first / b.r_ref
     it = res["iterations"]
@@ -178,14 +180,41 @@ def write_warm_start(net: WarmStartNet, scale, mesh: meshgen.Mesh, n_around: int
     return U
 
 
-def sequential_cold(conds, cfl, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, limiter=1):
-    """Plain (one instance at a time) cold solves: their wall time and count."""
+def plain_launch_path(cond, cfl, limiter=1, steps=300):
+    """The faster kernel arrangement for one instance solved on its own: the
+    per-stage gradient + fused flux kernels (kernel_path 0) or the whole stage
+    in one kernel (2), timed over a few hundred steps of a free-stream start."""
+    best, path_best = None, 0
+    for path in (0, 2):
+        m, sch = cases.naca(cond["n_around"], cond["n_layers"], mach=cond["mach"], aoa=cond["aoa"],
+                            thickness=cond["thickness"], cfl=cfl, limiter=limiter, kernel_path=path)
+        s = Solver(m, sch)
+        s.initialize_solution(1, 0.0)
+        s.time_integration(0.0, 20)
+        s.time_integration(0.0, steps)
+        ms = s.last_elapsed_ms
+        s.close()
+        if best is None or ms < best:
+            best, path_best = ms, path
+    return path_best
+
+
+def sequential_cold(conds, cfl, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, limiter=1,
+                    max_active=None):
+    """Plain (one instance at a time) cold solves: their wall time and count.
+    Each runs in the faster kernel arrangement for its resolution (picked
+    once per resolution, outside the timed solves)."""
+    paths = {}
+    for c in conds:
+        key = (c["n_around"], c["n_layers"])
+        if key not in paths:
+            paths[key] = plain_launch_path(c, cfl, limiter)
     t0 = time.perf_counter()
     ok = 0
     its = []
     for c in conds:
         m, sch = cases.naca(c["n_around"], c["n_layers"], mach=c["mach"], aoa=c["aoa"], thickness=c["thickness"],
-                            cfl=cfl, limiter=limiter)
+                            cfl=cfl, limiter=limiter, kernel_path=paths[(c["n_around"], c["n_layers"])])
         s = Solver(m, sch)
         s.set_graph(True)
         s.initialize_solution(1, 0.0)
@@ -221,7 +250,8 @@ def make_dataset(conds, group, cfl, max_steps, drop, chunk, freeze_after, freeze
 
 
 def run(n_train, n_test, res, seed, steps, batch, lr, depth, cfl, max_steps, drop, chunk, freeze_after, freeze_drop,
-        freeze_stall, diverge, fallback, group, n_seq, limiter=1, log=print, test_res=None, stall_scale=False):
+        freeze_stall, diverge, fallback, group, n_seq, limiter=1, log=print, test_res=None, stall_scale=False,
+        max_active=None):
     """Train on n_train converged solves at resolution ``res``, then solve
     n_test held-out conditions at ``test_res`` (default ``res``; the network's
     inputs do not depend on the grid) cold and warm.  With ``stall_scale``
@@ -234,7 +264,7 @@ def run(n_train, n_test, res, seed, steps, batch, lr, depth, cfl, max_steps, dro
                fallback=fallback, depth=depth, batch=batch, lr=lr, steps=steps, limiter=limiter)
     train_c = draw(n_train, seed, res)
     test_c = draw(n_test, seed + 1_000_003, test_res)
-    kw_test = dict(kw)
+    kw_test = dict(kw, max_active=max_active)
     if stall_scale and freeze_stall is not None:
         kw_test["freeze_stall"] = (int(freeze_stall[0] * test_res[0] / 64), freeze_stall[1])
     raw, du, its, gen_s = make_dataset(train_c, group, diverge=diverge, log=log, **kw)
@@ -318,6 +348,8 @@ def main(argv=None):
     ap.add_argument("--res", type=str, default="64x32")
     ap.add_argument("--test-res", type=str, default=None, help="resolution of the held-out solves (default --res)")
     ap.add_argument("--stall-scale", type=int, default=0, help="scale the stall window with n_around / 64")
+    ap.add_argument("--max-active", type=int, default=0,
+                    help="held-out instances stepping at once, the next starting as one finishes (0 = all)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=65536)
@@ -345,7 +377,7 @@ def main(argv=None):
               a.freeze_after, a.freeze_drop, stall, a.diverge, a.fallback, a.group, a.seq, a.limiter,
               log=lambda s: print(s, flush=True),
               test_res=None if a.test_res is None else tuple(int(v) for v in a.test_res.split("x")),
-              stall_scale=bool(a.stall_scale))
+              stall_scale=bool(a.stall_scale), max_active=a.max_active or None)
     print(json.dumps(out))
     return 0
 

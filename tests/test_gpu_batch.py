@@ -185,3 +185,25 @@ def test_steady_batch_restart_on_divergence():
     assert calls == [2]
     assert res2["restarted"][2] and res2["diverged"][2] and res2["iterations"][2] == -1
     assert np.array_equal(res2["iterations"][:2], res["iterations"][:2])
+
+
+def test_queued_batch_iterations_match_all_active():
+    """Continuous batching (at most two of four instances stepping, the next
+    starting as one converges): without the limiter every instance needs the
+    same number of iterations as when all four step from the start, since
+    masked instances are left untouched and each instance's count runs from
+    its own start."""
+    inst = [((32, 16), (0.6, -2.0)), ((40, 20), (0.5, 1.0)), ((36, 18), (0.55, 2.0)), ((32, 16), (0.65, 0.5))]
+
+    def run(max_active):
+        meshes = [cases.naca(n, l, cfl=1.0)[0] for (n, l), _ in inst]
+        _, sch = cases.naca(16, 8, cfl=1.0, limiter=0)
+        b = BatchSolver(meshes, [ma for _, ma in inst], sch)
+        b.initialize_freestream()
+        r_ref = b.reference_norms()
+        res = b.steady_solve_batch(20000, drop=1e-6, chunk=100, r_ref=r_ref, max_active=max_active)
+        return res["iterations"]
+
+    all_on, queued = run(None), run(2)
+    assert np.all(all_on > 0)
+    assert np.array_equal(all_on, queued)
