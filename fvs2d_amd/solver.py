@@ -346,7 +346,8 @@ class Solver:
 
     def steady_solve(self, max_steps: int, drop: float = 1e-8, chunk: int = 200, t1: float = 0.0,
                      freeze_at: Optional[float] = None, freeze_after: Optional[int] = None,
-                     freeze_stall: Optional[Tuple[int, float]] = None):
+                     freeze_stall: Optional[Tuple[int, float]] = None,
+                     give_up: Optional[Tuple[int, float]] = None):
         """Pseudo-time stepping until the density residual of
         src/runge_kutta.f90:396-398 has dropped by ``drop`` relative to the first
         step (the reference runs a fixed step count; the stopping rule is this
@@ -356,17 +357,21 @@ class Solver:
         ``freeze_after`` after that many steps (limiter chatter otherwise
         stalls convergence); with ``freeze_stall = (window, factor)`` once the
         norm at a chunk end is above ``factor`` times the norm ``window`` steps
-        earlier (the same rule as steady_solve_batch).  Returns a dict with the
-        iteration of the crossing (or None), the norm history, the iteration at
-        which the limiter was frozen, and whether a non-finite norm
-        (divergence) stopped the loop."""
+        earlier (the same rule as steady_solve_batch).  With ``give_up =
+        (window, factor)`` the solve stops unconverged once, after the limiter
+        was frozen, the norm at a chunk end is above ``factor`` times the norm
+        ``window`` steps earlier.  Returns a dict with the iteration of the
+        crossing (or None), the norm history, the iteration at which the
+        limiter was frozen, whether a non-finite norm (divergence) stopped the
+        loop and whether the give-up rule did."""
         hist = []
         done = 0
         conv_at = None
         frozen_at = None
         diverged = False
         r1 = None
-        ends = []
+        ends, ends_f = [], []
+        gave_up = False
         while done < max_steps:
             n = min(chunk, max_steps - done)
             if freeze_after is not None and frozen_at is None:
@@ -390,15 +395,21 @@ class Solver:
                 break
             done += n
             ends.append((done, float(norms[-1, 0])))
-            if frozen_at is None and ((freeze_at is not None and norms[-1, 0] <= freeze_at * r1)
-                                      or (freeze_stall is not None and _stalled(ends, freeze_stall))):
+            if frozen_at is not None:
+                ends_f.append(ends[-1])
+                if give_up is not None and _stalled(ends_f, give_up):
+                    gave_up = True
+                    break
+            elif ((freeze_at is not None and norms[-1, 0] <= freeze_at * r1)
+                  or (freeze_stall is not None and _stalled(ends, freeze_stall))):
                 self.freeze_limiter(True)
                 frozen_at = done
+                ends_f = [ends[-1]]
         if frozen_at is not None:
             self.freeze_limiter(False)
         h = np.concatenate(hist) if hist else np.zeros((0, 4))
         return dict(iterations=conv_at, steps_run=done, history=h, diverged=diverged, limiter_frozen_at=frozen_at,
-                    drop=float(h[-1, 0] / r1) if len(h) else None)
+                    drop=float(h[-1, 0] / r1) if len(h) else None, gave_up=gave_up)
 
     def write_save(self, base: str, itime: int, imach: int = 2):
         """The reference's save file of the current state (``<base>.cd`` +
@@ -451,10 +462,10 @@ class Solver:
 
     def autotune(self, nsteps: int = 10):
         """Pick plain or XCD-aware workgroup numbering per kernel class (bit
-        0) and, for the LSQ gradient or a limited Green-Gauss gradient's
-        limiter, whether it reads its stencil from a per-block halo staged in
-        LDS (bit 1); the state is left as it was.
-        Returns (choices, best ms per step)."""
+        0), for the LSQ gradient or a limited Green-Gauss gradient's limiter
+        whether it reads its stencil from a per-block halo staged in LDS (bit
+        1), and for the fused flux kernel 128- or 256-cell blocks (bit 2); the
+        state is left as it was.  Returns (choices, best ms per step)."""
         ch = np.zeros(4, np.int32)
         best = np.zeros(1)
         check(self._L.fvs_autotune(self._h, nsteps, ptr(ch), ptr(best)))
@@ -554,7 +565,7 @@ class BatchSolver(Solver):
                            freeze_after: Optional[int] = None, freeze_drop: Optional[float] = None,
                            r_ref=None, diverge: Optional[float] = None, on_diverge=None,
                            freeze_stall: Optional[Tuple[int, float]] = None, progress=None,
-                           max_active: Optional[int] = None):
+                           max_active: Optional[int] = None, give_up: Optional[Tuple[int, float]] = None):
         """Pseudo-time stepping of all instances, each until its density
         residual norm is at most ``drop * r_ref[k]`` (``r_ref`` defaults to
         each instance's first-step norm); converged instances are masked out
@@ -576,7 +587,12 @@ class BatchSolver(Solver):
         are not launched, so they cost nothing), and one starts at each chunk
         boundary for each instance that has finished (converged or given up).
         An instance's iterations, freezing and divergence count from its
-        start; ``max_steps`` bounds each instance's own steps."""
+        start; ``max_steps`` bounds each instance's own steps.
+
+        With ``give_up = (window, factor)`` (``window`` may be one value per
+        instance) an instance is given up, unconverged, once after its limiter
+        was frozen its norm at a chunk end is above ``factor`` times its norm
+        ``window`` steps earlier; ``gave_up`` flags them."""
         n = self.ninst
         cap = n if max_active is None else max(1, min(int(max_active), n))
         if freeze_stall is not None:
@@ -584,6 +600,8 @@ class BatchSolver(Solver):
 
             def stall_rule(k):
                 return int(win[k]), freeze_stall[1]
+        if give_up is not None:
+            gwin = np.broadcast_to(np.asarray(give_up[0], np.int64), (n,))
         act0 = np.zeros(n, np.int32)
         act0[:cap] = 1
         self.set_active(act0)
@@ -600,7 +618,9 @@ class BatchSolver(Solver):
         frozen_at = np.full(n, -1, np.int64)
         diverged = np.zeros(n, bool)
         restarted = np.zeros(n, bool)
+        gave_up = np.zeros(n, bool)
         ends = [[] for _ in range(n)]
+        ends_f = [[] for _ in range(n)]
         self.freeze_limiter_batch(frozen)
         while np.any(self.active):
             m = chunk
@@ -646,16 +666,23 @@ class BatchSolver(Solver):
                     continue
                 since[k] += m
                 ends[k].append((int(since[k]), float(nr[-1, k])))
-                if not frozen[k] and ((freeze_drop is not None and nr[-1, k] <= freeze_drop * r_ref[k])
-                                      or (freeze_after is not None and since[k] >= freeze_after)
-                                      or (freeze_stall is not None and _stalled(ends[k], stall_rule(k)))):
+                if frozen[k]:
+                    ends_f[k].append(ends[k][-1])
+                    if give_up is not None and _stalled(ends_f[k], (int(gwin[k]), give_up[1])):
+                        gave_up[k] = True
+                        self.active[k] = 0
+                elif ((freeze_drop is not None and nr[-1, k] <= freeze_drop * r_ref[k])
+                      or (freeze_after is not None and since[k] >= freeze_after)
+                      or (freeze_stall is not None and _stalled(ends[k], stall_rule(k)))):
                     frozen[k] = 1
                     frozen_at[k] = done - started[k] + m
+                    ends_f[k] = [ends[k][-1]]
             for k in reset:
                 since[k] = 0
                 frozen[k] = 0
                 frozen_at[k] = -1
                 ends[k] = []
+                ends_f[k] = []
             done += m
             self.freeze_limiter_batch(frozen)
             while queued and int(np.sum(self.active)) < cap:
@@ -669,7 +696,7 @@ class BatchSolver(Solver):
         self.freeze_limiter(False)
         self.set_active(np.ones(n, np.int32))
         return dict(iterations=iters, steps_run=done, diverged=diverged, restarted=restarted, frozen_at=frozen_at,
-                    r_ref=r_ref)
+                    r_ref=r_ref, gave_up=gave_up)
 
 
 def _stalled(ends, rule) -> bool:

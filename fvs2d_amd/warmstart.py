@@ -111,27 +111,28 @@ class Batch:
         self.solver.cvar = cv
         return True
 
-    def solve(self, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge, max_active=None):
+    def solve(self, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge, max_active=None,
+              give_up=None):
         return self.solver.steady_solve_batch(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after,
                                               freeze_drop=freeze_drop, r_ref=self.r_ref, diverge=diverge,
                                               on_diverge=self.reset, freeze_stall=freeze_stall,
-                                              max_active=max_active)
+                                              max_active=max_active, give_up=give_up)
 
 
 def cold_solve(conds, cfl, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge, limiter=1,
-               max_active=None):
+               max_active=None, give_up=None):
     t0 = time.perf_counter()
     b = Batch(conds, cfl, limiter=limiter)
     setup = time.perf_counter() - t0
     b.solver.initialize_freestream()
     t0 = time.perf_counter()
-    res = b.solve(max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge, max_active)
+    res = b.solve(max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge, max_active, give_up)
     sec = time.perf_counter() - t0
     return b, res, sec, setup
 
 
 def warm_solve(b: Batch, net: WarmStartNet, scale, cfl, max_steps, drop, chunk, freeze_after, freeze_drop,
-               freeze_stall, diverge, fallback: float, limiter=1, max_active=None):
+               freeze_stall, diverge, fallback: float, limiter=1, max_active=None, give_up=None):
     """Warm start of the instances of b (its solver is reused).  Returns the
     solve result with one extra step counted for the first-step check (and
     res["first_ratio"], that step's norm over the cold first-step norm), the
@@ -160,7 +161,7 @@ def warm_solve(b: Batch, net: WarmStartNet, scale, cfl, max_steps, drop, chunk, 
         for k in np.nonzero(fell)[0]:
             cv[b.off[k]:b.off[k + 1]] = freestream(b.conds[k]["mach"], b.conds[k]["aoa"])
         s.cvar = cv
-    res = b.solve(max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge, max_active)
+    res = b.solve(max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge, max_active, give_up)
     sec = time.perf_counter() - t0
     res["first_ratio"] = first / b.r_ref
     it = res["iterations"]
@@ -200,7 +201,7 @@ def plain_launch_path(cond, cfl, limiter=1, steps=300):
 
 
 def sequential_cold(conds, cfl, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, limiter=1,
-                    max_active=None):
+                    max_active=None, give_up=None):
     """Plain (one instance at a time) cold solves: their wall time and count.
     Each runs in the faster kernel arrangement for its resolution (picked
     once per resolution, outside the timed solves)."""
@@ -219,7 +220,7 @@ def sequential_cold(conds, cfl, max_steps, drop, chunk, freeze_after, freeze_dro
         s.set_graph(True)
         s.initialize_solution(1, 0.0)
         r = s.steady_solve(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after, freeze_at=freeze_drop,
-                           freeze_stall=freeze_stall)
+                           freeze_stall=freeze_stall, give_up=give_up)
         ok += r["iterations"] is not None
         its.append(r["iterations"] or -1)
         s.close()
@@ -227,13 +228,13 @@ def sequential_cold(conds, cfl, max_steps, drop, chunk, freeze_after, freeze_dro
 
 
 def make_dataset(conds, group, cfl, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall, diverge,
-                 limiter=1, log=print):
+                 limiter=1, log=print, give_up=None):
     """Cold batched solves in groups; rows of the converged instances."""
     raws, dus, its, secs, n_ok = [], [], [], 0.0, 0
     for g0 in range(0, len(conds), group):
         part = conds[g0:g0 + group]
         b, res, sec, _ = cold_solve(part, cfl, max_steps, drop, chunk, freeze_after, freeze_drop, freeze_stall,
-                                    diverge, limiter)
+                                    diverge, limiter, give_up=give_up)
         secs += sec
         cv = b.solver.cvar
         rec = b.records()
@@ -251,23 +252,29 @@ def make_dataset(conds, group, cfl, max_steps, drop, chunk, freeze_after, freeze
 
 def run(n_train, n_test, res, seed, steps, batch, lr, depth, cfl, max_steps, drop, chunk, freeze_after, freeze_drop,
         freeze_stall, diverge, fallback, group, n_seq, limiter=1, log=print, test_res=None, stall_scale=False,
-        max_active=None):
+        max_active=None, give_up=None):
     """Train on n_train converged solves at resolution ``res``, then solve
     n_test held-out conditions at ``test_res`` (default ``res``; the network's
     inputs do not depend on the grid) cold and warm.  With ``stall_scale``
-    the stall window scales with n_around / 64 (cases.stall_rule_for)."""
+    the stall and give-up windows scale with n_around / 64 at each
+    resolution (cases.stall_rule_for)."""
     test_res = tuple(res if test_res is None else test_res)
     kw = dict(cfl=cfl, max_steps=max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after,
-              freeze_drop=freeze_drop, freeze_stall=freeze_stall, limiter=limiter)
+              freeze_drop=freeze_drop, freeze_stall=freeze_stall, limiter=limiter, give_up=give_up)
     out = dict(resolution=list(res), test_resolution=list(test_res), n_train=n_train, n_test=n_test, seed=seed,
                cfl=cfl, drop=drop, freeze_after=freeze_after, freeze_drop=freeze_drop, freeze_stall=freeze_stall,
-               fallback=fallback, depth=depth, batch=batch, lr=lr, steps=steps, limiter=limiter)
+               give_up=give_up, fallback=fallback, depth=depth, batch=batch, lr=lr, steps=steps, limiter=limiter)
     train_c = draw(n_train, seed, res)
     test_c = draw(n_test, seed + 1_000_003, test_res)
+
+    def scaled(rule, r):
+        return None if rule is None or not stall_scale else (int(rule[0] * r[0] / 64), rule[1])
+    kw_train = dict(kw)
     kw_test = dict(kw, max_active=max_active)
-    if stall_scale and freeze_stall is not None:
-        kw_test["freeze_stall"] = (int(freeze_stall[0] * test_res[0] / 64), freeze_stall[1])
-    raw, du, its, gen_s = make_dataset(train_c, group, diverge=diverge, log=log, **kw)
+    if stall_scale:
+        kw_train.update(freeze_stall=scaled(freeze_stall, res), give_up=scaled(give_up, res))
+        kw_test.update(freeze_stall=scaled(freeze_stall, test_res), give_up=scaled(give_up, test_res))
+    raw, du, its, gen_s = make_dataset(train_c, group, diverge=diverge, log=log, **kw_train)
     out["train_converged"] = len(its)
     out["train_rows"] = int(raw.shape[0])
     out["train_solve_seconds"] = gen_s
@@ -305,10 +312,12 @@ def run(n_train, n_test, res, seed, steps, batch, lr, depth, cfl, max_steps, dro
     dstate = np.array(dstate)
     out.update(
         cold=dict(seconds=cold_s, converged=int((ic > 0).sum()), diverged=int(rc["diverged"].sum()),
+                  gave_up=int(rc["gave_up"].sum()),
                   solves_per_hour=3600.0 * (ic > 0).sum() / cold_s,
                   iterations_p50=float(np.percentile(ic[ic > 0], 50)) if np.any(ic > 0) else None,
                   iterations_p99=float(np.percentile(ic[ic > 0], 99)) if np.any(ic > 0) else None),
         warm=dict(seconds=warm_s, converged=int((iw > 0).sum()), diverged=int(rw["diverged"].sum()),
+                  gave_up=int(rw["gave_up"].sum()),
                   fell_back=int(fell.sum()), restarted=int(rw["restarted"].sum()),
                   solves_per_hour=3600.0 * (iw > 0).sum() / warm_s,
                   iterations_p50=float(np.percentile(iw[iw > 0], 50)) if np.any(iw > 0) else None,
@@ -328,6 +337,7 @@ def run(n_train, n_test, res, seed, steps, batch, lr, depth, cfl, max_steps, dro
     out["per_instance"] = [dict(mach=c["mach"], aoa=c["aoa"], thickness=c["thickness"], cold=int(ic[k]),
                                 warm=int(iw[k]), fell_back=bool(fell[k]), warm_first_ratio=float(rw["first_ratio"][k]),
                                 cold_frozen_at=int(rc["frozen_at"][k]), warm_frozen_at=int(rw["frozen_at"][k]),
+                                cold_gave_up=bool(rc["gave_up"][k]), warm_gave_up=bool(rw["gave_up"][k]),
                                 cl_cold=float(f_cold[k, 0]),
                                 cl_warm=float(f_warm[k, 0]), cd_cold=float(f_cold[k, 1]),
                                 cd_warm=float(f_warm[k, 1]), state_rel_diff=float(dstate[k]))
@@ -364,6 +374,9 @@ def main(argv=None):
     ap.add_argument("--freeze-stall", type=str, default="2000,0.1",
                     help="window,factor: freeze an instance's limiter when its residual fell by less than "
                          "1/factor over the last window steps ('none' to disable)")
+    ap.add_argument("--give-up", type=str, default="none",
+                    help="window,factor: give an instance up once, after its limiter was frozen, its residual "
+                         "fell by less than 1/factor over the last window steps ('none' to disable)")
     ap.add_argument("--diverge", type=float, default=1e3)
     ap.add_argument("--fallback", type=float, default=1.0)
     ap.add_argument("--group", type=int, default=256)
@@ -377,7 +390,8 @@ def main(argv=None):
               a.freeze_after, a.freeze_drop, stall, a.diverge, a.fallback, a.group, a.seq, a.limiter,
               log=lambda s: print(s, flush=True),
               test_res=None if a.test_res is None else tuple(int(v) for v in a.test_res.split("x")),
-              stall_scale=bool(a.stall_scale), max_active=a.max_active or None)
+              stall_scale=bool(a.stall_scale), max_active=a.max_active or None,
+              give_up=None if a.give_up == "none" else (int(a.give_up.split(",")[0]), float(a.give_up.split(",")[1])))
     print(json.dumps(out))
     return 0
 
