@@ -12,14 +12,15 @@ object of the dominant kernel (algorithmic bytes per launch / mean HIP-event
 duration of that kernel on its stream; HBM traffic per launch from the
 committed rocprofv3 counter summary of the same workload, when there is one),
 the other gradient / limiter variants of the same mesh, the CPU baseline (the
-C oracle, single thread, on a few steps of the same mesh), and two objects for
-the steady part of the metric:
+C oracle on all host cores, and on one, on a few steps of the same mesh), and
+two objects for the steady part of the metric:
 
 * "steady": steady solves/hr to a 1e-8 density-residual drop on NACA O-grids
-  (64x32 quads -> 4,096 triangles, Venkatakrishnan, SSP-RK(4,2), CFL 1.25):
-  a warm-start network trained on batched cold solves of seeded conditions,
-  then held-out conditions solved batched cold, batched warm and one at a time
-  cold (plain);
+  (128x64 quads -> 16,384 triangles, the smallest resolution of the batch
+  workload; Venkatakrishnan, SSP-RK(4,2), CFL 1.25): a warm-start network
+  trained on batched cold solves of seeded conditions, then held-out
+  conditions solved batched cold, batched warm (32 stepping at once, the next
+  starting as one finishes) and one at a time cold (plain);
 * "mlp": training-step rate of the warm-start network (68 -> 5 x 256 -> 4,
   65,536-row batches) with its per-kernel matrix-core rate against the bf16
   dense peak.
@@ -90,14 +91,16 @@ def cpu_baseline(mesh, dt: float, steps: int, threads: int = 1):
     return mesh.ncells * 3 * steps / sec, sec
 
 
-def measured_traffic(n: int, reorder: int, path: int, kernel: str):
+def measured_traffic(n: int, reorder: int, path: int, kernel: str, fused_choice: int):
     """HBM bytes per launch of the dominant kernel from the committed counter
-    summary, if it was taken on this workload."""
+    summary, if it was taken on this workload with the same fused-kernel
+    launch choice (block size and workgroup numbering)."""
     if not os.path.exists(TRAFFIC):
         return None
     with open(TRAFFIC) as f:
         t = json.load(f)
-    if (t.get("n"), t.get("reorder"), t.get("path"), t.get("kernel_class")) != (n, reorder, path, kernel):
+    if (t.get("n"), t.get("reorder"), t.get("path"), t.get("kernel_class"), t.get("fused_choice")) != \
+            (n, reorder, path, kernel, fused_choice):
         return None
     return t.get("bytes_per_launch")
 
@@ -161,15 +164,17 @@ def mlp_bench(steps: int):
             "roofline": roof}
 
 
-def steady_bench(n_train: int, n_test: int, n_seq: int, train_res=(64, 32), test_res=(128, 64), max_active=None):
+def steady_bench(n_train: int, n_test: int, n_seq: int, train_res=(128, 64), test_res=(128, 64), max_active=None,
+                 give_up=(8000, 0.5)):
     """Warm versus cold steady solves/hr (see the module docstring)."""
     from fvs2d_amd import warmstart
     out = warmstart.run(n_train, n_test, train_res, seed=0, steps=4000, batch=65536, lr=1e-3, depth=5, cfl=1.25,
                         max_steps=80000, drop=1e-8, chunk=250, freeze_after=30000, freeze_drop=None,
                         freeze_stall=(2000, 0.1), diverge=1e3, fallback=1.0, group=256, n_seq=n_seq,
                         log=lambda s: print(f"[bench] steady: {s}", file=sys.stderr, flush=True),
-                        test_res=test_res, stall_scale=True, max_active=max_active)
-    keep = ("n_train", "n_test", "train_converged", "cold", "warm", "plain_cold", "iteration_ratio_median",
+                        test_res=test_res, stall_scale=True, max_active=max_active, give_up=give_up)
+    keep = ("n_train", "n_test", "train_converged", "train_solve_seconds", "cold", "warm", "plain_cold",
+            "iteration_ratio_median",
             "cl_abs_diff_p50", "cd_abs_diff_p50", "forces_within_1e5", "warm_batched_vs_plain_cold")
     d = {k: out[k] for k in keep if k in out}
     tr, te = train_res, test_res
@@ -177,6 +182,8 @@ def steady_bench(n_train: int, n_test: int, n_seq: int, train_res=(64, 32), test
                      f"quads -> {2 * te[0] * te[1]} triangles each (the smallest resolution of the batch workload), "
                      "M U[0.5,0.85], AoA U[-2,4] deg, t U[0.10,0.15], Venkatakrishnan K=5 (frozen on stall), "
                      "SSP-RK(4,2) CFL 1.25 (reference convention), to 1e-8 x the cold first-step density residual; "
+                     f"an instance is given up when, after its limiter froze, its residual fell by less than "
+                     f"{1 / give_up[1]:g}x over {give_up[0]} x n_around/64 steps (cold and warm alike); "
                      f"warm start trained on {n_train} seeded solves of the same generator at {tr[0]}x{tr[1]}; "
                      + (f"at most {max_active} instances stepping at once, the next one starting as one finishes"
                         if max_active else "all instances stepping from the start"))
@@ -203,6 +210,9 @@ def main():
     ap.add_argument("--steady-seq", type=int, default=8)
     ap.add_argument("--steady-active", type=int, default=32,
                     help="instances stepping at once in the batched steady solves (0 = all)")
+    ap.add_argument("--choices", type=str, default=None,
+                    help="launch choices gradient,flux,update,fused (the autotuner's encoding) applied instead "
+                         "of autotuning, so that a profiler sees only the timed configuration")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only stream this many MiB through a plain read-once/write-once kernel (x1.0 "
                          "multiply), 3 times, and print its rate: the calibration of the HBM byte counters")
@@ -241,8 +251,14 @@ def main():
     print(f"[bench] mesh {N} cells, dt={dt:.6e}, setup {setup_s:.1f}s", file=sys.stderr, flush=True)
 
     s0.initialize_solution(1, 0.0)
-    tuned, tuned_ms = s0.autotune(10)
-    print(f"[bench] autotune: {tuned} ({tuned_ms:.4f} ms/step)", file=sys.stderr, flush=True)
+    if args.choices:
+        tuned = dict(zip(("gradient", "flux", "update", "fused"), (int(v) for v in args.choices.split(","))))
+        s0.set_launch_choices(tuned)
+        tuned_ms = None
+        print(f"[bench] launch choices: {tuned}", file=sys.stderr, flush=True)
+    else:
+        tuned, tuned_ms = s0.autotune(10)
+        print(f"[bench] autotune: {tuned} ({tuned_ms:.4f} ms/step)", file=sys.stderr, flush=True)
     wall, dev = timed_run(s0, args.steps, args.warmup, init=False)
     stages = 3
     value = N * stages * args.steps / wall
@@ -309,7 +325,7 @@ def main():
         steady = steady_bench(args.steady_train, args.steady_test, args.steady_seq,
                               max_active=args.steady_active or None)
 
-    traffic = measured_traffic(args.n, args.reorder, args.path, dom)
+    traffic = measured_traffic(args.n, args.reorder, args.path, dom, tuned.get("fused"))
     out = {
         "metric": "cell-stage updates/s (FP64)",
         "value": round(value, 1),
@@ -319,7 +335,7 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": round(wall / args.steps * 1e3, 4),
         "device_ms_per_step": round(dev / args.steps * 1e3, 4),
-        "autotune_ms_per_step": round(tuned_ms, 4),
+        "autotune_ms_per_step": None if tuned_ms is None else round(tuned_ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

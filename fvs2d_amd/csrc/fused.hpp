@@ -321,7 +321,8 @@ __device__ __forceinline__ void block_update_finish(int N, int lb, const Upd &A,
     }
     const double V = in.V;
     double R[4];
-    for (int v = 0; v < 4; ++v) R[v] = -acc[v] / V;
+    const double iV = 1.0 / V;  // one division per cell (src/residual.f90:164-166 divides each)
+    for (int v = 0; v < 4; ++v) R[v] = -acc[v] * iV;
     double hs = A.h;
     if (A.steady) {
       double dl;

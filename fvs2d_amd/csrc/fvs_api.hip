@@ -623,7 +623,8 @@ __global__ __launch_bounds__(kBlock) void k_update(int nbx, int N, int gw, Upd A
     }
     const double V = vol[i];
     double R[4];
-    for (int v = 0; v < 4; ++v) R[v] = -acc[v] / V;
+    const double iV = 1.0 / V;  // one division per cell (src/residual.f90:164-166 divides each)
+    for (int v = 0; v < 4; ++v) R[v] = -acc[v] * iV;
     if (A.mode == M_RESID) {
       st4(rout, i, R);
       wsout[i] = wacc;
