@@ -113,7 +113,15 @@ __global__ void k_prim(int N, double g, const double *__restrict__ U, double *__
 
 // Limiter of one cell from its stencil min/max and its gradient
 // (src/gradient_limiter.f90:47-91): phi = min over variables of
-// min(1, min over faces of L(dmax|dmin, q_f - q)).
+// min(1, min over faces of L(dmax|dmin, q_f - q)).  For Venkatakrishnan and
+// Barth every candidate L = n/d has d > 0 (Venkatakrishnan's denominator is
+// (a + b/2)^2 + 7b^2/4 + eps^2; Barth's quotient a/b is taken with both
+// signs flipped when b < 0), so the smallest one is found by comparing
+// n_i d_j with n_j d_i and divided out once per cell instead of once per
+// face and variable; the result is the selected candidate's own quotient,
+// which can differ from the reference's minimum only where two candidates
+// are within rounding of each other.  Van Albada divides per face as
+// written.
 __device__ __forceinline__ double cell_limiter(int lt, int nf, int N, int i, const double2 *__restrict__ off,
                                                double vol, const double w[4], const double mn[4],
                                                const double mx[4], const double gx[4], const double gy[4]) {
@@ -121,25 +129,55 @@ __device__ __forceinline__ double cell_limiter(int lt, int nf, int N, int i, con
   double2 o[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) o[k] = (k < nf) ? off[long(k) * N + i] : make_double2(0.0, 0.0);
-  double phi = 0.0;
+  if (lt == 3) {
+    double phi = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double q = w[v], dmax = mx[v] - q, dmin = mn[v] - q;
+      double pe = 100.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (k >= nf) break;
+        const double pf = q + o[k].x * gx[v] + o[k].y * gy[v];
+        const double diff = pf - q;
+        double e = 1.0;
+        if (diff > 0.0) e = limiter_fn(lt, dmax, diff, vol);
+        else if (diff < 0.0) e = limiter_fn(lt, dmin, diff, vol);
+        pe = (e < pe) ? e : pe;
+      }
+      const double pv = (1.0 < pe) ? 1.0 : pe;
+      phi = (v == 0 || pv < phi) ? pv : phi;
+    }
+    return phi;
+  }
+  const double eps2 = venkat_eps2(vol);
+  double bn = 1.0, bd = 1.0;  // smallest candidate so far (the cap at 1 to start)
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const double q = w[v], dmax = mx[v] - q, dmin = mn[v] - q;
-    double pe = 100.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (k >= nf) break;
       const double pf = q + o[k].x * gx[v] + o[k].y * gy[v];
-      const double diff = pf - q;
-      double e = 1.0;
-      if (diff > 0.0) e = limiter_fn(lt, dmax, diff, vol);
-      else if (diff < 0.0) e = limiter_fn(lt, dmin, diff, vol);
-      pe = (e < pe) ? e : pe;
+      const double b = pf - q;
+      if (b != 0.0) {
+        const double a = (b > 0.0) ? dmax : dmin;
+        double n, d;
+        if (lt == 1) {
+          n = (a * a + eps2) + 2.0 * b * a;
+          d = a * a + 2.0 * (b * b) + a * b + eps2;
+        } else {
+          n = (b > 0.0) ? a : -a;
+          d = (b > 0.0) ? b : -b;
+        }
+        if (n * bd < bn * d) {
+          bn = n;
+          bd = d;
+        }
+      }
     }
-    const double pv = (1.0 < pe) ? 1.0 : pe;
-    phi = (v == 0 || pv < phi) ? pv : phi;
   }
-  return phi;
+  return bn / bd;
 }
 
 // LSQ gradient (src/gradient_lsq.f90:371-419): grad_i = sum_j cw_j (q_j - q_i),

@@ -121,12 +121,18 @@ __device__ __forceinline__ void mms_state(double x, double y, double pv[4]) {
   pv[3] = 1.62 + 0.31 * sin((3.79 * kPiD) * x + (2.98 * kPiD) * y);
 }
 
+// Venkatakrishnan's eps^2 = (K h)^3 with K = 5 and h the diameter of the
+// circle of the cell's area (src/gradient_limiter.f90:114-116)
+__device__ __forceinline__ double venkat_eps2(double vol) {
+  const double h = 2.0 * sqrt(vol / kPiD);
+  const double t = 5.0 * h;
+  return t * t * t;
+}
+
 // limiter function: src/gradient_limiter.f90:103-134
 __device__ __forceinline__ double limiter_fn(int lt, double a, double b, double vol) {
   if (lt == 1) {
-    const double h = 2.0 * sqrt(vol / kPiD);
-    const double t = 5.0 * h;
-    const double eps2 = t * t * t;
+    const double eps2 = venkat_eps2(vol);
     return ((a * a + eps2) + 2.0 * b * a) / (a * a + 2.0 * (b * b) + a * b + eps2);
   } else if (lt == 2) {
     const double r = a / b;
