@@ -90,12 +90,14 @@ def stall_rule_for(d, freeze_stall, stall_scale: bool):
 
 def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, freeze_after: int,
               resolutions=RESOLUTIONS, sequential: int = 0, chunk: int = 250, graph: bool = True,
-              freeze_drop=None, freeze_stall=None, stall_scale: bool = False):
+              freeze_drop=None, freeze_stall=None, stall_scale: bool = False, give_up=None, max_active=None):
     """Batched steady solves of drawn instances; optionally the first
     ``sequential`` of them again one at a time for comparison.  With
-    ``stall_scale`` the stall window grows with the grid: ``window * n_around
-    / 64`` steps per instance (the residual of an explicit pseudo-time scheme
-    needs a number of steps proportional to the cells across the domain)."""
+    ``stall_scale`` the stall and give-up windows grow with the grid:
+    ``window * n_around / 64`` steps per instance (the residual of an explicit
+    pseudo-time scheme needs a number of steps proportional to the cells
+    across the domain).  ``give_up`` and ``max_active`` as in
+    BatchSolver.steady_solve_batch."""
     from .solver import BatchSolver, HostMesh
     from ._lib import FvsError
     inst = draw_instances(ninst, seed, resolutions)
@@ -121,6 +123,8 @@ def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, fr
         b = BatchSolver(meshes, [(d["mach"], d["aoa"]) for d in inst], sch)
     stall_of = lambda d: stall_rule_for(d, freeze_stall, stall_scale)  # noqa: E731
     batch_stall = None if freeze_stall is None else ([stall_of(d)[0] for d in inst], freeze_stall[1])
+    give_up_of = lambda d: stall_rule_for(d, give_up, stall_scale)  # noqa: E731
+    batch_give_up = None if give_up is None else ([give_up_of(d)[0] for d in inst], give_up[1])
     b.set_graph(graph)
     b.initialize_freestream()
     setup = time.perf_counter() - t0
@@ -132,15 +136,19 @@ def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, fr
             last[0] = time.perf_counter()
             print(f"batch: {done} steps, {active} instances active", file=sys.stderr, flush=True)
 
+    r_ref = None
+    if max_active is not None:  # queued instances need their reference norms up front
+        r_ref = b.reference_norms()
     res = b.steady_solve_batch(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after, freeze_drop=freeze_drop,
-                               freeze_stall=batch_stall, progress=progress)
+                               freeze_stall=batch_stall, progress=progress, give_up=batch_give_up,
+                               max_active=max_active, r_ref=r_ref)
     sec = time.perf_counter() - t0
     print(f"batch: {int((res['iterations'] > 0).sum())}/{ninst} converged in {sec:.1f} s", file=sys.stderr, flush=True)
     forces = b.wall_forces_batch()
     it = res["iterations"]
     ok = it > 0
     out = dict(instances=ninst, rejected=rejected, cells=int(b.ncells), seconds=sec, setup_seconds=setup,
-               converged=int(ok.sum()), diverged=int(res["diverged"].sum()),
+               converged=int(ok.sum()), diverged=int(res["diverged"].sum()), gave_up=int(res["gave_up"].sum()),
                solves_per_hour=3600.0 * ok.sum() / sec,
                iterations_p50=float(np.percentile(it[ok], 50)) if ok.any() else None,
                iterations_p99=float(np.percentile(it[ok], 99)) if ok.any() else None,
@@ -156,7 +164,7 @@ def run_batch(ninst: int, seed: int, max_steps: int, drop: float, cfl: float, fr
             s.set_graph(graph)
             s.initialize_solution(1, 0.0)
             r = s.steady_solve(max_steps, drop=drop, chunk=chunk, freeze_after=freeze_after, freeze_at=freeze_drop,
-                               freeze_stall=stall_of(d))
+                               freeze_stall=stall_of(d), give_up=give_up_of(d))
             n_ok += r["iterations"] is not None
             s.close()
         sq = time.perf_counter() - t0
@@ -192,6 +200,10 @@ def main(argv=None):
     bt.add_argument("--resolutions", type=str, default="128x64,256x128,384x192,512x256")
     bt.add_argument("--sequential", type=int, default=0)
     bt.add_argument("--graph", type=int, default=1)
+    bt.add_argument("--give-up", type=str, default="none",
+                    help="window,factor: give an instance up once, after its limiter froze, its residual fell by "
+                         "less than 1/factor over the last window steps (scaled like the stall window)")
+    bt.add_argument("--max-active", type=int, default=0, help="instances stepping at once (0 = all)")
     a = ap.parse_args(argv)
     if a.cmd == "steady":
         out = run_steady(a.n, a.layers, a.mach, a.aoa, a.thickness, a.cfl, a.max_steps, a.drop, a.limiter,
@@ -202,7 +214,10 @@ def main(argv=None):
                                                                                  a.freeze_stall.split(",")))
         out = run_batch(a.instances, a.seed, a.max_steps, a.drop, a.cfl, a.freeze_after, res, a.sequential,
                         graph=bool(a.graph), freeze_drop=a.freeze_drop, freeze_stall=stall,
-                        stall_scale=bool(a.stall_scale))
+                        stall_scale=bool(a.stall_scale),
+                        give_up=None if a.give_up == "none" else tuple(t(v) for t, v in zip(
+                            (int, float), a.give_up.split(","))),
+                        max_active=a.max_active or None)
     print(json.dumps(out))
     return 0
 
