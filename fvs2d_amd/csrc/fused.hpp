@@ -597,9 +597,11 @@ __device__ __forceinline__ double lds_gradient(int q, int id, int N, const doubl
 // as k_grad_lsqc, so the gradients are bit-identical to it.  With cons, W
 // holds conservative states, converted as they are staged.
 constexpr int kGradRegs = 16;
-template <int LIM, int PMODE>
-// (without the limiter the kernel fits 72 registers: seven wavefronts per SIMD)
-__global__ __launch_bounds__(kBlock, LIM ? 1 : 7) void k_grad_lsqh(int N, int nb, int nl, const double *__restrict__ W,
+// BS cells (and threads) per block, with the lists built for blocks of BS
+// cells (without the limiter the kernel fits 72 registers: seven wavefronts
+// per SIMD)
+template <int LIM, int PMODE, int BS = kBlock>
+__global__ __launch_bounds__(BS, LIM ? 1 : 7) void k_grad_lsqh(int N, int nb, int nl, const double *__restrict__ W,
                                                       int cons, double gamma,
                                                       const int *__restrict__ ghoff, const int *__restrict__ ghlist,
                                                       const int *__restrict__ poff, const int *__restrict__ swid,
@@ -616,7 +618,7 @@ __global__ __launch_bounds__(kBlock, LIM ? 1 : 7) void k_grad_lsqh(int N, int nb
   double2 *lX = reinterpret_cast<double2 *>(lds + 4 * nl);   // [nl]
   const int lb = blist ? blist[blockIdx.x] : xcd_block(nb);
   if (lb >= (nb < 0 ? -nb : nb)) return;  // the whole workgroup leaves before the barrier
-  const int i0 = lb * kBlock, id = i0 + threadIdx.x;
+  const int i0 = lb * BS, id = i0 + threadIdx.x;
   const bool act = id < N;
   const int s = id / kSlice, wid = act ? swid[s] : 0;
   const unsigned *lc = lcode + (act ? poff[s] : 0) + threadIdx.x % kSlice;
@@ -636,14 +638,14 @@ __global__ __launch_bounds__(kBlock, LIM ? 1 : 7) void k_grad_lsqh(int N, int nb
     make_prim(cons, gamma, w);
     st4(lW, threadIdx.x, w);
   }
-  for (int q = threadIdx.x; q < nh; q += kBlock) {
+  for (int q = threadIdx.x; q < nh; q += BS) {
     const int j = ghlist[o2 + q];
     const double2 xy = cxy[j];
     double w[4];
     ld4(W, j, w);
-    lX[kBlock + q] = xy;
+    lX[BS + q] = xy;
     make_prim(cons, gamma, w);
-    st4(lW, kBlock + q, w);
+    st4(lW, BS + q, w);
   }
   __syncthreads();
   if (!act) return;

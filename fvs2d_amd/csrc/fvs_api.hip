@@ -1053,6 +1053,12 @@ struct fvs_ctx {
   unsigned *lcode = nullptr;
   int maxh = 0;  // largest halo of a block
   long nghl = 0, nlcode = 0;
+  // the same lists for blocks of kSmallBlock cells (single meshes), used by
+  // the LSQ gradient when small_grad is set (autotuned)
+  int *ghoff_s = nullptr, *ghlist_s = nullptr;
+  unsigned *lcode_s = nullptr;
+  int maxh_s = 0;
+  bool have_small_grad = false, small_grad = false;
   // batch mode: per-instance frozen flags (dfrz) while only some are frozen
   int *dfrz = nullptr;
   bool frz_some = false;
@@ -1369,54 +1375,82 @@ int setup_stage(fvs_ctx &C, const std::vector<int> &boff, const std::vector<int>
 // Lists of the staged-halo gradient k_grad_lsqh (see the fvs_ctx fields).
 // Local entries keep the order of the global stencil, so the gradients are
 // bit-identical to k_grad_lsqc's.  Enabled (C.halo) when every block's local
-// cells number at most 4095 and their states and centres fit 64 KiB of LDS.
+// cells number at most 4095 and their states and centres fit 64 KiB of LDS;
+// built again for blocks of kSmallBlock cells on single meshes of at least
+// 64 such blocks (C.have_small_grad).
 int setup_grad_halo(fvs_ctx &C, const std::vector<int> &off, const std::vector<int> &wid,
                     const std::vector<unsigned> &code) {
-  const int N = C.N, nbk = nblk(N);
-  std::vector<int> ghoff(nbk + 1, 0), ghlist, halo, poff(wid.size() + 1, 0);
+  const int N = C.N;
+  std::vector<int> poff(wid.size() + 1, 0);
   for (size_t sg = 0; sg < wid.size(); ++sg) poff[sg + 1] = poff[sg] + (wid[sg] + 1) / 2 * kSlice;  // <= off
-  std::vector<unsigned> lcode(size_t(poff.back()), 0u);
+  // the lists for blocks of BS cells; ok stays false when they do not fit
+  auto build = [&](int BS, std::vector<int> &ghoff, std::vector<int> &ghlist, std::vector<unsigned> &lcode,
+                   int &maxh, bool &ok) -> int {
+    const int nbk = nblk(N, BS);
+    ghoff.assign(nbk + 1, 0);
+    ghlist.clear();
+    lcode.assign(size_t(poff.back()), 0u);
+    maxh = 0;
+    ok = false;
+    std::vector<int> halo;
+    for (int b = 0; b < nbk; ++b) {
+      const int i0 = b * BS, i1 = std::min(N, i0 + BS);
+      halo.clear();
+      for (int i = i0; i < i1; ++i) {
+        const int sg = i / kSlice, l = i % kSlice;
+        for (int k = 0; k < wid[sg]; ++k) {
+          const int j = int(code[off[sg] + long(k) * kSlice + l] & 0x07ffffffu);
+          if (j < i0 || j >= i1) halo.push_back(j);
+        }
+      }
+      std::sort(halo.begin(), halo.end());
+      halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+      if (BS + halo.size() > 4095) return 0;
+      for (int i = i0; i < i1; ++i) {
+        const int sg = i / kSlice, l = i % kSlice;
+        for (int k = 0; k < wid[sg]; ++k) {
+          const long e = off[sg] + long(k) * kSlice + l;
+          const int j = int(code[e] & 0x07ffffffu);
+          int loc = j - i0;
+          if (j < i0 || j >= i1) loc = BS + int(std::lower_bound(halo.begin(), halo.end(), j) - halo.begin());
+          if (loc < 0 || loc >= BS + int(halo.size()) || (loc < BS && i0 + loc >= i1))
+            return fail(FVS_EINVAL, "halo set-up: stencil entry outside the block's local cells");
+          lcode[size_t(poff[sg]) + size_t(k / 2) * kSlice + l] |= ((unsigned(loc) << 4) | (code[e] >> 27))
+                                                                  << (16 * (k & 1));
+        }
+      }
+      ghlist.insert(ghlist.end(), halo.begin(), halo.end());
+      ghoff[b + 1] = int(ghlist.size());
+      maxh = std::max(maxh, int(halo.size()));
+    }
+    ok = size_t(6) * (BS + maxh) * sizeof(double) <= 64 * 1024;
+    if (ghlist.empty()) ghlist.push_back(0);
+    return 0;
+  };
+  std::vector<int> ghoff, ghlist;
+  std::vector<unsigned> lcode;
   int maxh = 0;
-  for (int b = 0; b < nbk; ++b) {
-    const int i0 = b * kBlock, i1 = std::min(N, i0 + kBlock);
-    halo.clear();
-    for (int i = i0; i < i1; ++i) {
-      const int sg = i / kSlice, l = i % kSlice;
-      for (int k = 0; k < wid[sg]; ++k) {
-        const int j = int(code[off[sg] + long(k) * kSlice + l] & 0x07ffffffu);
-        if (j < i0 || j >= i1) halo.push_back(j);
-      }
-    }
-    std::sort(halo.begin(), halo.end());
-    halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
-    if (kBlock + halo.size() > 4095) return 0;
-    for (int i = i0; i < i1; ++i) {
-      const int sg = i / kSlice, l = i % kSlice;
-      for (int k = 0; k < wid[sg]; ++k) {
-        const long e = off[sg] + long(k) * kSlice + l;
-        const int j = int(code[e] & 0x07ffffffu);
-        int loc = j - i0;
-        if (j < i0 || j >= i1) loc = kBlock + int(std::lower_bound(halo.begin(), halo.end(), j) - halo.begin());
-        if (loc < 0 || loc >= kBlock + int(halo.size()) || (loc < kBlock && i0 + loc >= i1))
-          return fail(FVS_EINVAL, "halo set-up: stencil entry outside the block's local cells");
-        lcode[size_t(poff[sg]) + size_t(k / 2) * kSlice + l] |= ((unsigned(loc) << 4) | (code[e] >> 27))
-                                                                << (16 * (k & 1));
-      }
-    }
-    ghlist.insert(ghlist.end(), halo.begin(), halo.end());
-    ghoff[b + 1] = int(ghlist.size());
-    maxh = std::max(maxh, int(halo.size()));
-  }
-  if (size_t(6) * (kBlock + maxh) * sizeof(double) > 64 * 1024) return 0;
+  bool ok = false;
+  if (int rc = build(kBlock, ghoff, ghlist, lcode, maxh, ok)) return rc;
+  if (!ok) return 0;
   if (int rc = C.upload(C.ghoff, ghoff)) return rc;
-  if (ghlist.empty()) ghlist.push_back(0);
   if (int rc = C.upload(C.ghlist, ghlist)) return rc;
   if (int rc = C.upload(C.poff, poff)) return rc;
   if (int rc = C.upload(C.lcode, lcode)) return rc;
   C.maxh = maxh;
-  C.nghl = ghoff[nbk];
+  C.nghl = ghoff.back();
   C.nlcode = poff.back();
   C.halo = true;
+  if (C.ninst == 0 && N >= 64 * kSmallBlock) {
+    if (int rc = build(kSmallBlock, ghoff, ghlist, lcode, maxh, ok)) return rc;
+    if (ok) {
+      if (int rc = C.upload(C.ghoff_s, ghoff)) return rc;
+      if (int rc = C.upload(C.ghlist_s, ghlist)) return rc;
+      if (int rc = C.upload(C.lcode_s, lcode)) return rc;
+      C.maxh_s = maxh;
+      C.have_small_grad = true;
+    }
+  }
   return 0;
 }
 
@@ -1896,27 +1930,36 @@ int launch_gradient(fvs_ctx &C, bool with_limiter, const double *src = nullptr) 
   const bool ghalo = gg && lim && C.halo && C.use_halo;  // limited Green-Gauss, bounds from the staged halo
   const size_t glds = size_t(4) * (kBlock + C.maxh) * sizeof(double);
   if (C.opt.grad_method == 3 && C.halo && C.use_halo) {
+    // 128-cell blocks (single meshes only: batches keep the 256-cell blocks
+    // their instance masks are built for)
+    const bool small = C.small_grad && C.have_small_grad && !C.nact;
     if (C.nact) g = dim3(C.nact);
-    const int nb = nblk(N);
-    const int nl = kBlock + C.maxh;
+    const int nb = small ? nblk(N, kSmallBlock) : nblk(N);
+    if (small) g = dim3(xcd_grid(nb)), b = dim3(kSmallBlock);
+    const int nl = small ? kSmallBlock + C.maxh_s : kBlock + C.maxh;
     const size_t lds = size_t(6) * nl * sizeof(double);
-#define FVS_LSQH(L, PM)                                                                                          \
-  hipLaunchKernelGGL((k_grad_lsqh<L, PM>), g, b, lds, C.stream, N, xcd_arg(nb, C.xcd_grad), nl, src ? src : C.W, \
-                     src ? 1 : 0, C.opt.gamma, C.ghoff,                                                         \
-                     C.ghlist, C.poff, C.swid, C.lcode, C.cxy, C.ginv, C.shifts, C.opt.lsq_p, C.grad, C.phi, lt,     \
-                     C.nvrt, C.loff, C.vol, C.nact ? C.blist : nullptr, C.frz_some ? C.dinst_off : nullptr, C.dfrz,    \
-                     C.ninst)
-    if (lim) {
-      if (C.pmode == 0) FVS_LSQH(1, 0);
-      else if (C.pmode == 1) FVS_LSQH(1, 1);
-      else if (C.pmode == 2) FVS_LSQH(1, 2);
-      else FVS_LSQH(1, 3);
+    const int *ghoff = small ? C.ghoff_s : C.ghoff, *ghlist = small ? C.ghlist_s : C.ghlist;
+    const unsigned *lcode = small ? C.lcode_s : C.lcode;
+#define FVS_LSQH(L, PM, BS)                                                                                      \
+  hipLaunchKernelGGL((k_grad_lsqh<L, PM, BS>), g, b, lds, C.stream, N, xcd_arg(nb, C.xcd_grad), nl,             \
+                     src ? src : C.W, src ? 1 : 0, C.opt.gamma, ghoff, ghlist, C.poff, C.swid, lcode, C.cxy, C.ginv, \
+                     C.shifts, C.opt.lsq_p, C.grad, C.phi, lt, C.nvrt, C.loff, C.vol, C.nact ? C.blist : nullptr,   \
+                     C.frz_some ? C.dinst_off : nullptr, C.dfrz, C.ninst)
+#define FVS_LSQH_PM(L, BS)                     \
+  do {                                         \
+    if (C.pmode == 0) FVS_LSQH(L, 0, BS);      \
+    else if (C.pmode == 1) FVS_LSQH(L, 1, BS); \
+    else if (C.pmode == 2) FVS_LSQH(L, 2, BS); \
+    else FVS_LSQH(L, 3, BS);                   \
+  } while (0)
+    if (small) {
+      if (lim) FVS_LSQH_PM(1, kSmallBlock);
+      else FVS_LSQH_PM(0, kSmallBlock);
     } else {
-      if (C.pmode == 0) FVS_LSQH(0, 0);
-      else if (C.pmode == 1) FVS_LSQH(0, 1);
-      else if (C.pmode == 2) FVS_LSQH(0, 2);
-      else FVS_LSQH(0, 3);
+      if (lim) FVS_LSQH_PM(1, kBlock);
+      else FVS_LSQH_PM(0, kBlock);
     }
+#undef FVS_LSQH_PM
 #undef FVS_LSQH
   } else if (C.lsqc) {
     if (C.nact) g = dim3(C.nact);
@@ -3225,15 +3268,19 @@ int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) {
     ms = C->last_ms / nsteps;
     return 0;
   };
-  bool *knobs[6] = {&C->xcd_grad, &C->xcd_flux, &C->xcd_upd, &C->xcd_fused, &C->use_halo, &C->small_blocks};
-  const bool used[6] = {C->opt.recon != 1, !C->fused, !C->fused, C->fused && !C->stage,
-                        C->halo && !C->stage && C->opt.recon != 1, C->have_small && C->fused && !C->stage};
+  bool *knobs[7] = {&C->xcd_grad, &C->xcd_flux,  &C->xcd_upd,   &C->xcd_fused,
+                    &C->use_halo, &C->small_blocks, &C->small_grad};
+  const bool lsq_halo = C->halo && !C->stage && C->opt.recon != 1 && C->opt.grad_method == 3;
+  const bool used[7] = {C->opt.recon != 1, !C->fused, !C->fused, C->fused && !C->stage,
+                        C->halo && !C->stage && C->opt.recon != 1, C->have_small && C->fused && !C->stage,
+                        lsq_halo && C->have_small_grad};
   double best = 0.0;
   // warm-up: the first few dozen launches after the device was idle run
   // slower (measured up to 15% for the fused flux kernel)
   for (int w = 0; w < 3 && !rc; ++w) rc = trial(best);
-  for (int k = 0; k < 6 && !rc; ++k) {
+  for (int k = 0; k < 7 && !rc; ++k) {
     if (!used[k]) continue;
+    if (k == 6 && !C->use_halo) continue;  // 128-cell blocks exist for the staged-halo gradient only
     // the current and the flipped setting alternately, twice each, so that a
     // drift of the device's speed during the search does not decide
     double cur = 0.0, flip = 0.0;
@@ -3264,6 +3311,7 @@ int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) {
   if (choice4) {
     for (int k = 0; k < 4; ++k) choice4[k] = *knobs[k] ? 1 : 0;
     if (C->halo && C->use_halo) choice4[0] |= 2;
+    if (C->halo && C->use_halo && C->have_small_grad && C->small_grad) choice4[0] |= 4;
     if (C->small_blocks) choice4[3] |= 4;
   }
   if (best_ms) *best_ms = best;
@@ -3272,8 +3320,10 @@ int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) {
 
 int fvs_set_launch_choices(fvs_ctx *C, const int *choice4) {
   if (!C || !choice4) return fail(FVS_EINVAL, "null argument");
-  const bool halo = (choice4[0] & 2) != 0, small = (choice4[3] & 4) != 0;
+  const bool halo = (choice4[0] & 2) != 0, small = (choice4[3] & 4) != 0, small_grad = (choice4[0] & 4) != 0;
   if (halo && !C->halo) return fail(FVS_EINVAL, "launch choices: no staged-halo gradient data for this context");
+  if (small_grad && !(halo && C->have_small_grad))
+    return fail(FVS_EINVAL, "launch choices: no 128-cell staged-halo gradient lists for this context");
   if (small && !(C->have_small && C->fused && !C->stage))
     return fail(FVS_EINVAL, "launch choices: no 128-cell block lists for this context");
   C->xcd_grad = (choice4[0] & 1) != 0;
@@ -3281,6 +3331,7 @@ int fvs_set_launch_choices(fvs_ctx *C, const int *choice4) {
   C->xcd_upd = (choice4[2] & 1) != 0;
   C->xcd_fused = (choice4[3] & 1) != 0;
   C->use_halo = halo;
+  C->small_grad = small_grad;
   C->small_blocks = small;
   C->launch_gen++;
   return 0;

@@ -464,8 +464,9 @@ class Solver:
         """Pick plain or XCD-aware workgroup numbering per kernel class (bit
         0), for the LSQ gradient or a limited Green-Gauss gradient's limiter
         whether it reads its stencil from a per-block halo staged in LDS (bit
-        1), and for the fused flux kernel 128- or 256-cell blocks (bit 2); the
-        state is left as it was.  Returns (choices, best ms per step)."""
+        1), and for the fused flux kernel and the staged-halo LSQ gradient 128-
+        or 256-cell blocks (bit 2); the state is left as it was.  Returns
+        (choices, best ms per step)."""
         ch = np.zeros(4, np.int32)
         best = np.zeros(1)
         check(self._L.fvs_autotune(self._h, nsteps, ptr(ch), ptr(best)))

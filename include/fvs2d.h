@@ -260,14 +260,15 @@ int fvs_batch_freeze_limiter(fvs_ctx *ctx, const int *frozen);
  * (each choice against the current setting, alternately, twice each, after
  * three warm-up trials) and keeps, per kernel class [gradient, face flux,
  * gather+update, fused flux+update], the faster of plain and XCD-aware
- * workgroup numbering, and
- * the faster of 256 and 128 cells per workgroup for the fused flux + update
- * kernel (single meshes of at least 8,192 cells).  The state is restored
+ * workgroup numbering, and the faster of 256 and 128 cells per workgroup for
+ * the fused flux + update kernel and for the staged-halo LSQ gradient
+ * (single meshes of at least 8,192 cells).  The state is restored
  * afterwards.  choice4 (may be NULL) receives the choices (bit 0 set =
  * XCD-aware; for the gradient, bit 1 set = the LSQ gradient, or a limited
  * Green-Gauss gradient's limiter, reads its stencil from a per-block halo
- * staged in LDS, the other candidate it tries; for the fused kernel, bit 2
- * set = 128-cell workgroups), best_ms (may be NULL) the chosen setting's mean
+ * staged in LDS, the other candidate it tries, and bit 2 set = that LSQ
+ * gradient on 128-cell workgroups; for the fused kernel, bit 2 set = 128-cell
+ * workgroups), best_ms (may be NULL) the chosen setting's mean
  * time per step over its last two trials. */
 int fvs_autotune(fvs_ctx *ctx, int nsteps, int *choice4, double *best_ms);
 /* Applies launch choices in fvs_autotune's choice4 encoding (for instance

@@ -456,12 +456,15 @@ def test_lsq_general_weight_exponent(p, nn):
     assert rel_err(out[0], orc.cvar) <= 1e-10
 
 
+@pytest.mark.parametrize("choices", [{"gradient": 2, "fused": 4}, {"gradient": 6}, {"gradient": 6, "fused": 4}],
+                         ids=["flux128", "grad128", "both128"])
 @pytest.mark.parametrize("case", ["vortex_ssp3", "naca_steady_venk"])
-def test_small_flux_blocks_bitwise(case):
-    """The fused flux + update kernel on 128-cell workgroups (a launch shape
-    the autotuner tries) gives the states of the 256-cell kernel bit for bit
-    (each face flux and each cell's gather are the same arithmetic); the
-    residual norms sum other block partials, so they agree to rounding."""
+def test_small_flux_blocks_bitwise(case, choices):
+    """The fused flux + update kernel and the staged-halo LSQ gradient on
+    128-cell workgroups (launch shapes the autotuner tries) give the states of
+    the 256-cell kernels bit for bit (each face flux, each cell's gather and
+    each gradient are the same arithmetic); the residual norms sum other block
+    partials when the flux blocks differ, so they agree to rounding."""
     from fvs2d_amd import cases
     if case == "vortex_ssp3":
         m, sch = mesh("periodic", 96), vortex_scheme(dt=0.005, reorder=1)
@@ -470,7 +473,7 @@ def test_small_flux_blocks_bitwise(case):
         m, sch = cases.naca(128, 64, cfl=1.0)
         n = 30
     a, b = Solver(m, sch), Solver(m, sch)
-    b.set_launch_choices({"gradient": 2, "fused": 4})
+    b.set_launch_choices(choices)
     a.set_launch_choices({"gradient": 2})
     for s in (a, b):
         s.initialize_solution(1, 0.0)
