@@ -269,7 +269,14 @@ __device__ __forceinline__ void block_update_load(int N, int i, int gw, const Up
   for (int k = 0; k < 4; ++k) in.sv[k] = (k < gw) ? lgat[long(k) * N + i] : 0;
   in.V = vol[i];
   in.dl = (A.steady && !A.first) ? dtl[i] : 0.0;
-  if (!(x_given && A.first)) ld4(Ua, i, in.u0);
+  // a stage of the two-register form with a = 0 after the first does not
+  // read the step-start state (exact zeros keep a * u0 exact)
+  const bool u0_unused = A.mode == M_SSP3 && A.a == 0.0 && !A.first;
+  if (u0_unused) {
+    for (int v = 0; v < 4; ++v) in.u0[v] = 0.0;
+  } else if (!(x_given && A.first)) {
+    ld4(Ua, i, in.u0);
+  }
   if (A.mode == M_SSP3) {
     if (!x_given) {
       if (A.first) for (int v = 0; v < 4; ++v) in.x[v] = in.u0[v];
@@ -775,6 +782,26 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(StageArgs a, Upd A, con
   stage_block<LIM, PMODE>(a, lb, A, Win, Wout, d2out);
 }
 
+// Thread t's share of an instance's squared increments: cells a + t, a + t
+// + kBlock, ... < b added in that order, with kInstLoads loads in flight
+// before their adds (an instance holds up to 2^18 cells, one workgroup walks
+// it, and the loads' latency would otherwise be paid once per cell)
+constexpr int kInstLoads = 8;
+__device__ __forceinline__ void inst_share(int a, int b, int t, const double *__restrict__ d2, double s[4]) {
+  for (int i0 = a + t; i0 < b; i0 += kInstLoads * kBlock) {
+    double q[kInstLoads][4];
+#pragma unroll
+    for (int u = 0; u < kInstLoads; ++u) {
+      const int i = i0 + u * kBlock;
+      if (i < b) ld4(d2, i, q[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kInstLoads; ++u)
+      if (i0 + u * kBlock < b)
+        for (int v = 0; v < 4; ++v) s[v] = s[v] + q[u][v];
+  }
+}
+
 // per-instance residual norms of one step from the per-cell squared
 // increments: one workgroup per instance, fixed-shape tree (deterministic)
 __global__ __launch_bounds__(kBlock) void k_inst_norms(int ninst, const int *__restrict__ off,
@@ -785,11 +812,7 @@ __global__ __launch_bounds__(kBlock) void k_inst_norms(int ninst, const int *__r
   if (k >= ninst) return;
   const int a = off[k], b = off[k + 1];
   double s[4] = {0, 0, 0, 0};
-  for (int i = a + threadIdx.x; i < b; i += kBlock) {
-    double q[4];
-    ld4(d2, i, q);
-    for (int v = 0; v < 4; ++v) s[v] = s[v] + q[v];
-  }
+  inst_share(a, b, threadIdx.x, d2, s);
   block_sum4(s, red);
   if (threadIdx.x == 0) {
     double *o = out + 4 * (long(ctr ? ctr[0] : 0) * ninst + k);
@@ -924,11 +947,7 @@ __device__ __forceinline__ void inst_norm_block(int k, const int *__restrict__ o
   const int a = off[k], b = off[k + 1];
   if (t < kBlock) {
     double s[4] = {0, 0, 0, 0};
-    for (int i = a + t; i < b; i += kBlock) {
-      double q[4];
-      ld4(d2, i, q);
-      for (int v = 0; v < 4; ++v) s[v] = s[v] + q[v];
-    }
+    inst_share(a, b, t, d2, s);
     for (int v = 0; v < 4; ++v) red[v * kBlock + t] = s[v];
   }
   __syncthreads();

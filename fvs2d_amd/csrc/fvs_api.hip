@@ -2235,7 +2235,7 @@ static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, in
   *out = nullptr;
   if (opt->recon < 1 || opt->recon > 3) return fail(FVS_EINVAL, "recon must be 1, 2 or 3");
   if (opt->limiter < 0 || opt->limiter > 3) return fail(FVS_EINVAL, "limiter must be 0..3");
-  if (opt->rk_scheme < 0 || opt->rk_scheme > 2) return fail(FVS_EINVAL, "unknown rk_scheme");
+  if (opt->rk_scheme < 0 || opt->rk_scheme > 3) return fail(FVS_EINVAL, "unknown rk_scheme");
   if (opt->rk_scheme == FVS_RK_CLASSIC && (opt->rk_order < 1 || opt->rk_order > 4))
     return fail(FVS_EINVAL, "rk_order must be 1..4");
   if (!(opt->gamma > 1.0)) return fail(FVS_EINVAL, "gamma must exceed 1");
@@ -2667,7 +2667,7 @@ bool cons_stages(const fvs_ctx &C) {
   // staged halo (where the stage states are converted once, as staged)
   const bool gg = (C.opt.grad_method == 1 || C.opt.grad_method == 2) && C.opt.kernel_path != 3 &&
                   (C.opt.limiter == 0 || (C.halo && C.use_halo));
-  return C.fused && !C.stage && C.opt.rk_scheme == FVS_RK_SSP3 &&
+  return C.fused && !C.stage && (C.opt.rk_scheme == FVS_RK_SSP3 || C.opt.rk_scheme == FVS_RK_SSP42_SO) &&
          (C.opt.recon == 1 || (C.opt.grad_method == 3 && C.halo && C.use_halo) || gg);
 }
 
@@ -2698,6 +2698,15 @@ Upd stage_upd(const fvs_options &o, int k, double told, double &ts) {
     A.b = ssp3_b[k];
     A.h = o.steady ? 1.0 : dt;
     ts = (k == 0) ? told : (k == 1 ? told + dt : told + 0.5 * dt);
+  } else if (o.rk_scheme == FVS_RK_SSP42_SO) {
+    // the SSP-RK(4,2) stages of the reference in two-register form, run by
+    // the SSP-RK3 update: u_k = a u_0 + b (u_{k-1} + h R)
+    const double a42[4] = {0.0, 0.0, 0.0, 0.25}, b42[4] = {1.0, 1.0, 1.0, 0.75};
+    A.mode = M_SSP3;
+    A.a = a42[k];
+    A.b = b42[k];
+    A.h = o.steady ? 1.0 / 3.0 : dt / 3.0;
+    ts = told + dts[k];
   } else if (o.rk_scheme == FVS_RK_SSP42) {
     A.mode = M_SSP42;
     A.coef = 1.0;
@@ -2725,8 +2734,11 @@ int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {
     if (C.stage) {
       rc = launch_stage(C, A, ts);
     } else if (cons_stages(C)) {
-      double *const in[3] = {C.U0, C.Ub, C.Uc}, *const out[3] = {C.Ub, C.Uc, C.U0};
-      rc = launch_flux_update(C, A, ts, in[k], out[k]);
+      // the stage input and output states (a stage's output never
+      // overwrites a state its neighbouring blocks still read)
+      double *const in3[3] = {C.U0, C.Ub, C.Uc}, *const out3[3] = {C.Ub, C.Uc, C.U0};
+      double *const in4[4] = {C.U0, C.Ub, C.Uc, C.Ub}, *const out4[4] = {C.Ub, C.Uc, C.Ub, C.U0};
+      rc = nst == 3 ? launch_flux_update(C, A, ts, in3[k], out3[k]) : launch_flux_update(C, A, ts, in4[k], out4[k]);
     } else if (C.fused) {
       rc = launch_flux_update(C, A, ts);
     } else {
@@ -2744,9 +2756,11 @@ int launch_step(fvs_ctx &C, double told, double *dn, double *dinst) {
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kNormThreads), 0, C.stream, nparts, C.N, C.part, dn, C.dctr,
                        inst ? 0 : 1);
   }
-  if (inst)
+  if (inst) {
+    ProfScope ps(C, KC_NORM);
     hipLaunchKernelGGL(k_inst_norms, dim3(C.ninst), dim3(kBlock), 0, C.stream, C.ninst, C.dinst_off, C.d2buf, C.dctr,
                        dinst);
+  }
   if (C.nact || inst) hipLaunchKernelGGL(k_counter, dim3(1), dim3(1), 0, C.stream, C.dctr, 1, 1);
   HIPC(hipGetLastError());
   return 0;
@@ -3203,6 +3217,9 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
     // stage state; on the first stage also its step-start state), the
     // step-start state on the other stages, the new state; no W write
     double st3 = (32 + 64 + 64) / 3.0;
+    // two-register SSP-RK(4,2): the step-start state is read on the last
+    // stage only
+    if (C->opt.rk_scheme == FVS_RK_SSP42_SO) st3 = (32 + 32 + 32 + 64) / 4.0;
     if (C->opt.steady) st3 += (8.0 + 8.0 * (nst - 1)) / nst;
     bfu = nlisted * (8 + 56) + N * (32 + 64 + (phi1 ? 0 : 8)) + N * (C->GW * 2.0 + 8 + st3);
   }
