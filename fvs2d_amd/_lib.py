@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvs2d_hip.so")
 
 BC_FREESTREAM, BC_SLIP_WALL, BC_DIRICHLET, BC_SOLID_WALL = 0, 1, 2, 3
-RK_CLASSIC, RK_SSP42, RK_SSP3 = 0, 1, 2
+RK_CLASSIC, RK_SSP42, RK_SSP3, RK_SSP42_SO = 0, 1, 2, 3
 
 # every entry point of include/fvs2d.h
 EXPORTS = (

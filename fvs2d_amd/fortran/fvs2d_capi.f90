@@ -9,7 +9,7 @@ module fvs2d_capi
   implicit none
 
   integer(c_int), parameter :: FVS_BC_FREESTREAM = 0, FVS_BC_SLIP_WALL = 1, FVS_BC_DIRICHLET = 2
-  integer(c_int), parameter :: FVS_RK_CLASSIC = 0, FVS_RK_SSP42 = 1, FVS_RK_SSP3 = 2
+  integer(c_int), parameter :: FVS_RK_CLASSIC = 0, FVS_RK_SSP42 = 1, FVS_RK_SSP3 = 2, FVS_RK_SSP42_SO = 3
 
   ! fvs_options (include/fvs2d.h), field for field
   type, bind(C) :: fvs_options

@@ -73,9 +73,14 @@ class Scheme:
         return o
 
     def oracle_kwargs(self) -> dict:
+        """The scheme for the oracle, which evaluates SSP-RK(4,2) in the
+        reference's form only: the two-register form (RK_SSP42_SO) maps to
+        it (the same method, other rounding)."""
         d = dataclasses.asdict(self)
         d.pop("reorder")
         d.pop("kernel_path")
+        if d["rk_scheme"] == _lib.RK_SSP42_SO:
+            d["rk_scheme"] = _lib.RK_SSP42
         return d
 
 

@@ -33,8 +33,15 @@ typedef struct fvs_ctx fvs_ctx;
 enum { FVS_BC_FREESTREAM = 0, FVS_BC_SLIP_WALL = 1, FVS_BC_DIRICHLET = 2, FVS_BC_SOLID_WALL = 3 };
 
 /* time integrators: src/runge_kutta.f90:94-114; FVS_RK_SSP3 has no reference
- * counterpart (Shu-Osher SSP-RK3) */
-enum { FVS_RK_CLASSIC = 0, FVS_RK_SSP42 = 1, FVS_RK_SSP3 = 2 };
+ * counterpart (Shu-Osher SSP-RK3).  FVS_RK_SSP42_SO is the reference's
+ * SSP-RK(4,2) (rk_coef 1, stage steps h/3, h/3, h/3, h/4,
+ * src/runge_kutta.f90:56-60) evaluated in its two-register Shu-Osher form
+ * u_k = u_{k-1} + h/3 R(u_{k-1}) (k = 1..3), u_4 = u_0/4 + 3/4 (u_3 + h/3
+ * R(u_3)) instead of the reference's u_k = u_0 + h c_k (R + sum of earlier
+ * R): the same method, equal in exact arithmetic, with its own rounding and
+ * no accumulator, so its stages can run on conservative states (no
+ * reference counterpart for the form) */
+enum { FVS_RK_CLASSIC = 0, FVS_RK_SSP42 = 1, FVS_RK_SSP3 = 2, FVS_RK_SSP42_SO = 3 };
 
 /* error codes */
 enum { FVS_OK = 0, FVS_EINVAL = -1, FVS_EHIP = -2, FVS_ESTATE = -3, FVS_ENOMEM = -4 };

@@ -77,7 +77,7 @@ def test_argument_checks_before_the_device():
         return L.fvs_create(C.byref(h), m.nnodes, P(xy), m.ntri, 0, P(cn), None, len(nb_), P(nb_), P(ty),
                             P(cells), C.byref(o))
 
-    for kw, msg in ((dict(recon=0), b"recon"), (dict(limiter=4), b"limiter"), (dict(rk_scheme=3), b"rk_scheme"),
+    for kw, msg in ((dict(recon=0), b"recon"), (dict(limiter=4), b"limiter"), (dict(rk_scheme=4), b"rk_scheme"),
                     (dict(rk_scheme=0, rk_order=5), b"rk_order"), (dict(gamma=1.0), b"gamma")):
         assert create(**kw) == -1 and msg in L.fvs_last_error(), kw
     o = Scheme().to_c()
