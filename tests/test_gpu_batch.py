@@ -207,3 +207,17 @@ def test_queued_batch_iterations_match_all_active():
     all_on, queued = run(None), run(2)
     assert np.all(all_on > 0)
     assert np.array_equal(all_on, queued)
+
+
+def test_batch_driver_queue_and_give_up():
+    """The batch driver (python -m fvs2d_amd.cases batch) with the give-up
+    rule and at most two instances stepping at once gives every instance the
+    outcome and iteration count of the run with all instances stepping."""
+    kw = dict(ninst=6, seed=1, max_steps=30000, drop=1e-6, cfl=1.25, freeze_after=30000,
+              resolutions=((32, 16), (64, 32)), freeze_stall=(2000, 0.1), stall_scale=True, give_up=(8000, 0.5))
+    a = cases.run_batch(**kw)
+    q = cases.run_batch(**kw, max_active=2)
+    assert a["instances"] == q["instances"] == 6 - len(a["rejected"])
+    assert a["converged"] >= 4
+    assert [p["iterations"] for p in a["per_instance"]] == [p["iterations"] for p in q["per_instance"]]
+    assert a["gave_up"] == q["gave_up"]
