@@ -1200,6 +1200,29 @@ std::vector<int> morton_order(const MeshPrep &m, const std::vector<int> &inst) {
 
 // sliced-ELL packing of a CSR stencil (cells in device order); false when
 // the padded entry count does not fit the 32-bit slice offsets
+// worker threads for host-side preparation: the cores this process may use,
+// at most 16
+int host_threads() {
+  const unsigned n = std::thread::hardware_concurrency();
+  return int(std::max(1u, std::min(16u, n)));
+}
+
+// [0, n) in contiguous parts of at least grain items, one per worker thread:
+// parallel_parts gives their number, parallel_for runs fn(part, lo, hi) on
+// each (fn must touch disjoint data per part)
+long parallel_parts(long n, long grain) { return std::max(1L, std::min<long>(host_threads(), n / grain)); }
+template <class F>
+void parallel_for(long n, long grain, F fn) {
+  const long nt = parallel_parts(n, grain);
+  if (nt == 1) {
+    fn(0L, 0L, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (long t = 0; t < nt; ++t) th.emplace_back([&fn, n, nt, t] { fn(t, n * t / nt, n * (t + 1) / nt); });
+  for (auto &x : th) x.join();
+}
+
 bool pack_sell(int N, const std::vector<int> &perm, const std::vector<int> &iperm, const std::vector<int> &ptr,
                const std::vector<int> &col, const double *val2, std::vector<int> &off, std::vector<int> &wid,
                std::vector<int> &nb, std::vector<double2> &cw) {
@@ -1220,21 +1243,23 @@ bool pack_sell(int N, const std::vector<int> &perm, const std::vector<int> &iper
   }
   nb.assign(off[ns], 0);
   cw.assign(val2 ? off[ns] : 0, make_double2(0.0, 0.0));
-  for (int s = 0; s < ns; ++s)
-    for (int l = 0; l < kSlice; ++l) {
-      int i = s * kSlice + l;
-      for (int k = 0; k < wid[s]; ++k) {
-        long e = off[s] + long(k) * kSlice + l;
-        if (i >= N) { nb[e] = 0; continue; }
-        int o = perm[i], b = ptr[o], n = ptr[o + 1] - b;
-        if (k < n) {
-          nb[e] = iperm[col[b + k]];
-          if (val2) cw[e] = make_double2(val2[2 * size_t(b + k)], val2[2 * size_t(b + k) + 1]);
-        } else {
-          nb[e] = i;  // padding: the cell itself with zero weight
+  parallel_for(ns, 64, [&](long, long s0, long s1) {
+    for (long s = s0; s < s1; ++s)
+      for (int l = 0; l < kSlice; ++l) {
+        int i = int(s) * kSlice + l;
+        for (int k = 0; k < wid[s]; ++k) {
+          long e = off[s] + long(k) * kSlice + l;
+          if (i >= N) { nb[e] = 0; continue; }
+          int o = perm[i], b = ptr[o], n = ptr[o + 1] - b;
+          if (k < n) {
+            nb[e] = iperm[col[b + k]];
+            if (val2) cw[e] = make_double2(val2[2 * size_t(b + k)], val2[2 * size_t(b + k) + 1]);
+          } else {
+            nb[e] = i;  // padding: the cell itself with zero weight
+          }
         }
       }
-    }
+  });
   return true;
 }
 
@@ -1392,37 +1417,58 @@ int setup_grad_halo(fvs_ctx &C, const std::vector<int> &off, const std::vector<i
     lcode.assign(size_t(poff.back()), 0u);
     maxh = 0;
     ok = false;
-    std::vector<int> halo;
-    for (int b = 0; b < nbk; ++b) {
-      const int i0 = b * BS, i1 = std::min(N, i0 + BS);
-      halo.clear();
-      for (int i = i0; i < i1; ++i) {
-        const int sg = i / kSlice, l = i % kSlice;
-        for (int k = 0; k < wid[sg]; ++k) {
-          const int j = int(code[off[sg] + long(k) * kSlice + l] & 0x07ffffffu);
-          if (j < i0 || j >= i1) halo.push_back(j);
+    // blocks in contiguous parts on the host's cores (a block's cells are
+    // whole 64-cell slices, so parts write disjoint code words); each part's
+    // halo lists in block order, joined afterwards.  status: 1 = a block has
+    // too many local cells, 2 = a stencil entry outside the local cells
+    const long np = parallel_parts(nbk, 64);
+    std::vector<std::vector<int>> ph(np);
+    std::vector<int> pmax(np, 0), status(np, 0);
+    parallel_for(nbk, 64, [&](long t, long b0, long b1) {
+      std::vector<int> halo;
+      for (long b = b0; b < b1 && !status[t]; ++b) {
+        const int i0 = int(b) * BS, i1 = std::min(N, i0 + BS);
+        halo.clear();
+        for (int i = i0; i < i1; ++i) {
+          const int sg = i / kSlice, l = i % kSlice;
+          for (int k = 0; k < wid[sg]; ++k) {
+            const int j = int(code[off[sg] + long(k) * kSlice + l] & 0x07ffffffu);
+            if (j < i0 || j >= i1) halo.push_back(j);
+          }
         }
-      }
-      std::sort(halo.begin(), halo.end());
-      halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
-      if (BS + halo.size() > 4095) return 0;
-      for (int i = i0; i < i1; ++i) {
-        const int sg = i / kSlice, l = i % kSlice;
-        for (int k = 0; k < wid[sg]; ++k) {
-          const long e = off[sg] + long(k) * kSlice + l;
-          const int j = int(code[e] & 0x07ffffffu);
-          int loc = j - i0;
-          if (j < i0 || j >= i1) loc = BS + int(std::lower_bound(halo.begin(), halo.end(), j) - halo.begin());
-          if (loc < 0 || loc >= BS + int(halo.size()) || (loc < BS && i0 + loc >= i1))
-            return fail(FVS_EINVAL, "halo set-up: stencil entry outside the block's local cells");
-          lcode[size_t(poff[sg]) + size_t(k / 2) * kSlice + l] |= ((unsigned(loc) << 4) | (code[e] >> 27))
-                                                                  << (16 * (k & 1));
+        std::sort(halo.begin(), halo.end());
+        halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+        if (BS + halo.size() > 4095) {
+          status[t] = 1;
+          break;
         }
+        for (int i = i0; i < i1 && !status[t]; ++i) {
+          const int sg = i / kSlice, l = i % kSlice;
+          for (int k = 0; k < wid[sg]; ++k) {
+            const long e = off[sg] + long(k) * kSlice + l;
+            const int j = int(code[e] & 0x07ffffffu);
+            int loc = j - i0;
+            if (j < i0 || j >= i1) loc = BS + int(std::lower_bound(halo.begin(), halo.end(), j) - halo.begin());
+            if (loc < 0 || loc >= BS + int(halo.size()) || (loc < BS && i0 + loc >= i1)) {
+              status[t] = 2;
+              break;
+            }
+            lcode[size_t(poff[sg]) + size_t(k / 2) * kSlice + l] |= ((unsigned(loc) << 4) | (code[e] >> 27))
+                                                                    << (16 * (k & 1));
+          }
+        }
+        ph[t].insert(ph[t].end(), halo.begin(), halo.end());
+        ghoff[b + 1] = int(halo.size());
+        pmax[t] = std::max(pmax[t], int(halo.size()));
       }
-      ghlist.insert(ghlist.end(), halo.begin(), halo.end());
-      ghoff[b + 1] = int(ghlist.size());
-      maxh = std::max(maxh, int(halo.size()));
-    }
+    });
+    const int worst = *std::max_element(status.begin(), status.end());
+    if (worst == 2) return fail(FVS_EINVAL, "halo set-up: stencil entry outside the block's local cells");
+    if (worst == 1) return 0;
+    for (int b = 0; b < nbk; ++b) ghoff[b + 1] += ghoff[b];
+    ghlist.reserve(size_t(ghoff[nbk]));
+    for (auto &v : ph) ghlist.insert(ghlist.end(), v.begin(), v.end());
+    maxh = *std::max_element(pmax.begin(), pmax.end());
     ok = size_t(6) * (BS + maxh) * sizeof(double) <= 64 * 1024;
     if (ghlist.empty()) ghlist.push_back(0);
     return 0;
@@ -1477,28 +1523,42 @@ int setup_device(fvs_ctx &C) {
   std::vector<int2> fic(Fi);
   std::vector<double> fig(8 * size_t(Fi), 0.0);
   std::vector<int> slot_of_edge(m.nedges, -1);
-  for (int f = 0; f < Fi; ++f) {
-    const int e = m.edge_intr[f], c1 = m.ec1[e], c2 = m.ec2[e];
-    slot_of_edge[e] = f;
-    fic[f] = make_int2(IP[c1], IP[c2]);
-    const double xf = m.ex[e], yf = m.ey[e];
-    const double xr = m.cx[c2] + m.esx[e], yr = m.cy[c2] + m.esy[e];
-    double *g = &fig[8 * size_t(f)];
-    g[0] = m.enx[e]; g[1] = m.eny[e]; g[2] = m.ea[e];
-    g[3] = xf - m.cx[c1]; g[4] = yf - m.cy[c1];
-    g[5] = xf - xr; g[6] = yf - yr;
-  }
+  parallel_for(Fi, 4096, [&](long, long f0, long f1) {
+    for (long f = f0; f < f1; ++f) {
+      const int e = m.edge_intr[f], c1 = m.ec1[e], c2 = m.ec2[e];
+      slot_of_edge[e] = int(f);
+      fic[f] = make_int2(IP[c1], IP[c2]);
+      const double xf = m.ex[e], yf = m.ey[e];
+      const double xr = m.cx[c2] + m.esx[e], yr = m.cy[c2] + m.esy[e];
+      double *g = &fig[8 * size_t(f)];
+      g[0] = m.enx[e]; g[1] = m.eny[e]; g[2] = m.ea[e];
+      g[3] = xf - m.cx[c1]; g[4] = yf - m.cy[c1];
+      g[5] = xf - xr; g[6] = yf - yr;
+    }
+  });
   if (int rc = C.upload(C.fi_cells, fic)) return rc;
   if (int rc = C.upload(C.fi_geo, fig)) return rc;
   // boundary faces: per boundary, edge i paired with listed cell i
   const int Fb = C.Fb = int(m.bnd_edges.size());
   std::vector<int> fbc(Fb), fbt(Fb), wall;
   std::vector<double> fbg(8 * size_t(Fb), 0.0);
-  std::vector<std::vector<int>> cell_faces(N);  // reference order
+  // each cell's signed face slots (reference cell order, CSR): its interior
+  // faces in ascending slot, then its boundary faces in ascending slot
+  std::vector<int> cf_ptr(size_t(N) + 1, 0), cf_fill;
   for (int f = 0; f < Fi; ++f) {
     const int e = m.edge_intr[f];
-    cell_faces[m.ec1[e]].push_back(f + 1);
-    cell_faces[m.ec2[e]].push_back(-(f + 1));
+    ++cf_ptr[m.ec1[e] + 1];
+    ++cf_ptr[m.ec2[e] + 1];
+  }
+  for (int ib = 0; ib < m.nb; ++ib)
+    for (int k = 0; k < m.bnd_nedges[ib]; ++k) ++cf_ptr[m.bnd_cells[m.bnd_cptr[ib] + k] + 1];
+  for (int c = 0; c < N; ++c) cf_ptr[c + 1] += cf_ptr[c];
+  std::vector<int> cell_faces(static_cast<size_t>(cf_ptr[N]));
+  cf_fill.assign(cf_ptr.begin(), cf_ptr.end() - 1);
+  for (int f = 0; f < Fi; ++f) {
+    const int e = m.edge_intr[f];
+    cell_faces[cf_fill[m.ec1[e]]++] = f + 1;
+    cell_faces[cf_fill[m.ec2[e]]++] = -(f + 1);
   }
   int b = 0;
   for (int ib = 0; ib < m.nb; ++ib)
@@ -1511,7 +1571,7 @@ int setup_device(fvs_ctx &C) {
       g[0] = m.enx[e]; g[1] = m.eny[e]; g[2] = m.ea[e];
       g[3] = m.ex[e] - m.cx[c]; g[4] = m.ey[e] - m.cy[c];
       g[5] = m.ex[e]; g[6] = m.ey[e];
-      cell_faces[c].push_back(Fi + b + 1);
+      cell_faces[cf_fill[c]++] = Fi + b + 1;
       if (m.bnd_type[ib] == FVS_BC_SLIP_WALL) wall.push_back(b);
     }
   if (int rc = C.upload(C.fb_cell, fbc)) return rc;
@@ -1522,13 +1582,15 @@ int setup_device(fvs_ctx &C) {
   // gather table: interior faces are already in ascending edge id because
   // slots follow edge_intr; boundary slots follow them
   int gw = 1;
-  for (auto &v : cell_faces) gw = std::max(gw, int(v.size()));
+  for (int c = 0; c < N; ++c) gw = std::max(gw, cf_ptr[c + 1] - cf_ptr[c]);
   C.GW = gw;
   std::vector<int> gat(size_t(gw) * N, 0);
-  for (int i = 0; i < N; ++i) {
-    const auto &v = cell_faces[P[i]];
-    for (size_t k = 0; k < v.size(); ++k) gat[k * N + i] = v[k];
-  }
+  parallel_for(N, 4096, [&](long, long i0, long i1) {
+    for (long i = i0; i < i1; ++i) {
+      const int c = P[i], a = cf_ptr[c], n = cf_ptr[c + 1] - a;
+      for (int k = 0; k < n; ++k) gat[size_t(k) * N + i] = cell_faces[size_t(a) + k];
+    }
+  });
   if (int rc = C.upload(C.gat, gat)) return rc;
   // per-block face lists for the fused flux + update kernel: every face slot
   // touching block b's cells, ascending (interior slots, then boundary slots),
@@ -1539,53 +1601,65 @@ int setup_device(fvs_ctx &C) {
   auto block_lists = [&](int bs, std::vector<int> &boff, std::vector<int> &bfaces, std::vector<short> &lgat,
                          int &maxf) {
     const int nbk = nblk(N, bs);
-    std::vector<int> loc;
     boff.assign(nbk + 1, 0);
     bfaces.clear();
     lgat.assign(size_t(gw) * N, 0);
-    maxf = 0;
-    for (int b = 0; b < nbk; ++b) {
-      loc.clear();
-      const int i0 = b * bs, i1 = std::min(N, i0 + bs);
-      for (int i = i0; i < i1; ++i)
-        for (int k = 0; k < gw; ++k) {
-          const int sg = gat[size_t(k) * N + i];
-          if (sg) loc.push_back((sg > 0 ? sg : -sg) - 1);
-        }
-      std::sort(loc.begin(), loc.end());
-      loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
-      for (int i = i0; i < i1; ++i)
-        for (int k = 0; k < gw; ++k) {
-          const int sg = gat[size_t(k) * N + i];
-          if (!sg) continue;
-          const int slot = (sg > 0 ? sg : -sg) - 1;
-          const int li = int(std::lower_bound(loc.begin(), loc.end(), slot) - loc.begin()) + 1;
-          lgat[size_t(k) * N + i] = short(sg > 0 ? li : -li);
-        }
-      bfaces.insert(bfaces.end(), loc.begin(), loc.end());
-      boff[b + 1] = int(bfaces.size());
-      maxf = std::max(maxf, int(loc.size()));
-    }
+    // blocks in contiguous parts on the host's cores; each part's lists in
+    // block order, joined afterwards
+    const long np = parallel_parts(nbk, 64);
+    std::vector<std::vector<int>> pf(np);
+    std::vector<int> pmax(np, 0);
+    parallel_for(nbk, 64, [&](long t, long b0, long b1) {
+      std::vector<int> loc;
+      for (long b = b0; b < b1; ++b) {
+        loc.clear();
+        const int i0 = int(b) * bs, i1 = std::min(N, i0 + bs);
+        for (int i = i0; i < i1; ++i)
+          for (int k = 0; k < gw; ++k) {
+            const int sg = gat[size_t(k) * N + i];
+            if (sg) loc.push_back((sg > 0 ? sg : -sg) - 1);
+          }
+        std::sort(loc.begin(), loc.end());
+        loc.erase(std::unique(loc.begin(), loc.end()), loc.end());
+        for (int i = i0; i < i1; ++i)
+          for (int k = 0; k < gw; ++k) {
+            const int sg = gat[size_t(k) * N + i];
+            if (!sg) continue;
+            const int slot = (sg > 0 ? sg : -sg) - 1;
+            const int li = int(std::lower_bound(loc.begin(), loc.end(), slot) - loc.begin()) + 1;
+            lgat[size_t(k) * N + i] = short(sg > 0 ? li : -li);
+          }
+        pf[t].insert(pf[t].end(), loc.begin(), loc.end());
+        boff[b + 1] = int(loc.size());
+        pmax[t] = std::max(pmax[t], int(loc.size()));
+      }
+    });
+    for (int b = 0; b < nbk; ++b) boff[b + 1] += boff[b];
+    bfaces.reserve(size_t(boff[nbk]));
+    for (auto &v : pf) bfaces.insert(bfaces.end(), v.begin(), v.end());
+    maxf = *std::max_element(pmax.begin(), pmax.end());
   };
   // the listed faces' records in list order: cells and 7 geometry planes
   auto upload_records = [&](const std::vector<int> &bfaces, int2 *&dlrec, double *&dlgeo) -> int {
     const size_t L = bfaces.size();
     std::vector<int2> lrec(L);
     std::vector<double> lgeo(7 * L);
-    for (size_t f = 0; f < L; ++f) {
-      const int sl = bfaces[f];
-      const double *g;
-      if (sl < Fi) {
-        lrec[f] = fic[sl];
-        g = &fig[8 * size_t(sl)];
-      } else {
-        const int bq = sl - Fi;
-        const int inst = C.ninst > 0 ? C.inst_of_real[P[fbc[bq]]] : 0;
-        lrec[f] = make_int2(fbc[bq], -1 - (fbt[bq] | (inst << 2)));
-        g = &fbg[8 * size_t(bq)];
+    parallel_for(long(L), 4096, [&](long, long f0, long f1) {
+      for (long f = f0; f < f1; ++f) {
+        const int sl = bfaces[f];
+        const double *g;
+        if (sl < Fi) {
+          lrec[f] = fic[sl];
+          g = &fig[8 * size_t(sl)];
+        } else {
+          const int bq = sl - Fi;
+          const int inst = C.ninst > 0 ? C.inst_of_real[P[fbc[bq]]] : 0;
+          lrec[f] = make_int2(fbc[bq], -1 - (fbt[bq] | (inst << 2)));
+          g = &fbg[8 * size_t(bq)];
+        }
+        for (int k = 0; k < 7; ++k) lgeo[size_t(k) * L + f] = g[k];
       }
-      for (int k = 0; k < 7; ++k) lgeo[size_t(k) * L + f] = g[k];
-    }
+    });
     if (int rc = C.upload(dlrec, lrec)) return rc;
     return C.upload(dlgeo, lgeo);
   };
@@ -1624,24 +1698,28 @@ int setup_device(fvs_ctx &C) {
   // limiter face offsets (xf - xc) per local edge, in the cell's own frame
   if (C.opt.limiter > 0) {
     std::vector<double2> lo(4 * size_t(N), make_double2(0.0, 0.0));
-    for (int i = 0; i < N; ++i) {
-      const int c = P[i];
-      for (int k = 0; k < m.nvrt[c]; ++k) {
-        const int e = m.cedge[4 * c + k];
-        double xc, yc;
-        side_centroid(m, e, c, xc, yc);
-        lo[size_t(k) * N + i] = make_double2(m.ex[e] - xc, m.ey[e] - yc);
+    parallel_for(N, 4096, [&](long, long i0, long i1) {
+      for (long i = i0; i < i1; ++i) {
+        const int c = P[i];
+        for (int k = 0; k < m.nvrt[c]; ++k) {
+          const int e = m.cedge[4 * c + k];
+          double xc, yc;
+          side_centroid(m, e, c, xc, yc);
+          lo[size_t(k) * N + i] = make_double2(m.ex[e] - xc, m.ey[e] - yc);
+        }
       }
-    }
+    });
     if (int rc = C.upload(C.loff, lo)) return rc;
   }
   // LSQ stencil (gradient and/or limiter)
   if (!m.lsq_ptr.empty()) {
     std::vector<double> cw(2 * m.lsq_cell.size());
-    for (size_t k = 0; k < m.lsq_cell.size(); ++k) {
-      cw[2 * k] = m.lsq_coef[2 * k] * m.lsq_w[k];
-      cw[2 * k + 1] = m.lsq_coef[2 * k + 1] * m.lsq_w[k];
-    }
+    parallel_for(long(m.lsq_cell.size()), 4096, [&](long, long k0, long k1) {
+      for (long k = k0; k < k1; ++k) {
+        cw[2 * k] = m.lsq_coef[2 * k] * m.lsq_w[k];
+        cw[2 * k + 1] = m.lsq_coef[2 * k + 1] * m.lsq_w[k];
+      }
+    });
     std::vector<int> off, wid, nb;
     std::vector<double2> c2;
     if (!pack_sell(N, P, IP, m.lsq_ptr, m.lsq_cell, cw.data(), off, wid, nb, c2))
@@ -1689,33 +1767,35 @@ int setup_device(fvs_ctx &C) {
         std::vector<double2> cxy(N);
         std::vector<double> gi(3 * size_t(N), 0.0);  // planes: G^-1 11, 12, 22
         const double p = C.opt.lsq_p;
-        for (int i = 0; i < N; ++i) {
-          const int c = P[i];
-          cxy[i] = make_double2(m.cx[c], m.cy[c]);
-          // G = D^T D with rows w*(dx,dy), w = 1/d^p (src/gradient_lsq.f90:159-189)
-          double g11 = 0, g12 = 0, g22 = 0;
-          for (int k = m.lsq_ptr[c]; k < m.lsq_ptr[c + 1]; ++k) {
-            const int j = m.lsq_cell[k];
-            const double w = m.lsq_w[k];
-            double sx = 0.0, sy = 0.0;
-            if (m.periodic)
-              for (int a = 0; a < m.nvrt[c] && sx == 0.0 && sy == 0.0; ++a)
-                for (int b = 0; b < m.nvrt[j]; ++b)
-                  if (m.cnodec[4 * c + a] == m.cnodec[4 * j + b]) {
-                    sx = m.nxy[2 * m.cnode[4 * c + a]] - m.nxy[2 * m.cnode[4 * j + b]];
-                    sy = m.nxy[2 * m.cnode[4 * c + a] + 1] - m.nxy[2 * m.cnode[4 * j + b] + 1];
-                    break;
-                  }
-            const double d1 = w * ((m.cx[j] + sx) - m.cx[c]), d2 = w * ((m.cy[j] + sy) - m.cy[c]);
-            g11 = g11 + d1 * d1;
-            g12 = g12 + d1 * d2;
-            g22 = g22 + d2 * d2;
+        parallel_for(N, 4096, [&](long, long i0, long i1) {
+          for (long i = i0; i < i1; ++i) {
+            const int c = P[i];
+            cxy[i] = make_double2(m.cx[c], m.cy[c]);
+            // G = D^T D with rows w*(dx,dy), w = 1/d^p (src/gradient_lsq.f90:159-189)
+            double g11 = 0, g12 = 0, g22 = 0;
+            for (int k = m.lsq_ptr[c]; k < m.lsq_ptr[c + 1]; ++k) {
+              const int j = m.lsq_cell[k];
+              const double w = m.lsq_w[k];
+              double sx = 0.0, sy = 0.0;
+              if (m.periodic)
+                for (int a = 0; a < m.nvrt[c] && sx == 0.0 && sy == 0.0; ++a)
+                  for (int b = 0; b < m.nvrt[j]; ++b)
+                    if (m.cnodec[4 * c + a] == m.cnodec[4 * j + b]) {
+                      sx = m.nxy[2 * m.cnode[4 * c + a]] - m.nxy[2 * m.cnode[4 * j + b]];
+                      sy = m.nxy[2 * m.cnode[4 * c + a] + 1] - m.nxy[2 * m.cnode[4 * j + b] + 1];
+                      break;
+                    }
+              const double d1 = w * ((m.cx[j] + sx) - m.cx[c]), d2 = w * ((m.cy[j] + sy) - m.cy[c]);
+              g11 = g11 + d1 * d1;
+              g12 = g12 + d1 * d2;
+              g22 = g22 + d2 * d2;
+            }
+            const double det = g11 * g22 - g12 * g12;
+            gi[size_t(i)] = 1.0 / det * g22;
+            gi[size_t(N) + i] = -(1.0 / det * g12);
+            gi[2 * size_t(N) + i] = 1.0 / det * g11;
           }
-          const double det = g11 * g22 - g12 * g12;
-          gi[size_t(i)] = 1.0 / det * g22;
-          gi[size_t(N) + i] = -(1.0 / det * g12);
-          gi[2 * size_t(N) + i] = 1.0 / det * g11;
-        }
+        });
         C.pmode = (p == 0.0) ? 0 : (p == 1.0 ? 1 : (p == 2.0 ? 2 : 3));
         if (int rc = C.upload(C.cxy, cxy)) return rc;
         if (int rc = C.upload(C.ginv, gi)) return rc;
@@ -2218,13 +2298,6 @@ const char *fvs_last_error(void) { return g_err.c_str(); }
 namespace fvs {
 int set_error(int code, const std::string &msg) { return fail(code, msg); }
 }  // namespace fvs
-
-// worker threads for host-side preparation: the cores this process may use,
-// at most 16
-static int host_threads() {
-  const unsigned n = std::thread::hardware_concurrency();
-  return int(std::max(1u, std::min(16u, n)));
-}
 
 static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
                        const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types,

@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <thread>
+#include <type_traits>
 #include <unordered_map>
 
 namespace fvs {
@@ -529,28 +530,6 @@ std::string build_mesh(MeshPrep &m, int nnodes, const double *xy, int ntri, int 
   return "";
 }
 
-namespace {
-
-// append src to dst, adding `add` to every entry that is >= 0 (ids; -1 stays)
-void append_ids(std::vector<int> &dst, const std::vector<int> &src, int add) {
-  const size_t o = dst.size();
-  dst.resize(o + src.size());
-  for (size_t k = 0; k < src.size(); ++k) dst[o + k] = src[k] >= 0 ? src[k] + add : src[k];
-}
-template <class T>
-void append(std::vector<T> &dst, const std::vector<T> &src) {
-  dst.insert(dst.end(), src.begin(), src.end());
-}
-// append a CSR pointer array (src[0] = 0) continuing dst's last value
-void append_ptr(std::vector<int> &dst, const std::vector<int> &src) {
-  if (src.empty()) return;
-  const int base = dst.empty() ? 0 : dst.back();
-  if (dst.empty()) dst.push_back(0);
-  for (size_t k = 1; k < src.size(); ++k) dst.push_back(src[k] + base);
-}
-
-}  // namespace
-
 std::string build_mesh_batch(MeshPrep &m, int ninst, const int *cell_offsets, int nnodes, const double *xy, int ntri,
                              int nquad, const int *cell_nodes, int nb, const int *bnd_ncells, const int *bnd_types,
                              const int *bnd_cells, const SchemeFlags &flags, int nthreads) {
@@ -676,87 +655,105 @@ std::string build_mesh_batch(MeshPrep &m, int ninst, const int *cell_offsets, in
     m.bnd_cptr.push_back(int(m.bnd_cells.size()));
     m.bnd_eptr.push_back(int(m.bnd_edges.size()));
   }
-  // every merged array sized once (the parts are released as they are merged)
-  {
-    size_t tot[12] = {0};
-    for (const MeshPrep &p : part) {
-      tot[0] += p.nxy.size();
-      tot[1] += p.canon.size();
-      tot[2] += p.nvrt.size();
-      tot[3] += p.cnode.size();
-      tot[4] += p.n2c.size();
-      tot[5] += p.en1.size();
-      tot[6] += p.edge_intr.size();
-      tot[7] += p.cell_intr.size();
-      tot[8] += p.lsq_cell.size();
-      tot[9] += p.ggcb_ptr.size();
-      tot[10] += p.ggnb_ncoef.size();
-      tot[11] += p.ggnb_edg.size();
+  // field by field: every part's slice at its offset (node, cell and edge ids
+  // shifted, pointer arrays continued), the parts copied on the worker
+  // threads, each field of the parts released once it is merged
+  auto run_parts = [&](auto &&fn) {
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    const int nt = std::max(1, std::min(nthreads, ninst));
+    for (int t = 0; t < nt; ++t)
+      th.emplace_back([&] {
+        for (int k = next++; k < ninst; k = next++) fn(k);
+      });
+    for (auto &x : th) x.join();
+  };
+  auto starts = [&](auto field) {
+    std::vector<size_t> st(ninst + 1, 0);
+    for (int k = 0; k < ninst; ++k) st[k + 1] = st[k] + (part[k].*field).size();
+    return st;
+  };
+  auto merge_copy = [&](auto field) {
+    auto &dst = m.*field;
+    const std::vector<size_t> st = starts(field);
+    dst.resize(st[ninst]);
+    run_parts([&](int k) {
+      auto &src = part[k].*field;
+      std::copy(src.begin(), src.end(), dst.begin() + st[k]);
+      std::remove_reference_t<decltype(src)>().swap(src);
+    });
+  };
+  auto merge_ids = [&](auto field, const std::vector<int> &add) {
+    auto &dst = m.*field;
+    const std::vector<size_t> st = starts(field);
+    dst.resize(st[ninst]);
+    run_parts([&](int k) {
+      auto &src = part[k].*field;
+      for (size_t q = 0; q < src.size(); ++q) dst[st[k] + q] = src[q] >= 0 ? src[q] + add[k] : src[q];
+      std::vector<int>().swap(src);
+    });
+  };
+  // CSR pointer arrays (each part's starting at 0): one leading 0, then each
+  // part's entries after its first plus the running total before it
+  auto merge_ptr = [&](auto field) {
+    auto &dst = m.*field;
+    std::vector<size_t> st(ninst + 1, 1);
+    std::vector<int> base(ninst + 1, 0);
+    bool any = false;
+    for (int k = 0; k < ninst; ++k) {
+      const auto &src = part[k].*field;
+      const size_t n = src.empty() ? 0 : src.size() - 1;
+      any = any || !src.empty();
+      st[k + 1] = st[k] + n;
+      base[k + 1] = base[k] + (src.empty() ? 0 : src.back());
     }
-    m.nxy.reserve(tot[0]);
-    m.canon.reserve(tot[1]);
-    for (auto *v : {&m.nvrt}) v->reserve(tot[2]);
-    for (auto *v : {&m.cnode, &m.cnodec, &m.nghbre, &m.cedge}) v->reserve(tot[3]);
-    for (auto *v : {&m.cx, &m.cy, &m.vol}) v->reserve(tot[2]);
-    m.nrmlsign.reserve(tot[3]);
-    m.n2c_ptr.reserve(size_t(nnodes) + 1);
-    m.n2c.reserve(tot[4]);
-    for (auto *v : {&m.en1, &m.en2, &m.ec1, &m.ec2}) v->reserve(tot[5]);
-    for (auto *v : {&m.ex, &m.ey, &m.ea, &m.enx, &m.eny, &m.esx, &m.esy}) v->reserve(tot[5]);
-    m.edge_intr.reserve(tot[6]);
-    m.cell_intr.reserve(tot[7]);
-    m.lsq_ptr.reserve(size_t(N) + 1);
-    m.lsq_cell.reserve(tot[8]);
-    m.lsq_w.reserve(tot[8]);
-    m.lsq_coef.reserve(2 * tot[8]);
-    m.ggcb_ptr.reserve(tot[9]);
-    m.ggcb_coef0.reserve(tot[9] / 2);
-    m.ggcb_coefnb.reserve(2 * tot[9]);
-    m.ggnb_ncoef.reserve(tot[10]);
-    m.ggnb_edg.reserve(tot[11]);
-  }
+    if (!any) return;
+    dst.assign(st[ninst], 0);
+    run_parts([&](int k) {
+      auto &src = part[k].*field;
+      for (size_t q = 1; q < src.size(); ++q) dst[st[k] + q - 1] = src[q] + base[k];
+      std::vector<int>().swap(src);
+    });
+  };
+  merge_copy(&MeshPrep::nxy);
+  merge_ids(&MeshPrep::canon, vofs);
+  merge_copy(&MeshPrep::nvrt);
+  merge_ids(&MeshPrep::cnode, vofs);
+  merge_ids(&MeshPrep::cnodec, vofs);
+  merge_ids(&MeshPrep::nghbre, cofs);
+  merge_ids(&MeshPrep::cedge, eofs);
+  merge_copy(&MeshPrep::cx);
+  merge_copy(&MeshPrep::cy);
+  merge_copy(&MeshPrep::vol);
+  merge_copy(&MeshPrep::nrmlsign);
+  merge_ptr(&MeshPrep::n2c_ptr);
+  merge_ids(&MeshPrep::n2c, cofs);
+  merge_ids(&MeshPrep::en1, vofs);
+  merge_ids(&MeshPrep::en2, vofs);
+  merge_ids(&MeshPrep::ec1, cofs);
+  merge_ids(&MeshPrep::ec2, cofs);
+  merge_copy(&MeshPrep::ex);
+  merge_copy(&MeshPrep::ey);
+  merge_copy(&MeshPrep::ea);
+  merge_copy(&MeshPrep::enx);
+  merge_copy(&MeshPrep::eny);
+  merge_copy(&MeshPrep::esx);
+  merge_copy(&MeshPrep::esy);
+  merge_ids(&MeshPrep::edge_intr, eofs);
+  merge_ids(&MeshPrep::cell_intr, cofs);
   for (int k = 0; k < ninst; ++k) {
-    MeshPrep &p = part[k];
-    const int co = cofs[k], vo = vofs[k], eo = eofs[k];
-    append(m.nxy, p.nxy);
-    append_ids(m.canon, p.canon, vo);
-    append(m.nvrt, p.nvrt);
-    append_ids(m.cnode, p.cnode, vo);
-    append_ids(m.cnodec, p.cnodec, vo);
-    append_ids(m.nghbre, p.nghbre, co);
-    append_ids(m.cedge, p.cedge, eo);
-    append(m.cx, p.cx);
-    append(m.cy, p.cy);
-    append(m.vol, p.vol);
-    append(m.nrmlsign, p.nrmlsign);
-    append_ptr(m.n2c_ptr, p.n2c_ptr);
-    append_ids(m.n2c, p.n2c, co);
-    append_ids(m.en1, p.en1, vo);
-    append_ids(m.en2, p.en2, vo);
-    append_ids(m.ec1, p.ec1, co);
-    append_ids(m.ec2, p.ec2, co);
-    append(m.ex, p.ex);
-    append(m.ey, p.ey);
-    append(m.ea, p.ea);
-    append(m.enx, p.enx);
-    append(m.eny, p.eny);
-    append(m.esx, p.esx);
-    append(m.esy, p.esy);
-    append_ids(m.edge_intr, p.edge_intr, eo);
-    append_ids(m.cell_intr, p.cell_intr, co);
-    m.ncells_bndr += p.ncells_bndr;
-    m.nedges_bndr += p.nedges_bndr;
-    append_ptr(m.lsq_ptr, p.lsq_ptr);
-    append_ids(m.lsq_cell, p.lsq_cell, co);
-    append(m.lsq_w, p.lsq_w);
-    append(m.lsq_coef, p.lsq_coef);
-    append_ids(m.ggcb_ptr, p.ggcb_ptr, co);
-    append(m.ggcb_coef0, p.ggcb_coef0);
-    append(m.ggcb_coefnb, p.ggcb_coefnb);
-    append(m.ggnb_ncoef, p.ggnb_ncoef);
-    append(m.ggnb_edg, p.ggnb_edg);
-    p = MeshPrep();  // release as we go
+    m.ncells_bndr += part[k].ncells_bndr;
+    m.nedges_bndr += part[k].nedges_bndr;
   }
+  merge_ptr(&MeshPrep::lsq_ptr);
+  merge_ids(&MeshPrep::lsq_cell, cofs);
+  merge_copy(&MeshPrep::lsq_w);
+  merge_copy(&MeshPrep::lsq_coef);
+  merge_ids(&MeshPrep::ggcb_ptr, cofs);
+  merge_copy(&MeshPrep::ggcb_coef0);
+  merge_copy(&MeshPrep::ggcb_coefnb);
+  merge_copy(&MeshPrep::ggnb_ncoef);
+  merge_copy(&MeshPrep::ggnb_edg);
   m.nedges = eofs[ninst];
   return "";
 }
