@@ -168,7 +168,7 @@ def steady_bench(n_train: int, n_test: int, n_seq: int, train_res=(128, 64), tes
                  give_up=(8000, 0.5)):
     """Warm versus cold steady solves/hr (see the module docstring)."""
     from fvs2d_amd import warmstart
-    out = warmstart.run(n_train, n_test, train_res, seed=0, steps=4000, batch=65536, lr=1e-3, depth=5, cfl=1.25,
+    out = warmstart.run(n_train, n_test, train_res, seed=0, steps=16000, batch=65536, lr=1e-3, depth=5, cfl=1.25,
                         max_steps=80000, drop=1e-8, chunk=250, freeze_after=30000, freeze_drop=None,
                         freeze_stall=(2000, 0.1), diverge=1e3, fallback=1.0, group=256, n_seq=n_seq,
                         log=lambda s: print(f"[bench] steady: {s}", file=sys.stderr, flush=True),
