@@ -184,7 +184,8 @@ def steady_bench(n_train: int, n_test: int, n_seq: int, train_res=(128, 64), tes
                      "SSP-RK(4,2) CFL 1.25 (reference convention), to 1e-8 x the cold first-step density residual; "
                      f"an instance is given up when, after its limiter froze, its residual fell by less than "
                      f"{1 / give_up[1]:g}x over {give_up[0]} x n_around/64 steps (cold and warm alike); "
-                     f"warm start trained on {n_train} seeded solves of the same generator at {tr[0]}x{tr[1]}; "
+                     f"warm start trained on {n_train} seeded solves of the same generator at {tr[0]}x{tr[1]} "
+                     "(16,000 Adam steps of 65,536 rows); "
                      + (f"at most {max_active} instances stepping at once, the next one starting as one finishes"
                         if max_active else "all instances stepping from the start"))
     d["max_active"] = max_active
