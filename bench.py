@@ -202,7 +202,7 @@ def main():
     ap.add_argument("--variants", type=int, default=1, help="also time GGCB/GGNB x limiter on/off")
     ap.add_argument("--cpu-steps", type=int, default=6, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--reorder", type=int, default=1, help="device cell order: 1 Morton, 0 reference")
+    ap.add_argument("--reorder", type=int, default=1, help="device cell order: 1 Morton, 2 coordinate bisection, 0 reference")
     ap.add_argument("--path", type=int, default=0, help="0 fused LDS kernels, 1 face-flux buffer kernels")
     ap.add_argument("--mlp-steps", type=int, default=200, help="warm-start network training steps (0 = skip)")
     ap.add_argument("--steady", type=int, default=1, help="warm vs cold steady solves/hr (0 = skip)")
@@ -344,7 +344,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"isentropic vortex, periodic jittered square {args.n}x{args.n} quads -> {N} "
                                "triangles, Roe + unlimited LSQ-nn gradients, upwind-2nd, SSP-RK3, CFL 0.5",
-                   "cell_order": "morton" if args.reorder else "reference",
+                   "cell_order": ("reference", "morton", "bisection")[args.reorder],
                    "kernels": "fused" if args.path == 0 else "face-buffer",
                    "cells": N, "stages_per_step": stages, "dt": dt, "xcd_numbering": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

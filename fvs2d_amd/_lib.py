@@ -27,7 +27,7 @@ EXPORTS = (
     "fvs_time_integration_noalloc", "fvs_vortex_error", "fvs_wall_forces", "fvs_set_timestep", "fvs_kernel_times",
     "fvs_autotune", "fvs_freeze_limiter", "fvs_batch_freeze_limiter", "fvs_wall_cp", "fvs_last_elapsed_ms",
     "fvs_set_phase_timers", "fvs_phase_times", "fvs_set_persistent", "fvs_set_launch_choices",
-    "fvs_mesh_create_batch",
+    "fvs_mesh_create_batch", "fvs_mesh_cell_order",
 )
 
 
@@ -83,6 +83,7 @@ def load():
     L.fvs_mesh_lists.argtypes = [V] * 5
     L.fvs_mesh_lsq_nnz.argtypes = [V]
     L.fvs_mesh_lsq.argtypes = [V] * 5
+    L.fvs_mesh_cell_order.argtypes = [V, I, I, V, V]
     L.fvs_set_state.argtypes = [V, V]
     L.fvs_get_state.argtypes = [V, V]
     L.fvs_initialize.argtypes = [V, I, D]

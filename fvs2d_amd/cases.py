@@ -4,7 +4,8 @@
   (n = 64: 8,192 triangles; n = 1024: 2,097,152), LSQ-nn, SSP-RK3, CFL 0.5.
 * ``naca(n_around, n_layers, mach, aoa, thickness)`` -- steady Euler on the
   NACA 00tt O-grid, slip wall + free stream, LSQ-nn + Venkatakrishnan,
-  SSP-RK(4,2) with local time stepping.
+  SSP-RK(4,2) with local time stepping; cells in coordinate-bisection device
+  order (compact blocks on the graded grid; the states are those of any order).
 
 ``python -m fvs2d_amd.cases steady --n 512 --layers 256`` runs the steady case
 to a 1e-8 density-residual drop and prints C_L, C_D and the iteration count as
@@ -37,7 +38,7 @@ def naca(n_around: int = 512, n_layers: int = 256, mach: float = 0.8, aoa: float
          cfl: float = 1.25, **kw):
     m = meshgen.naca_ogrid(n_around, n_layers, thickness=thickness)
     d = dict(grad_method=3, lsq_nn=1, limiter=1, recon=2, gamma=1.4, mach=mach, aoa_deg=aoa, lvortex=0,
-             rk_scheme=RK_SSP42, rk_order=2, steady=1, cfl=cfl, dt=1.0, reorder=1)
+             rk_scheme=RK_SSP42, rk_order=2, steady=1, cfl=cfl, dt=1.0, reorder=2)
     d.update(kw)
     return m, Scheme(**d)
 

@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <thread>
 #include <numeric>
@@ -1198,8 +1199,6 @@ std::vector<int> morton_order(const MeshPrep &m, const std::vector<int> &inst) {
   return perm;
 }
 
-// sliced-ELL packing of a CSR stencil (cells in device order); false when
-// the padded entry count does not fit the 32-bit slice offsets
 // worker threads for host-side preparation: the cores this process may use,
 // at most 16
 int host_threads() {
@@ -1223,6 +1222,104 @@ void parallel_for(long n, long grain, F fn) {
   for (auto &x : th) x.join();
 }
 
+// Recursive coordinate bisection of cell centroids, instance-major: a range
+// [lo, hi) of device positions is cut at a multiple of 64 near its middle (at
+// the largest power-of-two multiple of 64 within its middle third) along the
+// longer side of its centroids' bounding box, down to 64-cell leaves. Cuts
+// are placed by global position, so every aligned 64-, 128- and 256-cell block
+// is one or a few compact pieces of one instance also where instances start
+// off alignment; on graded meshes (O-grids) the pieces stay compact where a
+// Z-curve over the bounding box does not. Ties and leaves are ordered by cell
+// id, so the order does not depend on the selection algorithm or threads.
+std::vector<int> rcb_order(const MeshPrep &m, const std::vector<int> &inst) {
+  const long N = m.ncells;
+  constexpr long kLeaf = 64;
+  struct Pt {
+    double x, y;
+    int id;
+  };
+  auto key = [](double v) { return std::isnan(v) ? HUGE_VAL : v; };
+  std::vector<Pt> p(N);
+  for (long c = 0; c < N; ++c) p[c] = {key(m.cx[c]), key(m.cy[c]), int(c)};
+  // instances are contiguous in the reference order and stay so
+  std::vector<long> start{0};
+  for (long c = 1; c < N; ++c)
+    if (!inst.empty() && inst[c] != inst[c - 1]) start.push_back(c);
+  start.push_back(N);
+  const long ni = long(start.size()) - 1;
+  auto cut = [](long lo, long hi) -> long {
+    const long first = (lo / kLeaf + 1) * kLeaf;
+    if (first >= hi) return -1;
+    const long two_mid = lo + hi, n = hi - lo;
+    long best = -1;
+    for (long P = kLeaf;; P *= 2) {
+      const long c = (two_mid / 2 / P) * P;
+      long pick = -1;
+      for (long k : {c, c + P})
+        if (k > lo && k < hi && 3 * std::labs(2 * k - two_mid) <= n &&
+            (pick < 0 || std::labs(2 * k - two_mid) < std::labs(2 * pick - two_mid)))
+          pick = k;
+      if (pick < 0) break;
+      best = pick;
+    }
+    if (best < 0) {
+      const long c = ((two_mid / 2 + kLeaf / 2) / kLeaf) * kLeaf;
+      best = std::min(std::max(c, first), ((hi - 1) / kLeaf) * kLeaf);
+    }
+    return best;
+  };
+  auto by_id = [](const Pt &a, const Pt &b) { return a.id < b.id; };
+  std::function<void(long, long, int)> run = [&](long lo, long hi, int spawn) {
+    const long mid = cut(lo, hi);
+    if (mid < 0) {
+      std::sort(p.begin() + lo, p.begin() + hi, by_id);
+      return;
+    }
+    double xa = HUGE_VAL, xb = -HUGE_VAL, ya = HUGE_VAL, yb = -HUGE_VAL;
+    for (long i = lo; i < hi; ++i) {
+      xa = std::min(xa, p[i].x), xb = std::max(xb, p[i].x);
+      ya = std::min(ya, p[i].y), yb = std::max(yb, p[i].y);
+    }
+    const bool byx = xb - xa >= yb - ya;
+    std::nth_element(p.begin() + lo, p.begin() + mid, p.begin() + hi, [byx](const Pt &a, const Pt &b) {
+      const double ka = byx ? a.x : a.y, kb = byx ? b.x : b.y;
+      return ka < kb || (ka == kb && a.id < b.id);
+    });
+    if (spawn > 0 && hi - lo > 65536) {
+      std::thread t([&run, lo, mid, spawn] { run(lo, mid, spawn - 1); });
+      run(mid, hi, spawn - 1);
+      t.join();
+    } else {
+      run(lo, mid, 0);
+      run(mid, hi, 0);
+    }
+  };
+  const int nt = host_threads();
+  if (ni >= nt) {
+    parallel_for(ni, 1, [&](long, long k0, long k1) {
+      for (long k = k0; k < k1; ++k) run(start[k], start[k + 1], 0);
+    });
+  } else {
+    int spawn = 0;
+    while ((2 << spawn) <= nt) ++spawn;
+    for (long k = 0; k < ni; ++k) run(start[k], start[k + 1], spawn);
+  }
+  std::vector<int> perm(N);
+  for (long i = 0; i < N; ++i) perm[i] = p[i].id;
+  return perm;
+}
+
+// device cell order for the reorder option (perm[device position] = cell id)
+std::vector<int> cell_order(const MeshPrep &m, int reorder, const std::vector<int> &inst) {
+  if (reorder == 1) return morton_order(m, inst);
+  if (reorder == 2) return rcb_order(m, inst);
+  std::vector<int> perm(m.ncells);
+  std::iota(perm.begin(), perm.end(), 0);
+  return perm;
+}
+
+// sliced-ELL packing of a CSR stencil (cells in device order); false when
+// the padded entry count does not fit the 32-bit slice offsets
 bool pack_sell(int N, const std::vector<int> &perm, const std::vector<int> &iperm, const std::vector<int> &ptr,
                const std::vector<int> &col, const double *val2, std::vector<int> &off, std::vector<int> &wid,
                std::vector<int> &nb, std::vector<double2> &cw) {
@@ -1504,8 +1601,7 @@ int setup_device(fvs_ctx &C) {
   MeshPrep &m = C.mp;
   const int N = C.N = m.ncells;
   C.perm.resize(N);
-  if (C.opt.reorder == 1) C.perm = morton_order(m, C.inst_of_real);
-  else std::iota(C.perm.begin(), C.perm.end(), 0);
+  C.perm = cell_order(m, C.opt.reorder, C.inst_of_real);
   C.iperm.resize(N);
   for (int i = 0; i < N; ++i) C.iperm[C.perm[i]] = i;
   const auto &P = C.perm, &IP = C.iperm;
@@ -2309,6 +2405,7 @@ static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, in
   if (opt->recon < 1 || opt->recon > 3) return fail(FVS_EINVAL, "recon must be 1, 2 or 3");
   if (opt->limiter < 0 || opt->limiter > 3) return fail(FVS_EINVAL, "limiter must be 0..3");
   if (opt->rk_scheme < 0 || opt->rk_scheme > 3) return fail(FVS_EINVAL, "unknown rk_scheme");
+  if (opt->reorder < 0 || opt->reorder > 2) return fail(FVS_EINVAL, "reorder must be 0, 1 or 2");
   if (opt->rk_scheme == FVS_RK_CLASSIC && (opt->rk_order < 1 || opt->rk_order > 4))
     return fail(FVS_EINVAL, "rk_order must be 1..4");
   if (!(opt->gamma > 1.0)) return fail(FVS_EINVAL, "gamma must exceed 1");
@@ -2518,6 +2615,26 @@ int fvs_mesh_lists(const fvs_mesh *M, int *edge_intr, int *cell_intr, int *bnd_n
   if (cell_intr) std::copy(m.cell_intr.begin(), m.cell_intr.end(), cell_intr);
   if (bnd_nedges) std::copy(m.bnd_nedges.begin(), m.bnd_nedges.end(), bnd_nedges);
   if (bnd_edges) std::copy(m.bnd_edges.begin(), m.bnd_edges.end(), bnd_edges);
+  return 0;
+}
+
+int fvs_mesh_cell_order(const fvs_mesh *M, int reorder, int ninst, const int *cell_offsets, int *perm) {
+  if (!M || !perm || (ninst > 0 && !cell_offsets)) return fail(FVS_EINVAL, "fvs_mesh_cell_order: null argument");
+  if (reorder < 0 || reorder > 2) return fail(FVS_EINVAL, "reorder must be 0, 1 or 2");
+  const MeshPrep &m = M->mp;
+  std::vector<int> inst;
+  if (ninst > 0) {
+    if (cell_offsets[0] != 0 || cell_offsets[ninst] != m.ncells)
+      return fail(FVS_EINVAL, "fvs_mesh_cell_order: cell_offsets must run from 0 to ncells");
+    inst.resize(m.ncells);
+    for (int k = 0; k < ninst; ++k) {
+      if (cell_offsets[k + 1] <= cell_offsets[k])
+        return fail(FVS_EINVAL, "fvs_mesh_cell_order: cell_offsets must increase (every instance has cells)");
+      std::fill(inst.begin() + cell_offsets[k], inst.begin() + cell_offsets[k + 1], k);
+    }
+  }
+  const std::vector<int> p = cell_order(m, reorder, inst);
+  std::copy(p.begin(), p.end(), perm);
   return 0;
 }
 

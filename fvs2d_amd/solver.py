@@ -59,7 +59,7 @@ class Scheme:
     steady: int = 0
     dt: float = 0.01
     cfl: float = 1.0
-    reorder: int = 0          # device cell order: 0 reference, 1 Morton
+    reorder: int = 0          # device cell order: 0 reference, 1 Morton, 2 coordinate bisection
     kernel_path: int = 0      # 0 fastest available, 1 face-flux buffer, 2 whole stage, 3 no block halo
 
     def to_c(self) -> _lib.Options:
@@ -241,6 +241,18 @@ class HostMesh:
         coef = np.zeros(max(2 * nnz, 2))
         check(self._L.fvs_mesh_lsq(self._h, ptr(p), ptr(cell), ptr(w), ptr(coef)))
         return dict(ptr=p, cell=cell[:nnz], w=w[:nnz], coef=coef[: 2 * nnz].reshape(-1, 2))
+
+    def cell_order(self, reorder: int, cell_offsets=None) -> np.ndarray:
+        """Device cell order a solver on this mesh uses (fvs_mesh_cell_order):
+        perm[i] = reference id of the cell at device position i; pass a batch's
+        cell_offsets to keep its instances contiguous."""
+        perm = np.zeros(self.ncells, np.int32)
+        if cell_offsets is None:
+            check(self._L.fvs_mesh_cell_order(self._h, reorder, 0, None, ptr(perm)))
+        else:
+            off = np.ascontiguousarray(cell_offsets, np.int32)
+            check(self._L.fvs_mesh_cell_order(self._h, reorder, len(off) - 1, ptr(off), ptr(perm)))
+        return perm
 
 
 class Solver:

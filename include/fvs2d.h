@@ -61,7 +61,9 @@ typedef struct {
   int rk_order;         /* 1..4, FVS_RK_CLASSIC only                            */
   int steady;           /* local time stepping                                  */
   double dt, cfl;
-  int reorder;          /* device cell order: 0 reference order, 1 Morton blocks */
+  int reorder;          /* device cell order: 0 reference order, 1 Morton (Z-curve)
+                           blocks, 2 recursive coordinate bisection (compact
+                           blocks on graded meshes); results are the same  */
   int kernel_path;      /* 0: fastest available (fused LDS flux+update, centroid LSQ;
                               SSP-RK3 stages read conservative stage states and
                               write no primitive state, with the LSQ gradient or an
@@ -183,6 +185,14 @@ int fvs_mesh_lists(const fvs_mesh *mesh, int *edge_intr, int *cell_intr, int *bn
  * caller sizes cell/w/coef from fvs_mesh_lsq_nnz; coef holds 2 per entry. */
 int fvs_mesh_lsq_nnz(const fvs_mesh *mesh);
 int fvs_mesh_lsq(const fvs_mesh *mesh, int *ptr, int *cell, double *w, double *coef);
+/* The device cell order fvs_create uses for a reorder option (0 reference,
+ * 1 Morton, 2 recursive coordinate bisection): perm[i] is the reference id of
+ * the cell at device position i (ncells entries).  For a batch mesh pass its
+ * ninst and cell_offsets (instances stay contiguous and in order), else 0 and
+ * NULL.  No reference counterpart: the reference keeps the input order
+ * (src/grid_procs.f90:170-794); this shows callers the order the device
+ * blocks are cut from. */
+int fvs_mesh_cell_order(const fvs_mesh *mesh, int reorder, int ninst, const int *cell_offsets, int *perm);
 
 /* cvar in / out (module array data_solution::cvar) */
 int fvs_set_state(fvs_ctx *ctx, const double *cvar);

@@ -221,24 +221,48 @@ def test_gradient_1var():
 
 
 def test_determinism_and_reorder():
-    """Bitwise run-to-run equality (no float atomics); the Morton device order
-    gives bitwise the same states as the reference order (each face flux and
+    """Bitwise run-to-run equality (no float atomics); the Morton and the
+    coordinate-bisection device orders give bitwise the same states as the
+    reference order (each face flux and
     each cell sum is computed by the same arithmetic in either order); the
     face-buffer path agrees to rounding."""
     m = mesh("periodic", 48)
     out = {}
     for key, path in (("a", dict(kernel_path=0, reorder=0)), ("b", dict(kernel_path=0, reorder=0)),
                       ("c", dict(kernel_path=0, reorder=1)), ("d", dict(kernel_path=1, reorder=0)),
-                      ("e", dict(kernel_path=1, reorder=1))):
+                      ("e", dict(kernel_path=1, reorder=1)), ("f", dict(kernel_path=0, reorder=2))):
         gpu = Solver(m, vortex_scheme(dt=0.01, limiter=1), **path)
         gpu.initialize_solution(1, 0.0)
         gpu.time_integration(0.0, 30)
         out[key] = gpu.cvar
     assert np.array_equal(out["a"], out["b"])
     assert np.array_equal(out["a"], out["c"])
+    assert np.array_equal(out["a"], out["f"])
     assert np.array_equal(out["d"], out["e"])
     assert rel_err(out["a"], out["d"]) <= 1e-12
 
+
+
+def test_bisection_order_on_ogrid_bitwise():
+    """A limited steady O-grid run (the limiter frozen half way) in the
+    coordinate-bisection order, with the staged-halo gradient and 128-cell flux
+    blocks too, gives bitwise the states of the Morton order."""
+    from fvs2d_amd import cases
+    m, sch = cases.naca(128, 64, cfl=1.25, limiter=1)
+    out = []
+    for r, ch in ((1, None), (2, None), (2, {"gradient": 6, "fused": 4}), (0, {"gradient": 0})):
+        s = Solver(m, sch, reorder=r)
+        if ch:
+            s.set_launch_choices(ch)
+        s.initialize_solution(1, 0.0)
+        s.time_integration(0.0, 30)
+        s.freeze_limiter(True)
+        s.time_integration(0.0, 10)
+        out.append(s.cvar)
+        s.close()
+    assert np.all(np.isfinite(out[0]))
+    for k in range(1, len(out)):
+        assert np.array_equal(out[0], out[k]), k
 
 def test_vortex_error_call():
     m = mesh("dirichlet", 32)
