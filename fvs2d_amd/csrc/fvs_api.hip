@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <numeric>
@@ -2384,6 +2385,22 @@ void ensure_prim(fvs_ctx &C) {
 // ===========================================================================
 // C-ABI
 // ===========================================================================
+// An exception thrown by host code inside an entry point (an allocation of
+// the preparation's or a transfer's host vectors that fails, a worker thread
+// that cannot start) ends at the C-ABI as an error code and message, never
+// as a C++ exception crossing into the caller.
+static int host_exception(const char *fn) {
+  try {
+    throw;
+  } catch (const std::bad_alloc &) {
+    return fail(FVS_ENOMEM, std::string(fn) + ": host memory exhausted");
+  } catch (const std::exception &e) {
+    return fail(FVS_ESTATE, std::string(fn) + ": " + e.what());
+  } catch (...) {
+    return fail(FVS_ESTATE, std::string(fn) + ": host exception");
+  }
+}
+
 extern "C" {
 
 const char *fvs_last_error(void) { return g_err.c_str(); }
@@ -2420,7 +2437,7 @@ static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, in
   }
   int devs = 0;
   if (hipGetDeviceCount(&devs) != hipSuccess || devs < 1) return fail(FVS_EHIP, "no HIP device available");
-  auto *C = new fvs_ctx();
+  std::unique_ptr<fvs_ctx> C(new fvs_ctx());
   C->opt = *opt;
   if (ninst > 0) {
     const int N = ntri + nquad;
@@ -2443,7 +2460,7 @@ static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, in
                         : build_mesh(C->mp, nnodes, xy, ntri, nquad, cell_nodes, node_canon, nb, bnd_ncells,
                                      bnd_types, bnd_cells, fl);
   if (!err.empty()) {
-    delete C;
+    C.reset();
     return fail(FVS_EINVAL, err);
   }
   const double kPi = 3.14159265358979323846;
@@ -2461,7 +2478,7 @@ static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, in
   P.lvortex = opt->lvortex;
   P.limiter = opt->limiter;
   if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete C;
+    C.reset();
     return fail(FVS_EHIP, "hipStreamCreate failed");
   }
   (void)hipEventCreate(&C->ev0);
@@ -2476,17 +2493,17 @@ static int create_impl(fvs_ctx **out, int nnodes, const double *xy, int ntri, in
   if (!rc0) rc0 = scratch_address(C->device, &C->scratch_dev);
   if (rc0) {
     std::string e = g_err;
-    delete C;
+    C.reset();
     g_err = e;
     return rc0;
   }
   if (int rc = setup_device(*C)) {
     std::string e = g_err;
-    delete C;
+    C.reset();
     g_err = e;
     return rc;
   }
-  *out = C;
+  *out = C.release();
   return 0;
 }
 
@@ -2494,17 +2511,21 @@ extern "C" {
 
 int fvs_create(fvs_ctx **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
                const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types, const int *bnd_cells,
-               const fvs_options *opt) {
+               const fvs_options *opt) try {
   return create_impl(out, nnodes, xy, ntri, nquad, cell_nodes, node_canon, nb, bnd_ncells, bnd_types, bnd_cells, opt,
                      0, nullptr, nullptr);
+} catch (...) {
+  return host_exception("fvs_create");
 }
 
 int fvs_create_batch(fvs_ctx **out, int ninst, const int *cell_offsets, const double *mach_aoa, int nnodes,
                      const double *xy, int ntri, int nquad, const int *cell_nodes, int nb, const int *bnd_ncells,
-                     const int *bnd_types, const int *bnd_cells, const fvs_options *opt) {
+                     const int *bnd_types, const int *bnd_cells, const fvs_options *opt) try {
   if (ninst <= 0) return fail(FVS_EINVAL, "fvs_create_batch: ninst must be positive");
   return create_impl(out, nnodes, xy, ntri, nquad, cell_nodes, nullptr, nb, bnd_ncells, bnd_types, bnd_cells, opt,
                      ninst, cell_offsets, mach_aoa);
+} catch (...) {
+  return host_exception("fvs_create_batch");
 }
 
 int fvs_destroy(fvs_ctx *ctx) {
@@ -2522,7 +2543,7 @@ int fvs_sizes(const fvs_ctx *C, int *o) {
 
 int fvs_mesh_create(fvs_mesh **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
                     const int *node_canon, int nb, const int *bnd_ncells, const int *bnd_types, const int *bnd_cells,
-                    int grad_method, int lsq_nn, double lsq_p, int limiter) {
+                    int grad_method, int lsq_nn, double lsq_p, int limiter) try {
   if (!out || !xy || !cell_nodes || (nb > 0 && (!bnd_ncells || !bnd_types)))
     return fail(FVS_EINVAL, "fvs_mesh_create: null argument");
   *out = nullptr;
@@ -2531,21 +2552,20 @@ int fvs_mesh_create(fvs_mesh **out, int nnodes, const double *xy, int ntri, int 
   fl.lsq_nn = lsq_nn;
   fl.lsq_p = lsq_p;
   fl.limiter = limiter;
-  auto *M = new fvs_mesh();
+  std::unique_ptr<fvs_mesh> M(new fvs_mesh());
   std::string err = build_mesh(M->mp, nnodes, xy, ntri, nquad, cell_nodes, node_canon, nb, bnd_ncells, bnd_types,
                                bnd_cells, fl);
-  if (!err.empty()) {
-    delete M;
-    return fail(FVS_EINVAL, err);
-  }
-  *out = M;
+  if (!err.empty()) return fail(FVS_EINVAL, err);
+  *out = M.release();
   return 0;
+} catch (...) {
+  return host_exception("fvs_mesh_create");
 }
 
 int fvs_mesh_create_batch(fvs_mesh **out, int ninst, const int *cell_offsets, int nnodes, const double *xy,
                           int ntri, int nquad, const int *cell_nodes, int nb, const int *bnd_ncells,
                           const int *bnd_types, const int *bnd_cells, int grad_method, int lsq_nn, double lsq_p,
-                          int limiter, int nthreads) {
+                          int limiter, int nthreads) try {
   if (!out || !xy || !cell_nodes || !cell_offsets || (nb > 0 && (!bnd_ncells || !bnd_types || !bnd_cells)))
     return fail(FVS_EINVAL, "fvs_mesh_create_batch: null argument");
   *out = nullptr;
@@ -2559,15 +2579,14 @@ int fvs_mesh_create_batch(fvs_mesh **out, int ninst, const int *cell_offsets, in
   fl.lsq_nn = lsq_nn;
   fl.lsq_p = lsq_p;
   fl.limiter = limiter;
-  auto *M = new fvs_mesh();
+  std::unique_ptr<fvs_mesh> M(new fvs_mesh());
   std::string err = build_mesh_batch(M->mp, ninst, cell_offsets, nnodes, xy, ntri, nquad, cell_nodes, nb, bnd_ncells,
                                      bnd_types, bnd_cells, fl, nthreads > 0 ? nthreads : host_threads());
-  if (!err.empty()) {
-    delete M;
-    return fail(FVS_EINVAL, err);
-  }
-  *out = M;
+  if (!err.empty()) return fail(FVS_EINVAL, err);
+  *out = M.release();
   return 0;
+} catch (...) {
+  return host_exception("fvs_mesh_create_batch");
 }
 
 int fvs_mesh_destroy(fvs_mesh *M) {
@@ -2618,7 +2637,7 @@ int fvs_mesh_lists(const fvs_mesh *M, int *edge_intr, int *cell_intr, int *bnd_n
   return 0;
 }
 
-int fvs_mesh_cell_order(const fvs_mesh *M, int reorder, int ninst, const int *cell_offsets, int *perm) {
+int fvs_mesh_cell_order(const fvs_mesh *M, int reorder, int ninst, const int *cell_offsets, int *perm) try {
   if (!M || !perm || (ninst > 0 && !cell_offsets)) return fail(FVS_EINVAL, "fvs_mesh_cell_order: null argument");
   if (reorder < 0 || reorder > 2) return fail(FVS_EINVAL, "reorder must be 0, 1 or 2");
   const MeshPrep &m = M->mp;
@@ -2636,6 +2655,8 @@ int fvs_mesh_cell_order(const fvs_mesh *M, int reorder, int ninst, const int *ce
   const std::vector<int> p = cell_order(m, reorder, inst);
   std::copy(p.begin(), p.end(), perm);
   return 0;
+} catch (...) {
+  return host_exception("fvs_mesh_cell_order");
 }
 
 int fvs_mesh_lsq_nnz(const fvs_mesh *M) { return (M && !M->mp.lsq_ptr.empty()) ? M->mp.lsq_ptr.back() : 0; }
@@ -2651,7 +2672,7 @@ int fvs_mesh_lsq(const fvs_mesh *M, int *ptr, int *cell, double *w, double *coef
   return 0;
 }
 
-int fvs_set_state(fvs_ctx *C, const double *cvar) {
+int fvs_set_state(fvs_ctx *C, const double *cvar) try {
   if (!C || !cvar) return fail(FVS_EINVAL, "null argument");
   std::vector<double> h;
   to_device_order(*C, cvar, h, 4);
@@ -2661,9 +2682,11 @@ int fvs_set_state(fvs_ctx *C, const double *cvar) {
   HIPC(hipStreamSynchronize(C->stream));
   C->have_state = true;
   return 0;
+} catch (...) {
+  return host_exception("fvs_set_state");
 }
 
-int fvs_get_state(fvs_ctx *C, double *cvar) {
+int fvs_get_state(fvs_ctx *C, double *cvar) try {
   if (int rc = need_state(C)) return rc;
   if (!cvar) return fail(FVS_EINVAL, "null argument");
   std::vector<double> h(4 * size_t(C->N));
@@ -2671,9 +2694,11 @@ int fvs_get_state(fvs_ctx *C, double *cvar) {
   HIPC(hipStreamSynchronize(C->stream));
   to_ref_order(*C, h, cvar, 4);
   return 0;
+} catch (...) {
+  return host_exception("fvs_get_state");
 }
 
-int fvs_initialize(fvs_ctx *C, int mode, double t0) {
+int fvs_initialize(fvs_ctx *C, int mode, double t0) try {
   if (!C) return fail(FVS_EINVAL, "null context");
   // initial primitive state at the cell centres, then pvar2cvar
   // (src/initialize.f90:36-54,79-87; src/data_solution.f90:92-106)
@@ -2708,9 +2733,11 @@ int fvs_initialize(fvs_ctx *C, int mode, double t0) {
     u[3] = pv[3] / (g - 1.0) + 0.5 * pv[0] * (pv[1] * pv[1] + pv[2] * pv[2]);
   }
   return fvs_set_state(C, cv.data());
+} catch (...) {
+  return host_exception("fvs_initialize");
 }
 
-int fvs_primitives(fvs_ctx *C, double *pvar) {
+int fvs_primitives(fvs_ctx *C, double *pvar) try {
   if (int rc = need_state(C)) return rc;
   refresh_prim(*C);
   HIPC(hipGetLastError());
@@ -2722,9 +2749,11 @@ int fvs_primitives(fvs_ctx *C, double *pvar) {
   }
   HIPC(hipStreamSynchronize(C->stream));
   return 0;
+} catch (...) {
+  return host_exception("fvs_primitives");
 }
 
-int fvs_gradient(fvs_ctx *C, double *grad) {
+int fvs_gradient(fvs_ctx *C, double *grad) try {
   if (int rc = need_state(C)) return rc;
   ensure_prim(*C);
   if (C->opt.recon != 1)
@@ -2743,9 +2772,11 @@ int fvs_gradient(fvs_ctx *C, double *grad) {
   }
   HIPC(hipStreamSynchronize(C->stream));
   return 0;
+} catch (...) {
+  return host_exception("fvs_gradient");
 }
 
-int fvs_gradient_1var(fvs_ctx *C, const double *fc, double *dfc) {
+int fvs_gradient_1var(fvs_ctx *C, const double *fc, double *dfc) try {
   if (!C || !fc || !dfc) return fail(FVS_EINVAL, "null argument");
   // the scalar rides in slot 0 of a scratch primitive record; the gradient
   // kernels treat the four slots independently
@@ -2765,9 +2796,11 @@ int fvs_gradient_1var(fvs_ctx *C, const double *fc, double *dfc) {
     dfc[size_t(N) + C->perm[i]] = g[8 * size_t(i) + 4];
   }
   return 0;
+} catch (...) {
+  return host_exception("fvs_gradient_1var");
 }
 
-int fvs_limiter(fvs_ctx *C, double *phi) {
+int fvs_limiter(fvs_ctx *C, double *phi) try {
   if (int rc = need_state(C)) return rc;
   ensure_prim(*C);
   if (C->opt.recon != 1 && C->opt.limiter > 0)
@@ -2780,9 +2813,11 @@ int fvs_limiter(fvs_ctx *C, double *phi) {
   }
   HIPC(hipStreamSynchronize(C->stream));
   return 0;
+} catch (...) {
+  return host_exception("fvs_limiter");
 }
 
-int fvs_compute_residual(fvs_ctx *C, double time, double *resid, double *ws) {
+int fvs_compute_residual(fvs_ctx *C, double time, double *resid, double *ws) try {
   if (int rc = need_state(C)) return rc;
   refresh_prim(*C);
   if (int rc = launch_fluxes(*C, time)) return rc;
@@ -2805,9 +2840,11 @@ int fvs_compute_residual(fvs_ctx *C, double time, double *resid, double *ws) {
   }
   HIPC(hipStreamSynchronize(C->stream));
   return 0;
+} catch (...) {
+  return host_exception("fvs_compute_residual");
 }
 
-int fvs_local_dt(fvs_ctx *C, double *dt) {
+int fvs_local_dt(fvs_ctx *C, double *dt) try {
   if (int rc = need_state(C)) return rc;
   // ws_nrml of the current state, then dt = CFL*vol/(0.5*ws)
   std::vector<double> ws(C->N);
@@ -2815,10 +2852,12 @@ int fvs_local_dt(fvs_ctx *C, double *dt) {
   if (dt)
     for (int c = 0; c < C->N; ++c) dt[c] = C->opt.cfl * C->mp.vol[c] / (0.5 * ws[c]);
   return 0;
+} catch (...) {
+  return host_exception("fvs_local_dt");
 }
 
 int fvs_roe_flux_batch(int n, double gamma, const double *pL, const double *pR, const double *nxy, double *flux,
-                       double *ws) {
+                       double *ws) try {
   if (n <= 0) return 0;
   if (!pL || !pR || !nxy || !flux || !ws) return fail(FVS_EINVAL, "null argument");
   double *d = nullptr;
@@ -2838,6 +2877,8 @@ int fvs_roe_flux_batch(int n, double gamma, const double *pL, const double *pR, 
   }
   (void)hipFree(d);
   return rc;
+} catch (...) {
+  return host_exception("fvs_roe_flux_batch");
 }
 
 }  // extern "C"
@@ -3138,13 +3179,15 @@ int integrate(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr, do
 
 extern "C" {
 
-int fvs_time_integration(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr) {
+int fvs_time_integration(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr) try {
   if (C && C->nact && norms) {
     // with inactive instances the whole-mesh norm would mix stale partials
     std::fill(norms, norms + 4 * size_t(std::max(nsteps, 0)), std::nan(""));
     return integrate(C, t1, nsteps, nullptr, verr, nullptr);
   }
   return integrate(C, t1, nsteps, norms, verr, nullptr);
+} catch (...) {
+  return host_exception("fvs_time_integration");
 }
 
 int fvs_time_integration_noalloc(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr) {
@@ -3207,13 +3250,15 @@ int fvs_set_graph(fvs_ctx *C, int on) {
   return 0;
 }
 
-int fvs_batch_time_integration(fvs_ctx *C, double t1, int nsteps, double *inst_norms) {
+int fvs_batch_time_integration(fvs_ctx *C, double t1, int nsteps, double *inst_norms) try {
   if (!C) return fail(FVS_EINVAL, "null context");
   if (C->ninst <= 0) return fail(FVS_ESTATE, "not a batch context (use fvs_create_batch)");
   return integrate(C, t1, nsteps, nullptr, nullptr, inst_norms);
+} catch (...) {
+  return host_exception("fvs_batch_time_integration");
 }
 
-int fvs_batch_set_active(fvs_ctx *C, const int *active) {
+int fvs_batch_set_active(fvs_ctx *C, const int *active) try {
   if (!C || !active) return fail(FVS_EINVAL, "null argument");
   if (C->ninst <= 0) return fail(FVS_ESTATE, "not a batch context (use fvs_create_batch)");
   if (!C->fused || !(C->lsqc || C->opt.recon == 1))
@@ -3252,9 +3297,11 @@ int fvs_batch_set_active(fvs_ctx *C, const int *active) {
   C->nact = all ? 0 : int(list.size());
   C->launch_gen++;
   return 0;
+} catch (...) {
+  return host_exception("fvs_batch_set_active");
 }
 
-int fvs_batch_wall_forces(fvs_ctx *C, double *out) {
+int fvs_batch_wall_forces(fvs_ctx *C, double *out) try {
   if (int rc = need_state(C)) return rc;
   if (C->ninst <= 0) return fail(FVS_ESTATE, "not a batch context (use fvs_create_batch)");
   if (!out) return fail(FVS_EINVAL, "null argument");
@@ -3270,18 +3317,22 @@ int fvs_batch_wall_forces(fvs_ctx *C, double *out) {
   HIPC(hipMemcpyAsync(out, C->dforces, 4 * size_t(C->ninst) * 8, hipMemcpyDeviceToHost, C->stream));
   HIPC(hipStreamSynchronize(C->stream));
   return 0;
+} catch (...) {
+  return host_exception("fvs_batch_wall_forces");
 }
 
-int fvs_vortex_error(fvs_ctx *C, double t, double *out15) {
+int fvs_vortex_error(fvs_ctx *C, double t, double *out15) try {
   if (int rc = need_state(C)) return rc;
   if (!out15) return fail(FVS_EINVAL, "null argument");
   if (int rc = launch_vortex_err(*C, t, C->small)) return rc;
   HIPC(hipMemcpyAsync(out15, C->small, kVerr * 8, hipMemcpyDeviceToHost, C->stream));
   HIPC(hipStreamSynchronize(C->stream));
   return 0;
+} catch (...) {
+  return host_exception("fvs_vortex_error");
 }
 
-int fvs_wall_forces(fvs_ctx *C, double *out4) {
+int fvs_wall_forces(fvs_ctx *C, double *out4) try {
   if (int rc = need_state(C)) return rc;
   if (!out4) return fail(FVS_EINVAL, "null argument");
   const double kPi = 3.14159265358979323846;
@@ -3294,9 +3345,11 @@ int fvs_wall_forces(fvs_ctx *C, double *out4) {
   HIPC(hipMemcpyAsync(out4, C->small + 16, 4 * 8, hipMemcpyDeviceToHost, C->stream));
   HIPC(hipStreamSynchronize(C->stream));
   return 0;
+} catch (...) {
+  return host_exception("fvs_wall_forces");
 }
 
-int fvs_wall_cp(fvs_ctx *C, int *nwall, double *out4, double *un3) {
+int fvs_wall_cp(fvs_ctx *C, int *nwall, double *out4, double *un3) try {
   if (int rc = need_state(C)) return rc;
   if (!nwall) return fail(FVS_EINVAL, "null argument");
   *nwall = C->nwall;
@@ -3336,6 +3389,8 @@ int fvs_wall_cp(fvs_ctx *C, int *nwall, double *out4, double *un3) {
     un3[2] = l1 / double(C->nwall);
   }
   return 0;
+} catch (...) {
+  return host_exception("fvs_wall_cp");
 }
 
 int fvs_set_timestep(fvs_ctx *C, double dt, double cfl) {
@@ -3347,7 +3402,7 @@ int fvs_set_timestep(fvs_ctx *C, double dt, double cfl) {
   return 0;
 }
 
-int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *bytes, int *launches) {
+int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *bytes, int *launches) try {
   if (int rc = need_state(C)) return rc;
   if (nsteps <= 0 || !ms || !bytes || !launches) return fail(FVS_EINVAL, "bad argument");
   C->profiling = true;
@@ -3426,6 +3481,8 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
     launches[k] = cnt[k];
   }
   return 0;
+} catch (...) {
+  return host_exception("fvs_kernel_times");
 }
 
 int fvs_freeze_limiter(fvs_ctx *C, int frozen) {
@@ -3436,7 +3493,7 @@ int fvs_freeze_limiter(fvs_ctx *C, int frozen) {
   return 0;
 }
 
-int fvs_batch_freeze_limiter(fvs_ctx *C, const int *frozen) {
+int fvs_batch_freeze_limiter(fvs_ctx *C, const int *frozen) try {
   if (!C || !frozen) return fail(FVS_EINVAL, "null argument");
   if (C->ninst <= 0) return fail(FVS_ESTATE, "not a batch context (use fvs_create_batch)");
   if (!C->lsqc) return fail(FVS_ESTATE, "per-instance freezing needs the centroid-form LSQ gradient");
@@ -3454,9 +3511,11 @@ int fvs_batch_freeze_limiter(fvs_ctx *C, const int *frozen) {
     HIPC(hipStreamSynchronize(C->stream));
   }
   return 0;
+} catch (...) {
+  return host_exception("fvs_batch_freeze_limiter");
 }
 
-int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) {
+int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) try {
   if (int rc = need_state(C)) return rc;
   if (nsteps <= 0) return fail(FVS_EINVAL, "nsteps must be positive");
   // the trials advance the state: keep a copy and restore it afterwards
@@ -3523,9 +3582,11 @@ int fvs_autotune(fvs_ctx *C, int nsteps, int *choice4, double *best_ms) {
   }
   if (best_ms) *best_ms = best;
   return 0;
+} catch (...) {
+  return host_exception("fvs_autotune");
 }
 
-int fvs_set_launch_choices(fvs_ctx *C, const int *choice4) {
+int fvs_set_launch_choices(fvs_ctx *C, const int *choice4) try {
   if (!C || !choice4) return fail(FVS_EINVAL, "null argument");
   const bool halo = (choice4[0] & 2) != 0, small = (choice4[3] & 4) != 0, small_grad = (choice4[0] & 4) != 0;
   if (halo && !C->halo) return fail(FVS_EINVAL, "launch choices: no staged-halo gradient data for this context");
@@ -3542,6 +3603,8 @@ int fvs_set_launch_choices(fvs_ctx *C, const int *choice4) {
   C->small_blocks = small;
   C->launch_gen++;
   return 0;
+} catch (...) {
+  return host_exception("fvs_set_launch_choices");
 }
 
 double fvs_last_elapsed_ms(const fvs_ctx *C) { return C ? C->last_ms : 0.0; }
