@@ -258,7 +258,8 @@ def main():
         tuned_ms = None
         print(f"[bench] launch choices: {tuned}", file=sys.stderr, flush=True)
     else:
-        tuned, tuned_ms = s0.autotune(10)
+        # 30-step trials: the two flux block sizes differ by about 1.5% of a step
+        tuned, tuned_ms = s0.autotune(30)
         print(f"[bench] autotune: {tuned} ({tuned_ms:.4f} ms/step)", file=sys.stderr, flush=True)
     wall, dev = timed_run(s0, args.steps, args.warmup, init=False)
     stages = 3
