@@ -68,14 +68,14 @@ def naca():
 
 def test_naca0012_example_steady(naca):
     """The deck as given (steady SSP-RK(4,2), CFL 1.25, LSQ-nn, no limiter,
-    M 0.8, 0 deg): the fused path in Morton and in the reference's cell
-    order and its cell-by-cell gradient arrangement (all bitwise equal), and
-    the face-buffer kernels, against the oracle; then the wall forces of the
-    same state."""
+    M 0.8, 0 deg): the fused path in Morton, coordinate-bisection and the
+    reference's cell order and its cell-by-cell gradient arrangement (all
+    bitwise equal), and the face-buffer kernels, against the oracle; then the
+    wall forces of the same state."""
     m, sch, orc, n_ref, tol = naca
     assert (m.ntri, m.nquad) == (0, 65536)
     out = {}
-    for path, reorder in ((0, 1), (3, 1), (0, 0), (1, 0)):
+    for path, reorder in ((0, 1), (3, 1), (0, 0), (0, 2), (1, 0)):
         s = Solver(m, sch, kernel_path=path, reorder=reorder)
         s.initialize_solution(1, 0.0)
         n = s.time_integration(0.0, NACA_STEPS)
@@ -87,6 +87,7 @@ def test_naca0012_example_steady(naca):
     assert err <= tol
     assert np.array_equal(u, out[3, 1][0])
     assert np.array_equal(u, out[0, 0][0])
+    assert np.array_equal(u, out[0, 2][0])
     assert sym_err(out[1, 0][0], orc.cvar) <= tol
     assert rel_err(out[0, 1][1], n_ref) <= 100 * tol
     orc.cvar2pvar()
