@@ -54,7 +54,8 @@ def build(verbose: bool = False) -> str:
             print(" ".join(c), flush=True)
         _run(c)
     tmp = LIB + ".tmp"
-    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", obj_dev, obj_mlp, obj_host, "-o", tmp]
+    # -z defs: a symbol left undefined is a link error here, not a load error on the GPU box
+    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-z,defs", obj_dev, obj_mlp, obj_host, "-o", tmp]
     if verbose:
         print(" ".join(link), flush=True)
     _run(link)

@@ -1218,9 +1218,7 @@ void parallel_for(long n, long grain, F fn) {
     fn(0L, 0L, n);
     return;
   }
-  std::vector<std::thread> th;
-  for (long t = 0; t < nt; ++t) th.emplace_back([&fn, n, nt, t] { fn(t, n * t / nt, n * (t + 1) / nt); });
-  for (auto &x : th) x.join();
+  run_threads(int(nt), [&](int t) { fn(long(t), n * t / nt, n * (t + 1) / nt); });
 }
 
 // Recursive coordinate bisection of cell centroids, instance-major: a range
@@ -1287,9 +1285,7 @@ std::vector<int> rcb_order(const MeshPrep &m, const std::vector<int> &inst) {
       return ka < kb || (ka == kb && a.id < b.id);
     });
     if (spawn > 0 && hi - lo > 65536) {
-      std::thread t([&run, lo, mid, spawn] { run(lo, mid, spawn - 1); });
-      run(mid, hi, spawn - 1);
-      t.join();
+      run_threads(2, [&](int t) { t ? run(lo, mid, spawn - 1) : run(mid, hi, spawn - 1); });
     } else {
       run(lo, mid, 0);
       run(mid, hi, 0);

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <exception>
 #include <thread>
 #include <type_traits>
 #include <unordered_map>
@@ -606,11 +607,7 @@ std::string build_mesh_batch(MeshPrep &m, int ninst, const int *cell_offsets, in
                            nullptr, nbk, bn.data(), bt.data(), bc.data(), flags);
     }
   };
-  const int nt = std::max(1, std::min(nthreads, ninst));
-  std::vector<std::thread> pool;
-  for (int t = 1; t < nt; ++t) pool.emplace_back(work);
-  work();
-  for (auto &t : pool) t.join();
+  run_threads(std::max(1, std::min(nthreads, ninst)), [&](int) { work(); });
   // every failing instance is named (callers may drop them all at once)
   {
     std::string bad, first;
@@ -660,13 +657,9 @@ std::string build_mesh_batch(MeshPrep &m, int ninst, const int *cell_offsets, in
   // threads, each field of the parts released once it is merged
   auto run_parts = [&](auto &&fn) {
     std::atomic<int> next{0};
-    std::vector<std::thread> th;
-    const int nt = std::max(1, std::min(nthreads, ninst));
-    for (int t = 0; t < nt; ++t)
-      th.emplace_back([&] {
-        for (int k = next++; k < ninst; k = next++) fn(k);
-      });
-    for (auto &x : th) x.join();
+    run_threads(std::max(1, std::min(nthreads, ninst)), [&](int) {
+      for (int k = next++; k < ninst; k = next++) fn(k);
+    });
   };
   auto starts = [&](auto field) {
     std::vector<size_t> st(ninst + 1, 0);
@@ -756,6 +749,30 @@ std::string build_mesh_batch(MeshPrep &m, int ninst, const int *cell_offsets, in
   merge_copy(&MeshPrep::ggnb_edg);
   m.nedges = eofs[ninst];
   return "";
+}
+
+void run_threads(int nt, const std::function<void(int)> &body) {
+  std::vector<std::exception_ptr> err(std::max(nt, 1));
+  auto guarded = [&](int t) {
+    try {
+      body(t);
+    } catch (...) {
+      err[t] = std::current_exception();
+    }
+  };
+  std::vector<std::thread> th;
+  int started = 1;
+  try {
+    th.reserve(std::max(nt - 1, 0));
+    for (; started < nt; ++started) th.emplace_back(guarded, started);
+  } catch (...) {
+    // no thread for parts started .. nt - 1: they run below
+  }
+  for (int t = started; t < nt; ++t) guarded(t);
+  guarded(0);
+  for (auto &x : th) x.join();
+  for (auto &e : err)
+    if (e) std::rethrow_exception(e);
 }
 
 }  // namespace fvs

@@ -3,6 +3,7 @@
 // the stencil constructions of src/gradient_lsq.f90, src/gradient_ggcb.f90 and
 // src/gradient_ggnb.f90.  All ids are 0-based, -1 meaning "none".
 #pragma once
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -62,6 +63,13 @@ std::string build_mesh(MeshPrep &m, int nnodes, const double *xy, int ntri, int 
 std::string build_mesh_batch(MeshPrep &m, int ninst, const int *cell_offsets, int nnodes, const double *xy, int ntri,
                              int nquad, const int *cell_nodes, int nb, const int *bnd_ncells, const int *bnd_types,
                              const int *bnd_cells, const SchemeFlags &flags, int nthreads);
+
+// Runs body(0) .. body(nt - 1): body(0) on the calling thread, the others on
+// worker threads; a part whose thread cannot be started runs on the calling
+// thread instead.  Every thread is joined before the first exception a part
+// threw (an allocation that failed) is rethrown to the caller, so no
+// exception ends a worker thread or leaves one running.
+void run_threads(int nt, const std::function<void(int)> &body);
 
 // Frame helpers for periodic meshes (no reference counterpart): the centroid
 // of edge `je`'s cell `c` expressed in the edge's c1 frame.
