@@ -425,14 +425,29 @@ __global__ __launch_bounds__(BS) void k_flux_update(int N, int nb, int gw, int m
   if (lb >= nbl) return;  // the whole workgroup leaves before any barrier
   const int i0 = lb * BS;
   const int f0 = boff[lb], nf = boff[lb + 1] - f0;
-  // the first face record is in flight while the block's own cells are staged
+  // the first face record is in flight while the block's own cells are
+  // staged, and so are its limiter values and its outside side's state and
+  // gradient (a face touching the block has at most one cell outside it), so
+  // that the first face waits on no global load after the barrier
   int f = threadIdx.x;
   int2 c = make_int2(0, 0);
   double g[8];
   g[7] = 0.0;
+  double wO[4] = {0, 0, 0, 0}, gO[8] = {0, 0, 0, 0, 0, 0, 0, 0}, pO[2] = {1.0, 1.0};
   if (f < nf) {
     c = lrec[f0 + f];
     for (int k = 0; k < 7; ++k) g[k] = lgeo[k * L + f0 + f];
+    if (!phi_one) {
+      pO[0] = phi[c.x];
+      if (c.y >= 0) pO[1] = phi[c.y];
+    }
+    if (c.y >= 0) {
+      const int co = unsigned(c.x - i0) >= unsigned(BS) ? c.x : (unsigned(c.y - i0) >= unsigned(BS) ? c.y : -1);
+      if (co >= 0) {
+        ld4(Win, co, wO);
+        ld8(grad, co, gO);
+      }
+    }
   }
   if (i0 + int(threadIdx.x) < N) {
     double gg[8], ww[4];
@@ -442,43 +457,54 @@ __global__ __launch_bounds__(BS) void k_flux_update(int N, int nb, int gw, int m
     st4(sW, threadIdx.x, ww);
   }
   __syncthreads();
-  for (; f < nf; f += BS) {
-    if (f != int(threadIdx.x)) {
-      c = lrec[f0 + f];
-      for (int k = 0; k < 7; ++k) g[k] = lgeo[k * L + f0 + f];
-    }
+  // one face's flux into LDS; pre: its outside side is the prefetched one
+  auto face = [&](int fs, int2 cs, const double gs[8], bool pre) __attribute__((always_inline)) {
     double fa[4], wsa;
-    if (c.y >= 0) {
+    if (cs.y >= 0) {
       double wL[4], wR[4], gL[8], gR[8];
-      const unsigned la = unsigned(c.x - i0), lc = unsigned(c.y - i0);
+      const unsigned la = unsigned(cs.x - i0), lc = unsigned(cs.y - i0);
       if (la < unsigned(BS)) {
         ld4(sW, la, wL);
         ld8(sG, la, gL);
+      } else if (pre) {
+        for (int v = 0; v < 4; ++v) wL[v] = wO[v];
+        for (int v = 0; v < 8; ++v) gL[v] = gO[v];
       } else {
-        ld4(Win, c.x, wL);
-        ld8(grad, c.x, gL);
+        ld4(Win, cs.x, wL);
+        ld8(grad, cs.x, gL);
       }
       if (lc < unsigned(BS)) {
         ld4(sW, lc, wR);
         ld8(sG, lc, gR);
+      } else if (pre) {
+        for (int v = 0; v < 4; ++v) wR[v] = wO[v];
+        for (int v = 0; v < 8; ++v) gR[v] = gO[v];
       } else {
-        ld4(Win, c.y, wR);
-        ld8(grad, c.y, gR);
+        ld4(Win, cs.y, wR);
+        ld8(grad, cs.y, gR);
       }
       make_prim(cons, A.gamma, wL);
       make_prim(cons, A.gamma, wR);
-      interior_flux_v(P, g, wL, wR, gL, gR, phi_one ? 1.0 : phi[c.x], phi_one ? 1.0 : phi[c.y], fa, wsa);
+      const double phL = phi_one ? 1.0 : (pre ? pO[0] : phi[cs.x]);
+      const double phR = phi_one ? 1.0 : (pre ? pO[1] : phi[cs.y]);
+      interior_flux_v(P, gs, wL, wR, gL, gR, phL, phR, fa, wsa);
     } else {
-      const int code = -1 - c.y;
+      const int code = -1 - cs.y;
       double w[4], gr[8];
-      ld4(sW, c.x - i0, w);  // a boundary face's cell is one of the block's own
-      ld8(sG, c.x - i0, gr);
+      ld4(sW, cs.x - i0, w);  // a boundary face's cell is one of the block's own
+      ld8(sG, cs.x - i0, gr);
       make_prim(cons, A.gamma, w);
-      boundary_flux_v(P, t, code & 3, g, w, gr, phi_one ? 1.0 : phi[c.x], fa, wsa,
+      boundary_flux_v(P, t, code & 3, gs, w, gr, phi_one ? 1.0 : (pre ? pO[0] : phi[cs.x]), fa, wsa,
                       pinf_inst ? pinf_inst + 4 * (code >> 2) : P.pinf);
     }
-    for (int v = 0; v < 4; ++v) lF[4 * f + v] = fa[v];
-    if (ws) lS[f] = wsa;
+    for (int v = 0; v < 4; ++v) lF[4 * fs + v] = fa[v];
+    if (ws) lS[fs] = wsa;
+  };
+  if (f < nf) face(f, c, g, true);
+  for (f += BS; f < nf; f += BS) {
+    c = lrec[f0 + f];
+    for (int k = 0; k < 7; ++k) g[k] = lgeo[k * L + f0 + f];
+    face(f, c, g, false);
   }
   UpdIn in;
   const int ic = i0 + threadIdx.x;
