@@ -434,6 +434,8 @@ __global__ __launch_bounds__(BS) void k_flux_update(int N, int nb, int gw, int m
   double g[8];
   g[7] = 0.0;
   double wO[4] = {0, 0, 0, 0}, gO[8] = {0, 0, 0, 0, 0, 0, 0, 0}, pO[2] = {1.0, 1.0};
+  int2 c2 = make_int2(0, 0);  // the second face's cells, for its gathers' early start
+  if (f + BS < nf) c2 = lrec[f0 + f + BS];
   if (f < nf) {
     c = lrec[f0 + f];
     for (int k = 0; k < 7; ++k) g[k] = lgeo[k * L + f0 + f];
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(BS) void k_flux_update(int N, int nb, int gw, int m
   };
   if (f < nf) face(f, c, g, true);
   for (f += BS; f < nf; f += BS) {
-    c = lrec[f0 + f];
+    c = (f == int(threadIdx.x) + BS) ? c2 : lrec[f0 + f];
     for (int k = 0; k < 7; ++k) g[k] = lgeo[k * L + f0 + f];
     face(f, c, g, false);
   }
