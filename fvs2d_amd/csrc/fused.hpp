@@ -632,13 +632,13 @@ __device__ __forceinline__ double lds_gradient(int q, int id, int N, const doubl
 // as k_grad_lsqc, so the gradients are bit-identical to it.  With cons, W
 // holds conservative states, converted as they are staged.
 constexpr int kGradRegs = 16;
+constexpr int kHaloRounds = 4;  // halo list entries per thread loaded up front
 // BS cells (and threads) per block, with the lists built for blocks of BS
 // cells (without the limiter the kernel fits 72 registers: seven wavefronts
 // per SIMD)
 template <int LIM, int PMODE, int BS = kBlock>
 __global__ __launch_bounds__(BS, LIM ? 1 : 7) void k_grad_lsqh(int N, int nb, int nl, const double *__restrict__ W,
-                                                      int cons, double gamma,
-                                                      const int *__restrict__ ghoff, const int *__restrict__ ghlist,
+                                                      int cons, double gamma, const int *__restrict__ ghpad,
                                                       const int *__restrict__ poff, const int *__restrict__ swid,
                                                       const unsigned *__restrict__ lcode,
                                                       const double2 *__restrict__ cxy,
@@ -660,7 +660,7 @@ __global__ __launch_bounds__(BS, LIM ? 1 : 7) void k_grad_lsqh(int N, int nb, in
   unsigned cr[kGradRegs / 2];
 #pragma unroll
   for (int kk = 0; kk < kGradRegs / 2; ++kk) cr[kk] = (2 * kk < wid) ? lc[kk * kSlice] : 0u;
-  const int o2 = ghoff[lb], nh = ghoff[lb + 1] - o2;
+  const int *hl = ghpad + long(lb) * (nl - BS);  // the block's halo list, -1 after its end
   double G[3] = {0.0, 0.0, 0.0};
   if (act) {
     G[0] = ginv[id];
@@ -673,8 +673,30 @@ __global__ __launch_bounds__(BS, LIM ? 1 : 7) void k_grad_lsqh(int N, int nb, in
     make_prim(cons, gamma, w);
     st4(lW, threadIdx.x, w);
   }
-  for (int q = threadIdx.x; q < nh; q += BS) {
-    const int j = ghlist[o2 + q];
+  // the thread's first kHaloRounds list entries load together (no round
+  // waits on the previous one to learn whether the list goes on); longer
+  // halos continue entry by entry
+  const int maxh = nl - BS;
+  int jh[kHaloRounds];
+#pragma unroll
+  for (int r = 0; r < kHaloRounds; ++r) {
+    const int q = int(threadIdx.x) + r * BS;
+    jh[r] = q < maxh ? hl[q] : -1;
+  }
+#pragma unroll
+  for (int r = 0; r < kHaloRounds; ++r) {
+    const int q = int(threadIdx.x) + r * BS, j = jh[r];
+    if (j < 0) break;
+    const double2 xy = cxy[j];
+    double w[4];
+    ld4(W, j, w);
+    lX[BS + q] = xy;
+    make_prim(cons, gamma, w);
+    st4(lW, BS + q, w);
+  }
+  for (int q = int(threadIdx.x) + kHaloRounds * BS; q < maxh && jh[kHaloRounds - 1] >= 0; q += BS) {
+    const int j = hl[q];
+    if (j < 0) break;
     const double2 xy = cxy[j];
     double w[4];
     ld4(W, j, w);

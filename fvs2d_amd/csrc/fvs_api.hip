@@ -1052,6 +1052,7 @@ struct fvs_ctx {
   // entries per word: sliced ELL with the global stencil's widths, entries
   // 2kk, 2kk+1 of lane l of slice s at lcode[poff[s] + 64 kk + l]
   int *ghoff = nullptr, *ghlist = nullptr, *poff = nullptr;
+  int *ghpad = nullptr, *ghpad_s = nullptr;  // the halo lists padded to maxh per block (-1 after the end)
   unsigned *lcode = nullptr;
   int maxh = 0;  // largest halo of a block
   long nghl = 0, nlcode = 0;
@@ -1567,6 +1568,17 @@ int setup_grad_halo(fvs_ctx &C, const std::vector<int> &off, const std::vector<i
     if (ghlist.empty()) ghlist.push_back(0);
     return 0;
   };
+  // every block's list at a fixed stride of maxh entries (-1 after its end),
+  // so the staged-halo gradient finds its list without loading its offset
+  auto pad = [&](const std::vector<int> &ghoff, const std::vector<int> &ghlist, int maxh, int *&dst) -> int {
+    const long nbk = long(ghoff.size()) - 1;
+    std::vector<int> p(size_t(nbk) * size_t(std::max(maxh, 1)), -1);
+    parallel_for(nbk, 256, [&](long, long b0, long b1) {
+      for (long b = b0; b < b1; ++b)
+        std::copy(ghlist.begin() + ghoff[b], ghlist.begin() + ghoff[b + 1], p.begin() + b * long(maxh));
+    });
+    return C.upload(dst, p);
+  };
   std::vector<int> ghoff, ghlist;
   std::vector<unsigned> lcode;
   int maxh = 0;
@@ -1575,6 +1587,7 @@ int setup_grad_halo(fvs_ctx &C, const std::vector<int> &off, const std::vector<i
   if (!ok) return 0;
   if (int rc = C.upload(C.ghoff, ghoff)) return rc;
   if (int rc = C.upload(C.ghlist, ghlist)) return rc;
+  if (int rc = pad(ghoff, ghlist, maxh, C.ghpad)) return rc;
   if (int rc = C.upload(C.poff, poff)) return rc;
   if (int rc = C.upload(C.lcode, lcode)) return rc;
   C.maxh = maxh;
@@ -1586,6 +1599,7 @@ int setup_grad_halo(fvs_ctx &C, const std::vector<int> &off, const std::vector<i
     if (ok) {
       if (int rc = C.upload(C.ghoff_s, ghoff)) return rc;
       if (int rc = C.upload(C.ghlist_s, ghlist)) return rc;
+      if (int rc = pad(ghoff, ghlist, maxh, C.ghpad_s)) return rc;
       if (int rc = C.upload(C.lcode_s, lcode)) return rc;
       C.maxh_s = maxh;
       C.have_small_grad = true;
@@ -2111,11 +2125,11 @@ int launch_gradient(fvs_ctx &C, bool with_limiter, const double *src = nullptr) 
     if (small) g = dim3(xcd_grid(nb)), b = dim3(kSmallBlock);
     const int nl = small ? kSmallBlock + C.maxh_s : kBlock + C.maxh;
     const size_t lds = size_t(6) * nl * sizeof(double);
-    const int *ghoff = small ? C.ghoff_s : C.ghoff, *ghlist = small ? C.ghlist_s : C.ghlist;
+    const int *ghpad = small ? C.ghpad_s : C.ghpad;
     const unsigned *lcode = small ? C.lcode_s : C.lcode;
 #define FVS_LSQH(L, PM, BS)                                                                                      \
   hipLaunchKernelGGL((k_grad_lsqh<L, PM, BS>), g, b, lds, C.stream, N, xcd_arg(nb, C.xcd_grad), nl,             \
-                     src ? src : C.W, src ? 1 : 0, C.opt.gamma, ghoff, ghlist, C.poff, C.swid, lcode, C.cxy, C.ginv, \
+                     src ? src : C.W, src ? 1 : 0, C.opt.gamma, ghpad, C.poff, C.swid, lcode, C.cxy, C.ginv,        \
                      C.shifts, C.opt.lsq_p, C.grad, C.phi, lt, C.nvrt, C.loff, C.vol, C.nact ? C.blist : nullptr,   \
                      C.frz_some ? C.dinst_off : nullptr, C.dfrz, C.ninst)
 #define FVS_LSQH_PM(L, BS)                     \
