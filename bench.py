@@ -98,11 +98,12 @@ def measured_traffic(n: int, reorder: int, path: int, kernel: str, fused_choice:
     if not os.path.exists(TRAFFIC):
         return None
     with open(TRAFFIC) as f:
-        t = json.load(f)
-    if (t.get("n"), t.get("reorder"), t.get("path"), t.get("kernel_class"), t.get("fused_choice")) != \
-            (n, reorder, path, kernel, fused_choice):
-        return None
-    return t.get("bytes_per_launch")
+        entries = json.load(f)
+    for t in entries if isinstance(entries, list) else [entries]:
+        if (t.get("n"), t.get("reorder"), t.get("path"), t.get("kernel_class"), t.get("fused_choice")) == \
+                (n, reorder, path, kernel, fused_choice):
+            return t.get("bytes_per_launch")
+    return None
 
 
 def mlp_bench(steps: int):
