@@ -16,7 +16,9 @@
  *   - cvar / pvar / resid are (4, ncells) column-major, i.e. C double[N][4]
  *     like the Fortran cvar(4,N); grad is double[2][N][4] like grad(4,N,2);
  *   - every function returns 0 on success and a negative code on error; the
- *     message is then available from fvs_last_error();
+ *     message is then available from fvs_last_error(); no C++ exception
+ *     leaves the library (host memory exhausted during preparation or a
+ *     transfer returns FVS_ENOMEM, any other host failure FVS_ESTATE);
  *   - one context owns one HIP stream and is not re-entrant; calls on one
  *     context must come from one host thread at a time.
  */
