@@ -2538,17 +2538,21 @@ int fvs_create_batch(fvs_ctx **out, int ninst, const int *cell_offsets, const do
   return host_exception("fvs_create_batch");
 }
 
-int fvs_destroy(fvs_ctx *ctx) {
+int fvs_destroy(fvs_ctx *ctx) try {
   delete ctx;
   return 0;
+} catch (...) {
+  return host_exception("fvs_destroy");
 }
 
-int fvs_sizes(const fvs_ctx *C, int *o) {
+int fvs_sizes(const fvs_ctx *C, int *o) try {
   if (!C || !o) return fail(FVS_EINVAL, "null argument");
   const MeshPrep &m = C->mp;
   o[0] = m.ncells; o[1] = m.nedges; o[2] = m.nnodes; o[3] = int(m.edge_intr.size());
   o[4] = m.nedges_bndr; o[5] = int(m.cell_intr.size()); o[6] = m.ncells_bndr; o[7] = m.nb;
   return 0;
+} catch (...) {
+  return host_exception("fvs_sizes");
 }
 
 int fvs_mesh_create(fvs_mesh **out, int nnodes, const double *xy, int ntri, int nquad, const int *cell_nodes,
@@ -2599,23 +2603,27 @@ int fvs_mesh_create_batch(fvs_mesh **out, int ninst, const int *cell_offsets, in
   return host_exception("fvs_mesh_create_batch");
 }
 
-int fvs_mesh_destroy(fvs_mesh *M) {
+int fvs_mesh_destroy(fvs_mesh *M) try {
   delete M;
   return 0;
+} catch (...) {
+  return host_exception("fvs_mesh_destroy");
 }
 
 const fvs_mesh *fvs_ctx_mesh(const fvs_ctx *C) { return C ? &C->mesh : nullptr; }
 
-int fvs_mesh_sizes(const fvs_mesh *M, int *o) {
+int fvs_mesh_sizes(const fvs_mesh *M, int *o) try {
   if (!M || !o) return fail(FVS_EINVAL, "null argument");
   const MeshPrep &m = M->mp;
   o[0] = m.ncells; o[1] = m.nedges; o[2] = m.nnodes; o[3] = int(m.edge_intr.size());
   o[4] = m.nedges_bndr; o[5] = int(m.cell_intr.size()); o[6] = m.ncells_bndr; o[7] = m.nb;
   return 0;
+} catch (...) {
+  return host_exception("fvs_mesh_sizes");
 }
 
 int fvs_mesh_topology(const fvs_mesh *M, int *n1, int *n2, int *c1, int *c2, double *g7, double *xyv3, int *nghbre4,
-                      int *edge4) {
+                      int *edge4) try {
   if (!M) return fail(FVS_EINVAL, "null mesh");
   const MeshPrep &m = M->mp;
   const size_t E = m.nedges, N = m.ncells;
@@ -2635,9 +2643,11 @@ int fvs_mesh_topology(const fvs_mesh *M, int *n1, int *n2, int *c1, int *c2, dou
   if (nghbre4) std::copy(m.nghbre.begin(), m.nghbre.end(), nghbre4);
   if (edge4) std::copy(m.cedge.begin(), m.cedge.end(), edge4);
   return 0;
+} catch (...) {
+  return host_exception("fvs_mesh_topology");
 }
 
-int fvs_mesh_lists(const fvs_mesh *M, int *edge_intr, int *cell_intr, int *bnd_nedges, int *bnd_edges) {
+int fvs_mesh_lists(const fvs_mesh *M, int *edge_intr, int *cell_intr, int *bnd_nedges, int *bnd_edges) try {
   if (!M) return fail(FVS_EINVAL, "null mesh");
   const MeshPrep &m = M->mp;
   if (edge_intr) std::copy(m.edge_intr.begin(), m.edge_intr.end(), edge_intr);
@@ -2645,6 +2655,8 @@ int fvs_mesh_lists(const fvs_mesh *M, int *edge_intr, int *cell_intr, int *bnd_n
   if (bnd_nedges) std::copy(m.bnd_nedges.begin(), m.bnd_nedges.end(), bnd_nedges);
   if (bnd_edges) std::copy(m.bnd_edges.begin(), m.bnd_edges.end(), bnd_edges);
   return 0;
+} catch (...) {
+  return host_exception("fvs_mesh_lists");
 }
 
 int fvs_mesh_cell_order(const fvs_mesh *M, int reorder, int ninst, const int *cell_offsets, int *perm) try {
@@ -2671,7 +2683,7 @@ int fvs_mesh_cell_order(const fvs_mesh *M, int reorder, int ninst, const int *ce
 
 int fvs_mesh_lsq_nnz(const fvs_mesh *M) { return (M && !M->mp.lsq_ptr.empty()) ? M->mp.lsq_ptr.back() : 0; }
 
-int fvs_mesh_lsq(const fvs_mesh *M, int *ptr, int *cell, double *w, double *coef) {
+int fvs_mesh_lsq(const fvs_mesh *M, int *ptr, int *cell, double *w, double *coef) try {
   if (!M) return fail(FVS_EINVAL, "null mesh");
   const MeshPrep &m = M->mp;
   if (m.lsq_ptr.empty()) return fail(FVS_ESTATE, "no LSQ stencil in this configuration");
@@ -2680,6 +2692,8 @@ int fvs_mesh_lsq(const fvs_mesh *M, int *ptr, int *cell, double *w, double *coef
   if (w) std::copy(m.lsq_w.begin(), m.lsq_w.end(), w);
   if (coef) std::copy(m.lsq_coef.begin(), m.lsq_coef.end(), coef);
   return 0;
+} catch (...) {
+  return host_exception("fvs_mesh_lsq");
 }
 
 int fvs_set_state(fvs_ctx *C, const double *cvar) try {
@@ -3200,7 +3214,7 @@ int fvs_time_integration(fvs_ctx *C, double t1, int nsteps, double *norms, doubl
   return host_exception("fvs_time_integration");
 }
 
-int fvs_time_integration_noalloc(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr) {
+int fvs_time_integration_noalloc(fvs_ctx *C, double t1, int nsteps, double *norms, double *verr) try {
   if (int rc = need_state(C)) return rc;
   if (nsteps < 0) return fail(FVS_EINVAL, "nsteps < 0");
   if (nsteps > kNoAllocMaxSteps)
@@ -3226,15 +3240,19 @@ int fvs_time_integration_noalloc(fvs_ctx *C, double t1, int nsteps, double *norm
   HIPC(hipEventElapsedTime(&ms, C->ev0, C->ev1));
   C->last_ms = ms;
   return 0;
+} catch (...) {
+  return host_exception("fvs_time_integration_noalloc");
 }
 
-int fvs_set_phase_timers(fvs_ctx *C, int on) {
+int fvs_set_phase_timers(fvs_ctx *C, int on) try {
   if (!C) return fail(FVS_EINVAL, "null context");
   C->phase_timing = on != 0;
   return 0;
+} catch (...) {
+  return host_exception("fvs_set_phase_timers");
 }
 
-int fvs_phase_times(const fvs_ctx *C, double *sec4) {
+int fvs_phase_times(const fvs_ctx *C, double *sec4) try {
   if (!C || !sec4) return fail(FVS_EINVAL, "null argument");
   const double *t = C->phase_ms;
   sec4[0] = t[KC_GRAD] * 1e-3;                                    // gradient, limiter fused in
@@ -3242,22 +3260,28 @@ int fvs_phase_times(const fvs_ctx *C, double *sec4) {
   sec4[2] = (t[KC_FLUX_INT] + t[KC_FLUX_BND] + t[KC_FUSED]) * 1e-3;  // face fluxes (+ fused stage update)
   sec4[3] = (t[KC_UPDATE] + t[KC_NORM]) * 1e-3;                   // stage updates, residual norms
   return 0;
+} catch (...) {
+  return host_exception("fvs_phase_times");
 }
 
-int fvs_set_persistent(fvs_ctx *C, int on) {
+int fvs_set_persistent(fvs_ctx *C, int on) try {
   if (!C) return fail(FVS_EINVAL, "null context");
   if (on && !C->stage)
     return fail(FVS_ESTATE, "persistent launches run the whole-stage kernel: create the context with kernel_path 2 "
                             "(LSQ gradient, triangle-sized blocks)");
   C->use_persistent = on != 0;
   return 0;
+} catch (...) {
+  return host_exception("fvs_set_persistent");
 }
 
-int fvs_set_graph(fvs_ctx *C, int on) {
+int fvs_set_graph(fvs_ctx *C, int on) try {
   if (!C) return fail(FVS_EINVAL, "null context");
   C->use_graph = on != 0;
   if (!C->use_graph) C->drop_graph();
   return 0;
+} catch (...) {
+  return host_exception("fvs_set_graph");
 }
 
 int fvs_batch_time_integration(fvs_ctx *C, double t1, int nsteps, double *inst_norms) try {
@@ -3403,13 +3427,15 @@ int fvs_wall_cp(fvs_ctx *C, int *nwall, double *out4, double *un3) try {
   return host_exception("fvs_wall_cp");
 }
 
-int fvs_set_timestep(fvs_ctx *C, double dt, double cfl) {
+int fvs_set_timestep(fvs_ctx *C, double dt, double cfl) try {
   if (!C) return fail(FVS_EINVAL, "null context");
   if (!(dt > 0.0) || !(cfl > 0.0)) return fail(FVS_EINVAL, "dt and cfl must be positive");
   C->opt.dt = dt;
   C->opt.cfl = cfl;
   C->launch_gen++;
   return 0;
+} catch (...) {
+  return host_exception("fvs_set_timestep");
 }
 
 int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *bytes, int *launches) try {
@@ -3495,12 +3521,14 @@ int fvs_kernel_times(fvs_ctx *C, double t1, int nsteps, double *ms, double *byte
   return host_exception("fvs_kernel_times");
 }
 
-int fvs_freeze_limiter(fvs_ctx *C, int frozen) {
+int fvs_freeze_limiter(fvs_ctx *C, int frozen) try {
   if (!C) return fail(FVS_EINVAL, "null context");
   C->limiter_frozen = frozen != 0;
   C->frz_some = false;
   C->launch_gen++;
   return 0;
+} catch (...) {
+  return host_exception("fvs_freeze_limiter");
 }
 
 int fvs_batch_freeze_limiter(fvs_ctx *C, const int *frozen) try {

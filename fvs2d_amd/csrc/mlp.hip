@@ -24,6 +24,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -55,6 +58,19 @@ constexpr int kRows = 64;      // GEMM rows per workgroup
 static_assert(kW == 256, "the GEMM tiling assumes width 256");
 
 int fail(int code, const std::string &msg) { return fvs::set_error(code, msg); }
+// an exception thrown by host code inside an entry point ends as an error
+// code and message, never as a C++ exception crossing into the caller
+int host_exception(const char *fn) {
+  try {
+    throw;
+  } catch (const std::bad_alloc &) {
+    return fail(FVS_ENOMEM, std::string(fn) + ": host memory exhausted");
+  } catch (const std::exception &e) {
+    return fail(FVS_ESTATE, std::string(fn) + ": " + e.what());
+  } catch (...) {
+    return fail(FVS_ESTATE, std::string(fn) + ": host exception");
+  }
+}
 
 #define MLPC(x)                                                                           \
   do {                                                                                    \
@@ -885,7 +901,7 @@ int round64(long n) { return int((n + kRows - 1) / kRows * kRows); }
 
 extern "C" {
 
-int fvs_mlp_create(fvs_mlp **out, const fvs_mlp_options *opt) {
+int fvs_mlp_create(fvs_mlp **out, const fvs_mlp_options *opt) try {
   if (!out || !opt) return fail(FVS_EINVAL, "null argument");
   *out = nullptr;
   if (opt->depth < 1 || opt->depth > kMaxDepth) return fail(FVS_EINVAL, "depth must be 1..8");
@@ -895,7 +911,8 @@ int fvs_mlp_create(fvs_mlp **out, const fvs_mlp_options *opt) {
     return fail(FVS_EINVAL, "Adam parameters out of range");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FVS_EHIP, "no HIP device");
-  auto *m = new fvs_mlp();
+  std::unique_ptr<fvs_mlp> holder(new fvs_mlp());  // freed on every early return or exception
+  fvs_mlp *m = holder.get();
   m->opt = *opt;
   m->L = opt->depth;
   m->M = opt->batch;
@@ -912,13 +929,9 @@ int fvs_mlp_create(fvs_mlp **out, const fvs_mlp_options *opt) {
     off += tb.out[l];
   }
   m->P = off;
-  auto bail = [&](int rc) {
-    delete m;
-    return rc;
-  };
-  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(FVS_EHIP, "stream"));
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) return fail(FVS_EHIP, "stream");
   if (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess)
-    return bail(fail(FVS_EHIP, "event"));
+    return fail(FVS_EHIP, "event");
   const size_t M = size_t(m->M);
   int rc = 0;
   rc = rc ? rc : m->alloc(m->w, m->P);
@@ -946,11 +959,11 @@ int fvs_mlp_create(fvs_mlp **out, const fvs_mlp_options *opt) {
     rc = m->alloc(tb.Wt[l], size_t(kW) * tb.inpad[l]);
     if (!rc && l > 0) rc = m->alloc(tb.Wk[l], size_t(kW) * kW);
   }
-  if (rc) return bail(rc);
+  if (rc) return rc;
   tb.Wt[m->L] = tb.Wk[m->L] = nullptr;
   tb.Wk[0] = nullptr;
   if (hipMemsetAsync(tb.Wt[0], 0, size_t(kW) * kInPad * sizeof(u16), m->stream) != hipSuccess)
-    return bail(fail(FVS_EHIP, "memset"));
+    return fail(FVS_EHIP, "memset");
   // Glorot-uniform weights from the seed, zero biases
   std::vector<float> w(m->P, 0.f);
   for (int l = 0; l <= m->L; ++l) {
@@ -961,30 +974,33 @@ int fvs_mlp_create(fvs_mlp **out, const fvs_mlp_options *opt) {
       w[tb.offW[l] + e] = float((2.0 * u - 1.0) * a);
     }
   }
-  *out = m;
   rc = fvs_mlp_set_params(m, w.data());
-  if (rc) {
-    *out = nullptr;
-    return bail(rc);
-  }
+  if (rc) return rc;
+  *out = holder.release();
   return 0;
+} catch (...) {
+  return host_exception("fvs_mlp_create");
 }
 
-int fvs_mlp_destroy(fvs_mlp *m) {
+int fvs_mlp_destroy(fvs_mlp *m) try {
   delete m;
   return 0;
+} catch (...) {
+  return host_exception("fvs_mlp_destroy");
 }
 
 long long fvs_mlp_nparams(const fvs_mlp *m) { return m ? m->P : -1; }
 
-int fvs_mlp_get_params(fvs_mlp *m, float *p) {
+int fvs_mlp_get_params(fvs_mlp *m, float *p) try {
   if (!m || !p) return fail(FVS_EINVAL, "null argument");
   MLPC(hipMemcpyAsync(p, m->w, size_t(m->P) * sizeof(float), hipMemcpyDeviceToHost, m->stream));
   MLPC(hipStreamSynchronize(m->stream));
   return 0;
+} catch (...) {
+  return host_exception("fvs_mlp_get_params");
 }
 
-int fvs_mlp_set_params(fvs_mlp *m, const float *p) {
+int fvs_mlp_set_params(fvs_mlp *m, const float *p) try {
   if (!m || !p) return fail(FVS_EINVAL, "null argument");
   MLPC(hipMemcpyAsync(m->w, p, size_t(m->P) * sizeof(float), hipMemcpyHostToDevice, m->stream));
   MLPC(hipMemsetAsync(m->m1, 0, size_t(m->P) * sizeof(float), m->stream));
@@ -993,9 +1009,11 @@ int fvs_mlp_set_params(fvs_mlp *m, const float *p) {
   if (int rc = launch_adam(*m, 0, 0, 0, 0.0, nullptr, nullptr)) return rc;
   MLPC(hipStreamSynchronize(m->stream));
   return 0;
+} catch (...) {
+  return host_exception("fvs_mlp_set_params");
 }
 
-int fvs_mlp_features(fvs_mlp *m, int n, const float *raw, float *feat) {
+int fvs_mlp_features(fvs_mlp *m, int n, const float *raw, float *feat) try {
   if (!m || (n > 0 && (!raw || !feat))) return fail(FVS_EINVAL, "null argument");
   if (n < 0 || n > m->M) return fail(FVS_EINVAL, "n must be 0..batch");
   if (n == 0) return 0;
@@ -1005,9 +1023,11 @@ int fvs_mlp_features(fvs_mlp *m, int n, const float *raw, float *feat) {
   MLPC(hipMemcpyAsync(feat, m->F32, size_t(n) * kFeat * sizeof(float), hipMemcpyDeviceToHost, m->stream));
   MLPC(hipStreamSynchronize(m->stream));
   return 0;
+} catch (...) {
+  return host_exception("fvs_mlp_features");
 }
 
-int fvs_mlp_set_data(fvs_mlp *m, long long n, const float *raw, const float *target) {
+int fvs_mlp_set_data(fvs_mlp *m, long long n, const float *raw, const float *target) try {
   if (!m || !raw || !target) return fail(FVS_EINVAL, "null argument");
   if (n < 1) return fail(FVS_EINVAL, "empty training set");
   if (m->raw) {
@@ -1024,9 +1044,11 @@ int fvs_mlp_set_data(fvs_mlp *m, long long n, const float *raw, const float *tar
   MLPC(hipStreamSynchronize(m->stream));
   m->ndata = long(n);
   return 0;
+} catch (...) {
+  return host_exception("fvs_mlp_set_data");
 }
 
-int fvs_mlp_train(fvs_mlp *m, int steps, double *loss) {
+int fvs_mlp_train(fvs_mlp *m, int steps, double *loss) try {
   if (!m) return fail(FVS_EINVAL, "null argument");
   if (!m->raw) return fail(FVS_ESTATE, "no training set (fvs_mlp_set_data)");
   if (steps < 0) return fail(FVS_EINVAL, "steps < 0");
@@ -1052,6 +1074,8 @@ int fvs_mlp_train(fvs_mlp *m, int steps, double *loss) {
     rc = fail(FVS_EHIP, "copy of losses");
   (void)hipFree(dl);
   return rc;
+} catch (...) {
+  return host_exception("fvs_mlp_train");
 }
 
 static int grad_or_step(fvs_mlp *m, int n, const float *raw, const float *target, double *loss, float *grad,
@@ -1074,15 +1098,19 @@ static int grad_or_step(fvs_mlp *m, int n, const float *raw, const float *target
   return 0;
 }
 
-int fvs_mlp_grad(fvs_mlp *m, int n, const float *raw, const float *target, double *loss, float *grad) {
+int fvs_mlp_grad(fvs_mlp *m, int n, const float *raw, const float *target, double *loss, float *grad) try {
   return grad_or_step(m, n, raw, target, loss, grad, false);
+} catch (...) {
+  return host_exception("fvs_mlp_grad");
 }
 
-int fvs_mlp_step(fvs_mlp *m, int n, const float *raw, const float *target, double *loss) {
+int fvs_mlp_step(fvs_mlp *m, int n, const float *raw, const float *target, double *loss) try {
   return grad_or_step(m, n, raw, target, loss, nullptr, true);
+} catch (...) {
+  return host_exception("fvs_mlp_step");
 }
 
-int fvs_mlp_predict(fvs_mlp *m, long long n, const float *raw, float *out) {
+int fvs_mlp_predict(fvs_mlp *m, long long n, const float *raw, float *out) try {
   if (!m || (n > 0 && (!raw || !out))) return fail(FVS_EINVAL, "null argument");
   if (n < 0) return fail(FVS_EINVAL, "n < 0");
   for (long long b = 0; b < n; b += m->M) {
@@ -1095,9 +1123,11 @@ int fvs_mlp_predict(fvs_mlp *m, long long n, const float *raw, float *out) {
     MLPC(hipStreamSynchronize(m->stream));
   }
   return 0;
+} catch (...) {
+  return host_exception("fvs_mlp_predict");
 }
 
-int fvs_mlp_kernel_times(fvs_mlp *m, int steps, double *ms6, double *flop6, int *launches6) {
+int fvs_mlp_kernel_times(fvs_mlp *m, int steps, double *ms6, double *flop6, int *launches6) try {
   if (!m || !ms6 || !flop6 || !launches6) return fail(FVS_EINVAL, "null argument");
   if (!m->raw) return fail(FVS_ESTATE, "no training set (fvs_mlp_set_data)");
   if (steps < 1) return fail(FVS_EINVAL, "steps must be positive");
@@ -1139,6 +1169,8 @@ int fvs_mlp_kernel_times(fvs_mlp *m, int steps, double *ms6, double *flop6, int 
     launches6[k] = int(cnt[k] / steps);
   }
   return 0;
+} catch (...) {
+  return host_exception("fvs_mlp_kernel_times");
 }
 
 double fvs_mlp_last_elapsed_ms(const fvs_mlp *m) { return m ? double(m->last_ms) : 0.0; }
