@@ -64,6 +64,7 @@ def test_batch_instances_stay_contiguous():
     cuts fall on global multiples of 64 (a 64-aligned window inside one
     instance is one leaf: ascending ids)."""
     meshes = [meshgen.naca_ogrid(16, 10, thickness=0.1), meshgen.jittered_square(9, seed=2),
+              meshgen.jittered_square(3, seed=1),  # 18 cells: an instance smaller than one leaf
               meshgen.naca_ogrid(24, 12, thickness=0.15), meshgen.jittered_square(13, seed=5)]
     hb = HostMesh.batch(meshes)
     from fvs2d_amd.meshgen import concat
